@@ -1,0 +1,896 @@
+// C ABI of libmlem_hip.so (include/mlem_hip.h): argument checks, HIP error
+// mapping, the resident matrix / reconstruction handles and mlem_run().
+//
+// The iteration is the loop of main.cu:329-449 on one GPU with the matrix and
+// its transpose resident: FP fused with the ratio (calcFwProj_* + calcCorrel),
+// BP fused with the update (calcBwProj_* + calcUpdate) -- the reference's
+// NCCL allreduce bp->f (main.cu:426-433) is a copy on one GPU and disappears,
+// and so do the two memsets (main.cu:436-448) because the kernels overwrite.
+// With using_trans == 1 BP is the atomic scatter over A followed by the
+// update, which also re-zeroes bp for the next iteration.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <array>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "launch.hpp"
+#include "mlem_hip.h"
+
+using namespace mlem;
+
+namespace {
+
+thread_local std::string g_err;
+
+int set_err(int st, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+int set_err(int st, const char *fmt, ...) {
+    char buf[1024];
+    va_list ap;
+    va_start(ap, fmt);
+    vsnprintf(buf, sizeof(buf), fmt, ap);
+    va_end(ap);
+    g_err = buf;
+    return st;
+}
+
+int hip_err(hipError_t e, const char *what, int line) {
+    int st = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? MLEM_ERR_OUT_OF_MEMORY
+             : (e == hipErrorNoDevice || e == hipErrorInvalidDevice) ? MLEM_ERR_NO_DEVICE
+                                                                       : MLEM_ERR_HIP;
+    return set_err(st, "%s failed: %s (abi.hip:%d)", what, hipGetErrorString(e), line);
+}
+
+#define HIPCHK(expr)                                                  \
+    do {                                                              \
+        hipError_t e_ = (expr);                                       \
+        if (e_ != hipSuccess) return hip_err(e_, #expr, __LINE__);    \
+    } while (0)
+
+#define ARGCHK(cond, ...)                                                                \
+    do {                                                                                 \
+        if (!(cond)) return set_err(MLEM_ERR_INVALID_ARGUMENT, __VA_ARGS__);             \
+    } while (0)
+
+inline hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
+
+// Device memory owned by a handle.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        reset();
+        bytes = n;
+        if (n == 0) return hipSuccess;
+        return hipMalloc(&p, n);
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+// Host-side checks of a CSR before it reaches the GPU: a bad row pointer or
+// column index would make the kernels read outside their arrays.
+int check_csr_host(const char *what, uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
+                   const uint32_t *col) {
+    if (rowptr[0] != 0 || rowptr[rows] != nnz)
+        return set_err(MLEM_ERR_INVALID_ARGUMENT, "%s: row pointers must run from 0 to nnz=%llu", what,
+                       (unsigned long long)nnz);
+    for (uint32_t r = 0; r < rows; ++r)
+        if (rowptr[r + 1] < rowptr[r])
+            return set_err(MLEM_ERR_INVALID_ARGUMENT, "%s: row pointers decrease at row %u", what, r);
+    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
+    if (nnz < (1u << 20)) nt = 1;
+    std::vector<uint64_t> bad(nt, UINT64_MAX);
+    std::vector<std::thread> th;
+    for (unsigned t = 0; t < nt; ++t)
+        th.emplace_back([&, t] {
+            uint64_t b = nnz * t / nt, e = nnz * (t + 1) / nt;
+            for (uint64_t k = b; k < e; ++k)
+                if (col[k] >= cols) {
+                    bad[t] = k;
+                    break;
+                }
+        });
+    for (auto &x : th) x.join();
+    for (unsigned t = 0; t < nt; ++t)
+        if (bad[t] != UINT64_MAX)
+            return set_err(MLEM_ERR_INVALID_ARGUMENT, "%s: column index %u at nonzero %llu >= %u", what,
+                           col[bad[t]], (unsigned long long)bad[t], cols);
+    return MLEM_OK;
+}
+
+uint32_t clamp_secsize(uint32_t s) { return std::min(std::max(s, 1u), kMaxSecsize); }
+
+// The SpMV kernels read col/val with 16-byte vector loads.
+bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+}  // namespace
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct Partition {
+    MergePlan plan;
+    DevBuf buf;
+};
+
+struct mlem_matrix {
+    int device = 0;
+    uint32_t rows = 0, cols = 0;
+    uint64_t nnz = 0;
+    DevBuf rowptr, col, val;
+    DevBuf rowptrT, colT, valT;
+    bool has_T = false;
+    std::mutex mu;
+    std::map<std::pair<int, uint32_t>, std::unique_ptr<Partition>> parts;  // (0: A / 1: A^T, secsize)
+
+    uint64_t device_bytes() const {
+        return rowptr.bytes + col.bytes + val.bytes + rowptrT.bytes + colT.bytes + valT.bytes;
+    }
+};
+
+struct IterEvents {
+    hipEvent_t e[3];
+};
+
+struct mlem_recon {
+    const mlem_matrix *m = nullptr;
+    mlem_opts opts{};
+    hipStream_t stream = nullptr;
+    DevBuf g, sens, f, ratio, bp, scratch_fp, scratch_bp, llpart;
+    const Partition *pfp = nullptr, *pbp = nullptr;
+    uint32_t lanes_fp = 64, lanes_bp = 64;
+    std::vector<IterEvents> events;  // pool, one set per profiled iteration
+    size_t events_used = 0;
+    double fp_ms = 0, bp_ms = 0;
+    uint32_t timed = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+namespace {
+
+int get_partition(mlem_matrix *m, int which, uint32_t secsize, const Partition **out) {
+    std::lock_guard<std::mutex> lk(m->mu);
+    auto key = std::make_pair(which, secsize);
+    auto it = m->parts.find(key);
+    if (it != m->parts.end()) {
+        *out = it->second.get();
+        return MLEM_OK;
+    }
+    auto p = std::make_unique<Partition>();
+    const uint32_t nrow = which == 0 ? m->rows : m->cols;
+    p->plan = merge_plan(nrow, m->nnz, secsize);
+    HIPCHK(p->buf.alloc(merge_partition_bytes(p->plan)));
+    const uint64_t *rp = which == 0 ? m->rowptr.as<uint64_t>() : m->rowptrT.as<uint64_t>();
+    HIPCHK(launch_merge_partition(p->plan, rp, p->buf.p, nullptr));
+    HIPCHK(hipStreamSynchronize(nullptr));
+    *out = p.get();
+    m->parts.emplace(key, std::move(p));
+    return MLEM_OK;
+}
+
+// One MLEM iteration on r->stream (main.cu:331-448).
+hipError_t enqueue_iteration(mlem_recon *r, IterEvents *ev) {
+    const mlem_matrix *m = r->m;
+    hipStream_t s = r->stream;
+    hipError_t e;
+    if (ev && (e = hipEventRecord(ev->e[0], s)) != hipSuccess) return e;
+    // FP + ratio
+    EpiArgs fa{};
+    fa.out = r->ratio.as<float>();
+    fa.g = r->g.as<int32_t>();
+    if (r->opts.spmv == 0)
+        e = launch_merge_spmv(r->pfp->plan, r->pfp->buf.p, r->scratch_fp.p, m->rowptr.as<uint64_t>(),
+                              m->col.as<uint32_t>(), m->val.as<float>(), r->f.as<float>(), EPI_RATIO, fa, s);
+    else
+        e = launch_csr_vector(r->lanes_fp, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(),
+                              r->f.as<float>(), m->rows, EPI_RATIO, fa, s);
+    if (e != hipSuccess) return e;
+    if (ev && (e = hipEventRecord(ev->e[1], s)) != hipSuccess) return e;
+    // BP + update
+    EpiArgs ba{};
+    ba.norm = r->sens.as<float>();
+    ba.f = r->f.as<float>();
+    if (r->opts.using_trans == 0) {
+        if (r->opts.spmv == 0)
+            e = launch_merge_spmv(r->pbp->plan, r->pbp->buf.p, r->scratch_bp.p, m->rowptrT.as<uint64_t>(),
+                                  m->colT.as<uint32_t>(), m->valT.as<float>(), r->ratio.as<float>(), EPI_UPDATE, ba,
+                                  s);
+        else
+            e = launch_csr_vector(r->lanes_bp, m->rowptrT.as<uint64_t>(), m->colT.as<uint32_t>(),
+                                  m->valT.as<float>(), r->ratio.as<float>(), m->cols, EPI_UPDATE, ba, s);
+        if (e != hipSuccess) return e;
+    } else {
+        e = launch_scatter(r->lanes_fp, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(),
+                           r->ratio.as<float>(), r->bp.as<float>(), m->rows, s);
+        if (e != hipSuccess) return e;
+        e = launch_update_image(r->f.as<float>(), r->sens.as<float>(), r->bp.as<float>(), m->cols, s);
+        if (e != hipSuccess) return e;
+    }
+    if (ev && (e = hipEventRecord(ev->e[2], s)) != hipSuccess) return e;
+    return hipSuccess;
+}
+
+int collect_times(mlem_recon *r) {
+    for (size_t i = 0; i < r->events_used; ++i) {
+        float a = 0, b = 0;
+        HIPCHK(hipEventSynchronize(r->events[i].e[2]));
+        HIPCHK(hipEventElapsedTime(&a, r->events[i].e[0], r->events[i].e[1]));
+        HIPCHK(hipEventElapsedTime(&b, r->events[i].e[1], r->events[i].e[2]));
+        r->fp_ms += a;
+        r->bp_ms += b;
+        r->timed++;
+    }
+    r->events_used = 0;
+    return MLEM_OK;
+}
+
+int destroy_graph(mlem_recon *r) {
+    if (r->graph_exec) (void)hipGraphExecDestroy(r->graph_exec);
+    if (r->graph) (void)hipGraphDestroy(r->graph);
+    r->graph_exec = nullptr;
+    r->graph = nullptr;
+    return MLEM_OK;
+}
+
+int sens_on_device(const mlem_matrix *m, float *s32, double *s64_or_null, hipStream_t s) {
+    DevBuf tmp;
+    double *s64 = s64_or_null;
+    if (m->has_T) {
+        HIPCHK(launch_colsum_rows_f64(m->rowptrT.as<uint64_t>(), m->valT.as<float>(), m->cols, s64, s32, s));
+    } else {
+        if (!s64) {
+            HIPCHK(tmp.alloc(sizeof(double) * std::max<size_t>(1, m->cols)));
+            s64 = tmp.as<double>();
+        }
+        HIPCHK(launch_fill_f64(s64, 0.0, m->cols, s));
+        HIPCHK(launch_colsum_scatter_f64(m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(), m->rows,
+                                         s64, s));
+        if (s32) HIPCHK(launch_f64_to_f32(s64, s32, m->cols, s));
+    }
+    HIPCHK(hipStreamSynchronize(s));
+    return MLEM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+const char *mlem_last_error(void) { return g_err.c_str(); }
+
+const char *mlem_status_string(int status) {
+    switch (status) {
+        case MLEM_OK: return "ok";
+        case MLEM_ERR_INVALID_ARGUMENT: return "invalid argument";
+        case MLEM_ERR_HIP: return "HIP runtime error";
+        case MLEM_ERR_OUT_OF_MEMORY: return "out of memory";
+        case MLEM_ERR_UNSUPPORTED: return "unsupported";
+        case MLEM_ERR_NO_DEVICE: return "no device";
+        case MLEM_ERR_IO: return "I/O error";
+        case MLEM_ERR_FORMAT: return "format error";
+        default: return "unknown status";
+    }
+}
+
+const char *mlem_version(void) { return "mlem-mi355x 0.1 (gfx950)"; }
+
+// ---------------------------------------------------------------- runtime
+int mlem_device_count(int *count) {
+    ARGCHK(count, "mlem_device_count: null count");
+    *count = 0;
+    hipError_t e = hipGetDeviceCount(count);
+    if (e == hipErrorNoDevice) {
+        *count = 0;
+        return MLEM_OK;
+    }
+    HIPCHK(e);
+    return MLEM_OK;
+}
+
+int mlem_set_device(int device) {
+    HIPCHK(hipSetDevice(device));
+    return MLEM_OK;
+}
+
+int mlem_device_info(int device, char *name, size_t name_len, uint64_t *total_mem, int *cu_count) {
+    hipDeviceProp_t p;
+    HIPCHK(hipGetDeviceProperties(&p, device));
+    if (name && name_len) {
+        std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    }
+    if (total_mem) *total_mem = p.totalGlobalMem;
+    if (cu_count) *cu_count = p.multiProcessorCount;
+    return MLEM_OK;
+}
+
+int mlem_device_alloc(void **ptr, size_t bytes) {
+    ARGCHK(ptr, "mlem_device_alloc: null ptr");
+    *ptr = nullptr;
+    if (bytes == 0) return MLEM_OK;
+    HIPCHK(hipMalloc(ptr, bytes));
+    return MLEM_OK;
+}
+
+int mlem_device_free(void *ptr) {
+    if (ptr) HIPCHK(hipFree(ptr));
+    return MLEM_OK;
+}
+
+int mlem_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return MLEM_OK;
+    ARGCHK(dst && src, "mlem_memcpy_h2d: null pointer");
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyHostToDevice, S(stream)));
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return MLEM_OK;
+    ARGCHK(dst && src, "mlem_memcpy_d2h: null pointer");
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToHost, S(stream)));
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream) {
+    if (!bytes) return MLEM_OK;
+    ARGCHK(dst && src, "mlem_memcpy_d2d: null pointer");
+    HIPCHK(hipMemcpyAsync(dst, src, bytes, hipMemcpyDeviceToDevice, S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_stream_create(void **stream) {
+    ARGCHK(stream, "mlem_stream_create: null stream");
+    hipStream_t s;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    *stream = s;
+    return MLEM_OK;
+}
+
+int mlem_stream_destroy(void *stream) {
+    if (stream) HIPCHK(hipStreamDestroy(S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_stream_sync(void *stream) {
+    HIPCHK(hipStreamSynchronize(S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_device_sync(void) {
+    HIPCHK(hipDeviceSynchronize());
+    return MLEM_OK;
+}
+
+int mlem_event_create(void **event) {
+    ARGCHK(event, "mlem_event_create: null event");
+    hipEvent_t e;
+    HIPCHK(hipEventCreate(&e));
+    *event = e;
+    return MLEM_OK;
+}
+
+int mlem_event_destroy(void *event) {
+    if (event) HIPCHK(hipEventDestroy(static_cast<hipEvent_t>(event)));
+    return MLEM_OK;
+}
+
+int mlem_event_record(void *event, void *stream) {
+    ARGCHK(event, "mlem_event_record: null event");
+    HIPCHK(hipEventRecord(static_cast<hipEvent_t>(event), S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_event_elapsed_ms(void *start, void *stop, float *ms) {
+    ARGCHK(start && stop && ms, "mlem_event_elapsed_ms: null argument");
+    HIPCHK(hipEventSynchronize(static_cast<hipEvent_t>(stop)));
+    HIPCHK(hipEventElapsedTime(ms, static_cast<hipEvent_t>(start), static_cast<hipEvent_t>(stop)));
+    return MLEM_OK;
+}
+
+int mlem_fill_f32(float *dst, float value, uint64_t n, void *stream) {
+    ARGCHK(dst || n == 0, "mlem_fill_f32: null dst");
+    HIPCHK(launch_fill(dst, value, n, S(stream)));
+    return MLEM_OK;
+}
+
+// ---------------------------------------------------------------- kernels
+int mlem_ratio(const int32_t *g, float *fwproj, uint32_t rows, void *stream) {
+    ARGCHK(rows == 0 || (g && fwproj), "mlem_ratio: null pointer");
+    HIPCHK(launch_ratio(g, fwproj, rows, S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_update(const float *f, const float *norm, float *bwproj, uint32_t cols, void *stream) {
+    ARGCHK(cols == 0 || (f && norm && bwproj), "mlem_update: null pointer");
+    HIPCHK(launch_update(f, norm, bwproj, cols, S(stream)));
+    return MLEM_OK;
+}
+
+size_t mlem_merge_workspace_bytes(uint32_t rows, uint64_t nnz, uint32_t secsize) {
+    if (secsize < 1 || secsize > kMaxSecsize) return 0;
+    MergePlan p = merge_plan(rows, nnz, secsize);
+    return merge_partition_bytes(p) + merge_scratch_bytes(p);
+}
+
+static int merge_call(const char *what, const uint64_t *rowptr, const uint32_t *col, const float *val,
+                      const float *x, uint32_t secsize, uint32_t rows, uint64_t nnz, void *ws, size_t ws_bytes,
+                      Epi epi, const EpiArgs &ea, void *stream) {
+    ARGCHK(secsize >= 1 && secsize <= kMaxSecsize, "%s: secsize %u outside 1..%u", what, secsize, kMaxSecsize);
+    if (rows == 0) return MLEM_OK;
+    ARGCHK(rowptr && x && (nnz == 0 || (col && val)), "%s: null pointer", what);
+    ARGCHK(aligned16(col) && aligned16(val), "%s: col and val must be 16-byte aligned", what);
+    MergePlan p = merge_plan(rows, nnz, secsize);
+    const size_t need = merge_partition_bytes(p) + merge_scratch_bytes(p);
+    ARGCHK(ws && ws_bytes >= need, "%s: workspace of %zu bytes needed (mlem_merge_workspace_bytes)", what, need);
+    HIPCHK(launch_merge_partition(p, rowptr, ws, S(stream)));
+    HIPCHK(launch_merge_spmv(p, ws, static_cast<char *>(ws) + merge_partition_bytes(p), rowptr, col, val, x, epi, ea,
+                             S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_fp_merge(const uint64_t *rowptr, const uint32_t *col, const float *val, const float *f, float *fwproj,
+                  uint32_t secsize, uint32_t rows, uint64_t nnz, void *workspace, size_t workspace_bytes,
+                  void *stream) {
+    EpiArgs ea{};
+    ea.out = fwproj;
+    ARGCHK(rows == 0 || fwproj, "mlem_fp_merge: null output");
+    return merge_call("mlem_fp_merge", rowptr, col, val, f, secsize, rows, nnz, workspace, workspace_bytes, EPI_PLAIN,
+                      ea, stream);
+}
+
+int mlem_bp_merge(const uint64_t *rowptrT, const uint32_t *colT, const float *valT, const float *correl,
+                  float *bwproj, uint32_t secsize, uint32_t cols, uint64_t nnz, void *workspace,
+                  size_t workspace_bytes, void *stream) {
+    EpiArgs ea{};
+    ea.out = bwproj;
+    ARGCHK(cols == 0 || bwproj, "mlem_bp_merge: null output");
+    return merge_call("mlem_bp_merge", rowptrT, colT, valT, correl, secsize, cols, nnz, workspace, workspace_bytes,
+                      EPI_PLAIN, ea, stream);
+}
+
+int mlem_fp_ratio_merge(const uint64_t *rowptr, const uint32_t *col, const float *val, const float *f,
+                        const int32_t *g, float *ratio, uint32_t secsize, uint32_t rows, uint64_t nnz,
+                        void *workspace, size_t workspace_bytes, void *stream) {
+    EpiArgs ea{};
+    ea.out = ratio;
+    ea.g = g;
+    ARGCHK(rows == 0 || (ratio && g), "mlem_fp_ratio_merge: null pointer");
+    return merge_call("mlem_fp_ratio_merge", rowptr, col, val, f, secsize, rows, nnz, workspace, workspace_bytes,
+                      EPI_RATIO, ea, stream);
+}
+
+int mlem_bp_update_merge(const uint64_t *rowptrT, const uint32_t *colT, const float *valT, const float *ratio,
+                         const float *norm, float *f, uint32_t secsize, uint32_t cols, uint64_t nnz,
+                         void *workspace, size_t workspace_bytes, void *stream) {
+    EpiArgs ea{};
+    ea.norm = norm;
+    ea.f = f;
+    ARGCHK(cols == 0 || (norm && f), "mlem_bp_update_merge: null pointer");
+    return merge_call("mlem_bp_update_merge", rowptrT, colT, valT, ratio, secsize, cols, nnz, workspace,
+                      workspace_bytes, EPI_UPDATE, ea, stream);
+}
+
+static int vector_call(const char *what, const uint64_t *rowptr, const uint32_t *col, const float *val,
+                       const float *x, uint32_t rows, uint64_t nnz, uint32_t lanes, Epi epi, const EpiArgs &ea,
+                       void *stream) {
+    if (rows == 0) return MLEM_OK;
+    ARGCHK(rowptr && x && (nnz == 0 || (col && val)), "%s: null pointer", what);
+    ARGCHK(aligned16(col) && aligned16(val), "%s: col and val must be 16-byte aligned", what);
+    if (lanes == 0) lanes = auto_lanes(rows, nnz);
+    ARGCHK(valid_lanes(lanes), "%s: lanes must be 4, 8, 16, 32 or 64 (got %u)", what, lanes);
+    HIPCHK(launch_csr_vector(lanes, rowptr, col, val, x, rows, epi, ea, S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_fp_csr_vector(const uint64_t *rowptr, const uint32_t *col, const float *val, const float *f, float *fwproj,
+                       uint32_t rows, uint64_t nnz, uint32_t lanes, void *stream) {
+    EpiArgs ea{};
+    ea.out = fwproj;
+    ARGCHK(rows == 0 || fwproj, "mlem_fp_csr_vector: null output");
+    return vector_call("mlem_fp_csr_vector", rowptr, col, val, f, rows, nnz, lanes, EPI_PLAIN, ea, stream);
+}
+
+int mlem_bp_csr_vector(const uint64_t *rowptrT, const uint32_t *colT, const float *valT, const float *correl,
+                       float *bwproj, uint32_t cols, uint64_t nnz, uint32_t lanes, void *stream) {
+    EpiArgs ea{};
+    ea.out = bwproj;
+    ARGCHK(cols == 0 || bwproj, "mlem_bp_csr_vector: null output");
+    return vector_call("mlem_bp_csr_vector", rowptrT, colT, valT, correl, cols, nnz, lanes, EPI_PLAIN, ea, stream);
+}
+
+int mlem_fp_ratio_csr_vector(const uint64_t *rowptr, const uint32_t *col, const float *val, const float *f,
+                             const int32_t *g, float *ratio, uint32_t rows, uint64_t nnz, uint32_t lanes,
+                             void *stream) {
+    EpiArgs ea{};
+    ea.out = ratio;
+    ea.g = g;
+    ARGCHK(rows == 0 || (ratio && g), "mlem_fp_ratio_csr_vector: null pointer");
+    return vector_call("mlem_fp_ratio_csr_vector", rowptr, col, val, f, rows, nnz, lanes, EPI_RATIO, ea, stream);
+}
+
+int mlem_bp_update_csr_vector(const uint64_t *rowptrT, const uint32_t *colT, const float *valT, const float *ratio,
+                              const float *norm, float *f, uint32_t cols, uint64_t nnz, uint32_t lanes,
+                              void *stream) {
+    EpiArgs ea{};
+    ea.norm = norm;
+    ea.f = f;
+    ARGCHK(cols == 0 || (norm && f), "mlem_bp_update_csr_vector: null pointer");
+    return vector_call("mlem_bp_update_csr_vector", rowptrT, colT, valT, ratio, cols, nnz, lanes, EPI_UPDATE, ea,
+                       stream);
+}
+
+int mlem_bp_scatter(const uint64_t *rowptr, const uint32_t *col, const float *val, const float *correl,
+                    float *bwproj, uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t lanes, void *stream) {
+    ARGCHK(cols == 0 || bwproj, "mlem_bp_scatter: null output");
+    HIPCHK(launch_fill(bwproj, 0.0f, cols, S(stream)));
+    if (rows == 0) return MLEM_OK;
+    ARGCHK(rowptr && correl && (nnz == 0 || (col && val)), "mlem_bp_scatter: null pointer");
+    if (lanes == 0) lanes = auto_lanes(rows, nnz);
+    ARGCHK(valid_lanes(lanes), "mlem_bp_scatter: lanes must be 4, 8, 16, 32 or 64 (got %u)", lanes);
+    HIPCHK(launch_scatter(lanes, rowptr, col, val, correl, bwproj, rows, S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_colsum_f64(const uint64_t *rowptrT, const float *valT, uint32_t cols, uint64_t nnz, double *sens64,
+                    float *sens32, void *stream) {
+    if (cols == 0) return MLEM_OK;
+    ARGCHK(rowptrT && (nnz == 0 || valT), "mlem_colsum_f64: null pointer");
+    HIPCHK(launch_colsum_rows_f64(rowptrT, valT, cols, sens64, sens32, S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_colsum_scatter_f64(const uint64_t *rowptr, const uint32_t *col, const float *val, uint32_t rows,
+                            uint32_t cols, uint64_t nnz, double *sens64, float *sens32, void *stream) {
+    if (cols == 0) return MLEM_OK;
+    ARGCHK(rowptr && (nnz == 0 || (col && val)), "mlem_colsum_scatter_f64: null pointer");
+    DevBuf tmp;
+    double *s64 = sens64;
+    if (!s64) {
+        HIPCHK(tmp.alloc(sizeof(double) * cols));
+        s64 = tmp.as<double>();
+    }
+    HIPCHK(launch_fill_f64(s64, 0.0, cols, S(stream)));
+    HIPCHK(launch_colsum_scatter_f64(rowptr, col, val, rows, s64, S(stream)));
+    if (sens32) HIPCHK(launch_f64_to_f32(s64, sens32, cols, S(stream)));
+    if (!sens64) HIPCHK(hipStreamSynchronize(S(stream)));  // before tmp is freed
+    return MLEM_OK;
+}
+
+// ---------------------------------------------------------------- matrix
+void mlem_opts_default(mlem_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->spmv = 0;
+    o->using_trans = 0;
+    o->secsize_fp = 8;
+    o->secsize_bp = 8;
+}
+
+int mlem_matrix_create(int device, uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
+                       const uint32_t *col, const float *val, const uint64_t *rowptrT, const uint32_t *colT,
+                       const float *valT, mlem_matrix **out) {
+    ARGCHK(out, "mlem_matrix_create: null out");
+    *out = nullptr;
+    ARGCHK(rowptr && (nnz == 0 || (col && val)), "mlem_matrix_create: null matrix array");
+    ARGCHK(cols > 0, "mlem_matrix_create: cols must be > 0");
+    if (int st = check_csr_host("mlem_matrix_create (A)", rows, cols, nnz, rowptr, col)) return st;
+    if (rowptrT) {
+        ARGCHK(nnz == 0 || (colT && valT), "mlem_matrix_create: transpose arrays incomplete");
+        if (int st = check_csr_host("mlem_matrix_create (A^T)", cols, rows, nnz, rowptrT, colT)) return st;
+    }
+    HIPCHK(hipSetDevice(device));
+    auto m = std::make_unique<mlem_matrix>();
+    m->device = device;
+    m->rows = rows;
+    m->cols = cols;
+    m->nnz = nnz;
+    HIPCHK(m->rowptr.alloc(sizeof(uint64_t) * ((size_t)rows + 1)));
+    HIPCHK(m->col.alloc(sizeof(uint32_t) * nnz));
+    HIPCHK(m->val.alloc(sizeof(float) * nnz));
+    HIPCHK(hipMemcpy(m->rowptr.p, rowptr, m->rowptr.bytes, hipMemcpyHostToDevice));
+    if (nnz) {
+        HIPCHK(hipMemcpy(m->col.p, col, m->col.bytes, hipMemcpyHostToDevice));
+        HIPCHK(hipMemcpy(m->val.p, val, m->val.bytes, hipMemcpyHostToDevice));
+    }
+    if (rowptrT) {
+        HIPCHK(m->rowptrT.alloc(sizeof(uint64_t) * ((size_t)cols + 1)));
+        HIPCHK(m->colT.alloc(sizeof(uint32_t) * nnz));
+        HIPCHK(m->valT.alloc(sizeof(float) * nnz));
+        HIPCHK(hipMemcpy(m->rowptrT.p, rowptrT, m->rowptrT.bytes, hipMemcpyHostToDevice));
+        if (nnz) {
+            HIPCHK(hipMemcpy(m->colT.p, colT, m->colT.bytes, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(m->valT.p, valT, m->valT.bytes, hipMemcpyHostToDevice));
+        }
+        m->has_T = true;
+    }
+    *out = m.release();
+    return MLEM_OK;
+}
+
+int mlem_matrix_build_transpose(mlem_matrix *m) {
+    ARGCHK(m, "mlem_matrix_build_transpose: null matrix");
+    HIPCHK(hipSetDevice(m->device));
+    DevBuf scratch, rp, ci, va;
+    HIPCHK(rp.alloc(sizeof(uint64_t) * ((size_t)m->cols + 1)));
+    HIPCHK(ci.alloc(sizeof(uint32_t) * m->nnz));
+    HIPCHK(va.alloc(sizeof(float) * m->nnz));
+    HIPCHK(scratch.alloc(transpose_scratch_bytes(m->rows, m->cols, m->nnz)));
+    HIPCHK(launch_transpose(m->rows, m->cols, m->nnz, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(),
+                            m->val.as<float>(), rp.as<uint64_t>(), ci.as<uint32_t>(), va.as<float>(), scratch.p,
+                            nullptr));
+    HIPCHK(hipStreamSynchronize(nullptr));
+    std::lock_guard<std::mutex> lk(m->mu);
+    std::swap(m->rowptrT.p, rp.p);
+    std::swap(m->rowptrT.bytes, rp.bytes);
+    std::swap(m->colT.p, ci.p);
+    std::swap(m->colT.bytes, ci.bytes);
+    std::swap(m->valT.p, va.p);
+    std::swap(m->valT.bytes, va.bytes);
+    m->has_T = true;
+    for (auto it = m->parts.begin(); it != m->parts.end();)
+        it = it->first.first == 1 ? m->parts.erase(it) : std::next(it);
+    return MLEM_OK;
+}
+
+int mlem_matrix_destroy(mlem_matrix *m) {
+    if (!m) return MLEM_OK;
+    (void)hipSetDevice(m->device);
+    delete m;
+    return MLEM_OK;
+}
+
+int mlem_matrix_info(const mlem_matrix *m, uint32_t *rows, uint32_t *cols, uint64_t *nnz, int *has_transpose,
+                     uint64_t *device_bytes) {
+    ARGCHK(m, "mlem_matrix_info: null matrix");
+    if (rows) *rows = m->rows;
+    if (cols) *cols = m->cols;
+    if (nnz) *nnz = m->nnz;
+    if (has_transpose) *has_transpose = m->has_T ? 1 : 0;
+    if (device_bytes) *device_bytes = m->device_bytes();
+    return MLEM_OK;
+}
+
+int mlem_matrix_device_arrays(const mlem_matrix *m, const uint64_t **rowptr, const uint32_t **col, const float **val,
+                              const uint64_t **rowptrT, const uint32_t **colT, const float **valT) {
+    ARGCHK(m, "mlem_matrix_device_arrays: null matrix");
+    if (rowptr) *rowptr = m->rowptr.as<uint64_t>();
+    if (col) *col = m->col.as<uint32_t>();
+    if (val) *val = m->val.as<float>();
+    if (rowptrT) *rowptrT = m->rowptrT.as<uint64_t>();
+    if (colT) *colT = m->colT.as<uint32_t>();
+    if (valT) *valT = m->valT.as<float>();
+    return MLEM_OK;
+}
+
+int mlem_matrix_download_transpose(const mlem_matrix *m, uint64_t *colptrT, uint32_t *rowT, float *valT) {
+    ARGCHK(m, "mlem_matrix_download_transpose: null matrix");
+    ARGCHK(m->has_T, "mlem_matrix_download_transpose: matrix has no transpose");
+    HIPCHK(hipSetDevice(m->device));
+    if (colptrT) HIPCHK(hipMemcpy(colptrT, m->rowptrT.p, m->rowptrT.bytes, hipMemcpyDeviceToHost));
+    if (rowT && m->nnz) HIPCHK(hipMemcpy(rowT, m->colT.p, m->colT.bytes, hipMemcpyDeviceToHost));
+    if (valT && m->nnz) HIPCHK(hipMemcpy(valT, m->valT.p, m->valT.bytes, hipMemcpyDeviceToHost));
+    return MLEM_OK;
+}
+
+int mlem_matrix_sensitivity(const mlem_matrix *m, float *sens32, double *sens64) {
+    ARGCHK(m, "mlem_matrix_sensitivity: null matrix");
+    HIPCHK(hipSetDevice(m->device));
+    DevBuf d32, d64;
+    HIPCHK(d32.alloc(sizeof(float) * m->cols));
+    HIPCHK(d64.alloc(sizeof(double) * m->cols));
+    if (int st = sens_on_device(m, d32.as<float>(), d64.as<double>(), nullptr)) return st;
+    if (sens32) HIPCHK(hipMemcpy(sens32, d32.p, d32.bytes, hipMemcpyDeviceToHost));
+    if (sens64) HIPCHK(hipMemcpy(sens64, d64.p, d64.bytes, hipMemcpyDeviceToHost));
+    return MLEM_OK;
+}
+
+// ---------------------------------------------------------------- recon
+int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens, const float *f0,
+                      const mlem_opts *opts, mlem_recon **out) {
+    ARGCHK(out, "mlem_recon_create: null out");
+    *out = nullptr;
+    ARGCHK(mc && g && f0, "mlem_recon_create: null matrix, counts or image");
+    mlem_matrix *m = const_cast<mlem_matrix *>(mc);  // only the partition cache changes
+    mlem_opts o;
+    if (opts)
+        o = *opts;
+    else
+        mlem_opts_default(&o);
+    ARGCHK(o.spmv <= 1, "mlem_recon_create: spmv must be 0 (merge-based) or 1 (CSR-vector)");
+    ARGCHK(o.using_trans <= 1, "mlem_recon_create: using_trans must be 0 (transpose) or 1 (scatter)");
+    ARGCHK(o.secsize_fp >= 1 && o.secsize_bp >= 1, "mlem_recon_create: section sizes must be >= 1");
+    ARGCHK(o.fp_lanes == 0 || valid_lanes(o.fp_lanes), "mlem_recon_create: fp_lanes must be 0,4,8,16,32,64");
+    ARGCHK(o.bp_lanes == 0 || valid_lanes(o.bp_lanes), "mlem_recon_create: bp_lanes must be 0,4,8,16,32,64");
+    if (o.using_trans == 0 && !m->has_T)
+        return set_err(MLEM_ERR_INVALID_ARGUMENT,
+                       "mlem_recon_create: using_trans == 0 needs the transpose (upload it or build it)");
+    HIPCHK(hipSetDevice(m->device));
+    auto r = std::make_unique<mlem_recon>();
+    r->m = m;
+    r->opts = o;
+    r->opts.secsize_fp = clamp_secsize(o.secsize_fp);
+    r->opts.secsize_bp = clamp_secsize(o.secsize_bp);
+    r->lanes_fp = o.fp_lanes ? o.fp_lanes : auto_lanes(m->rows, m->nnz);
+    r->lanes_bp = o.bp_lanes ? o.bp_lanes : auto_lanes(m->cols, m->nnz);
+    HIPCHK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
+    struct StreamGuard {
+        mlem_recon *r;
+        bool armed = true;
+        ~StreamGuard() {
+            if (armed && r->stream) (void)hipStreamDestroy(r->stream);
+        }
+    } guard{r.get()};
+    HIPCHK(r->g.alloc(sizeof(int32_t) * std::max<size_t>(1, m->rows)));
+    HIPCHK(r->ratio.alloc(sizeof(float) * std::max<size_t>(1, m->rows)));
+    HIPCHK(r->f.alloc(sizeof(float) * m->cols));
+    HIPCHK(r->sens.alloc(sizeof(float) * m->cols));
+    if (m->rows) HIPCHK(hipMemcpyAsync(r->g.p, g, sizeof(int32_t) * m->rows, hipMemcpyHostToDevice, r->stream));
+    HIPCHK(hipMemcpyAsync(r->f.p, f0, sizeof(float) * m->cols, hipMemcpyHostToDevice, r->stream));
+    if (sens) {
+        HIPCHK(hipMemcpyAsync(r->sens.p, sens, sizeof(float) * m->cols, hipMemcpyHostToDevice, r->stream));
+    } else {
+        if (int st = sens_on_device(m, r->sens.as<float>(), nullptr, r->stream)) return st;
+    }
+    if (o.using_trans == 1) {
+        HIPCHK(r->bp.alloc(sizeof(float) * m->cols));
+        HIPCHK(launch_fill(r->bp.as<float>(), 0.0f, m->cols, r->stream));
+    }
+    if (o.spmv == 0) {
+        if (int st = get_partition(m, 0, r->opts.secsize_fp, &r->pfp)) return st;
+        HIPCHK(r->scratch_fp.alloc(merge_scratch_bytes(r->pfp->plan)));
+        if (o.using_trans == 0) {
+            if (int st = get_partition(m, 1, r->opts.secsize_bp, &r->pbp)) return st;
+            HIPCHK(r->scratch_bp.alloc(merge_scratch_bytes(r->pbp->plan)));
+        }
+    }
+    HIPCHK(r->llpart.alloc(sizeof(double) * std::max<uint32_t>(1, loglik_blocks(m->rows))));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    guard.armed = false;
+    *out = r.release();
+    return MLEM_OK;
+}
+
+int mlem_recon_iterate(mlem_recon *r, uint32_t iters) {
+    ARGCHK(r, "mlem_recon_iterate: null handle");
+    HIPCHK(hipSetDevice(r->m->device));
+    if (r->opts.use_graph) {
+        if (!r->graph_exec) {
+            HIPCHK(hipStreamBeginCapture(r->stream, hipStreamCaptureModeThreadLocal));
+            hipError_t e = enqueue_iteration(r, nullptr);
+            hipGraph_t gr = nullptr;
+            hipError_t e2 = hipStreamEndCapture(r->stream, &gr);
+            if (e != hipSuccess) {
+                if (gr) (void)hipGraphDestroy(gr);
+                return hip_err(e, "enqueue_iteration (capture)", __LINE__);
+            }
+            HIPCHK(e2);
+            r->graph = gr;
+            HIPCHK(hipGraphInstantiate(&r->graph_exec, r->graph, nullptr, nullptr, 0));
+        }
+        for (uint32_t it = 0; it < iters; ++it) HIPCHK(hipGraphLaunch(r->graph_exec, r->stream));
+        return MLEM_OK;
+    }
+    for (uint32_t it = 0; it < iters; ++it) {
+        IterEvents *ev = nullptr;
+        if (r->opts.profile) {
+            if (r->events_used == r->events.size()) {
+                IterEvents x;
+                for (auto &e : x.e) HIPCHK(hipEventCreate(&e));
+                r->events.push_back(x);
+            }
+            ev = &r->events[r->events_used++];
+        }
+        hipError_t e = enqueue_iteration(r, ev);
+        if (e != hipSuccess) return hip_err(e, "enqueue_iteration", __LINE__);
+    }
+    return MLEM_OK;
+}
+
+int mlem_recon_sync(mlem_recon *r) {
+    ARGCHK(r, "mlem_recon_sync: null handle");
+    HIPCHK(hipSetDevice(r->m->device));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return collect_times(r);
+}
+
+int mlem_recon_get_image(mlem_recon *r, float *f_out) {
+    ARGCHK(r && f_out, "mlem_recon_get_image: null argument");
+    HIPCHK(hipSetDevice(r->m->device));
+    HIPCHK(hipMemcpyAsync(f_out, r->f.p, sizeof(float) * r->m->cols, hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return collect_times(r);
+}
+
+int mlem_recon_set_image(mlem_recon *r, const float *f) {
+    ARGCHK(r && f, "mlem_recon_set_image: null argument");
+    HIPCHK(hipSetDevice(r->m->device));
+    HIPCHK(hipMemcpyAsync(r->f.p, f, sizeof(float) * r->m->cols, hipMemcpyHostToDevice, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    return MLEM_OK;
+}
+
+int mlem_recon_kernel_times(mlem_recon *r, double *fp_ms, double *bp_ms, uint32_t *iters_timed) {
+    ARGCHK(r, "mlem_recon_kernel_times: null handle");
+    if (int st = collect_times(r)) return st;
+    if (fp_ms) *fp_ms = r->fp_ms;
+    if (bp_ms) *bp_ms = r->bp_ms;
+    if (iters_timed) *iters_timed = r->timed;
+    return MLEM_OK;
+}
+
+int mlem_recon_reset_times(mlem_recon *r) {
+    ARGCHK(r, "mlem_recon_reset_times: null handle");
+    if (int st = collect_times(r)) return st;
+    r->fp_ms = r->bp_ms = 0;
+    r->timed = 0;
+    return MLEM_OK;
+}
+
+int mlem_recon_loglik(mlem_recon *r, double *ll) {
+    ARGCHK(r && ll, "mlem_recon_loglik: null argument");
+    HIPCHK(hipSetDevice(r->m->device));
+    const mlem_matrix *m = r->m;
+    const uint32_t nb = loglik_blocks(m->rows);
+    HIPCHK(launch_loglik(m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(), r->f.as<float>(),
+                         r->g.as<int32_t>(), m->rows, r->llpart.as<double>(), r->stream));
+    std::vector<double> part(nb);
+    if (nb) HIPCHK(hipMemcpyAsync(part.data(), r->llpart.p, sizeof(double) * nb, hipMemcpyDeviceToHost, r->stream));
+    HIPCHK(hipStreamSynchronize(r->stream));
+    double s = 0.0;
+    for (double v : part) s += v;
+    *ll = s;
+    return MLEM_OK;
+}
+
+void *mlem_recon_stream(mlem_recon *r) { return r ? r->stream : nullptr; }
+
+int mlem_recon_destroy(mlem_recon *r) {
+    if (!r) return MLEM_OK;
+    (void)hipSetDevice(r->m->device);
+    if (r->stream) (void)hipStreamSynchronize(r->stream);
+    destroy_graph(r);
+    for (auto &x : r->events)
+        for (auto &e : x.e) (void)hipEventDestroy(e);
+    if (r->stream) (void)hipStreamDestroy(r->stream);
+    delete r;
+    return MLEM_OK;
+}
+
+int mlem_run(const mlem_matrix *m, const int32_t *g, const float *sens, const float *f0, float *f_out, uint32_t iters,
+             const mlem_opts *opts) {
+    ARGCHK(m && g && f0 && f_out, "mlem_run: null argument");
+    mlem_recon *r = nullptr;
+    if (int st = mlem_recon_create(m, g, sens, f0, opts, &r)) return st;
+    int st = mlem_recon_iterate(r, iters);
+    if (st == MLEM_OK) st = mlem_recon_get_image(r, f_out);
+    if (st != MLEM_OK) {
+        std::string msg = g_err;
+        mlem_recon_destroy(r);
+        g_err = msg;
+        return st;
+    }
+    return mlem_recon_destroy(r);
+}
+
+}  // extern "C"

@@ -1,0 +1,80 @@
+// Internal launchers shared by the C ABI (abi.hip) and the kernels
+// (spmv.hip, transpose.hip).  All launchers are asynchronous on `s` and
+// return hipSuccess or the launch error.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <cstddef>
+#include <cstdint>
+
+namespace mlem {
+
+constexpr int kWG = 256;              // work-items per workgroup (4 wave64)
+constexpr uint32_t kMaxSecsize = 32;  // merge-path items per work-item
+
+// What a finished row sum y_r turns into.
+enum Epi : int {
+    EPI_PLAIN = 0,   // out[r] = y_r
+    EPI_RATIO = 1,   // out[r] = y_r != 0 ? (float)g[r] / y_r : 0      (calcCorrel)
+    EPI_UPDATE = 2,  // f[r] = norm[r] == 0 ? f[r]*y_r : f[r]*y_r/norm[r] (calcUpdate)
+};
+
+struct EpiArgs {
+    float *out;         // PLAIN / RATIO output
+    const int32_t *g;   // RATIO counts
+    const float *norm;  // UPDATE sensitivity
+    float *f;           // UPDATE image, in place
+};
+
+// Tile partition of the merge path for one matrix and one secsize.
+struct MergePlan {
+    uint32_t rows = 0;
+    uint64_t nnz = 0;
+    uint32_t secsize = 0;
+    uint64_t tile_items = 0;  // kWG * secsize
+    uint32_t ntiles = 0;
+};
+
+MergePlan merge_plan(uint32_t rows, uint64_t nnz, uint32_t secsize);
+// Workspace split: tile_row u32[ntiles+1], tile_nz u64[ntiles+1] (the
+// partition, matrix-dependent only) and carry/head f32[ntiles] (per call).
+size_t merge_partition_bytes(const MergePlan &p);
+size_t merge_scratch_bytes(const MergePlan &p);
+
+hipError_t launch_merge_partition(const MergePlan &p, const uint64_t *rowptr, void *partition, hipStream_t s);
+hipError_t launch_merge_spmv(const MergePlan &p, const void *partition, void *scratch, const uint64_t *rowptr,
+                             const uint32_t *col, const float *val, const float *x, Epi epi, const EpiArgs &ea,
+                             hipStream_t s);
+
+uint32_t auto_lanes(uint32_t rows, uint64_t nnz);
+bool valid_lanes(uint32_t lanes);
+hipError_t launch_csr_vector(uint32_t lanes, const uint64_t *rowptr, const uint32_t *col, const float *val,
+                             const float *x, uint32_t rows, Epi epi, const EpiArgs &ea, hipStream_t s);
+hipError_t launch_scatter(uint32_t lanes, const uint64_t *rowptr, const uint32_t *col, const float *val,
+                          const float *xrow, float *out, uint32_t rows, hipStream_t s);
+hipError_t launch_ratio(const int32_t *g, float *fwproj, uint32_t rows, hipStream_t s);
+// In place over bwproj: bwproj = update(f, bwproj, norm)  (calcUpdate)
+hipError_t launch_update(const float *f, const float *norm, float *bwproj, uint32_t cols, hipStream_t s);
+// f = update(f, bp, norm) and bp = 0 (the scatter path's update + re-zero)
+hipError_t launch_update_image(float *f, const float *norm, float *bp, uint32_t cols, hipStream_t s);
+hipError_t launch_fill(float *dst, float v, uint64_t n, hipStream_t s);
+hipError_t launch_fill_f64(double *dst, double v, uint64_t n, hipStream_t s);
+hipError_t launch_colsum_rows_f64(const uint64_t *rowptrT, const float *valT, uint32_t cols, double *s64, float *s32,
+                                  hipStream_t s);
+hipError_t launch_colsum_scatter_f64(const uint64_t *rowptr, const uint32_t *col, const float *val, uint32_t rows,
+                                     double *s64, hipStream_t s);
+hipError_t launch_f64_to_f32(const double *a, float *b, uint32_t n, hipStream_t s);
+// Poisson log-likelihood partial sums over rows of A x: per-block FP64
+// partials into part[grid], grid = loglik_blocks(rows).
+uint32_t loglik_blocks(uint32_t rows);
+hipError_t launch_loglik(const uint64_t *rowptr, const uint32_t *col, const float *val, const float *x,
+                         const int32_t *g, uint32_t rows, double *part, hipStream_t s);
+
+// Device transpose (stable counting sort): see transpose.hip.
+size_t transpose_scratch_bytes(uint32_t rows, uint32_t cols, uint64_t nnz);
+hipError_t launch_transpose(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr, const uint32_t *col,
+                            const float *val, uint64_t *colptrT, uint32_t *rowT, float *valT, void *scratch,
+                            hipStream_t s);
+
+}  // namespace mlem

@@ -1,0 +1,432 @@
+"""Device half of the ctypes bindings: libmlem_hip.so (include/mlem_hip.h).
+
+Importing this module does not touch the GPU; the library is loaded on first
+use and MlemError is raised if it is not built.
+"""
+from __future__ import annotations
+
+import ctypes as C
+from typing import Optional
+
+import numpy as np
+
+from . import (HIP_LIB_PATH, MLEM_OK, Csr, MlemError, Opts, _load, _ptr, cint, f32, f64, i32, sz, u32, u64, vp)
+
+__all__ = ["hip_lib", "device_count", "device_info", "DeviceArray", "Stream", "Event", "ResidentMatrix", "Recon",
+           "mlem_run", "default_opts", "opts", "merge_workspace_bytes", "fp_merge", "bp_merge", "fp_csr_vector",
+           "bp_csr_vector", "bp_scatter", "fp_ratio_merge", "fp_ratio_csr_vector", "bp_update_merge",
+           "bp_update_csr_vector", "ratio", "update", "colsum_f64", "colsum_scatter_f64", "fill_f32", "device_sync",
+           "version"]
+
+_hip = None
+
+
+def hip_lib() -> C.CDLL:
+    global _hip
+    if _hip is None:
+        lib = _load(HIP_LIB_PATH, "libmlem_hip.so")
+        P = C.POINTER
+        sigs = {
+            "mlem_last_error": (C.c_char_p, []),
+            "mlem_status_string": (C.c_char_p, [cint]),
+            "mlem_version": (C.c_char_p, []),
+            "mlem_device_count": (cint, [P(cint)]),
+            "mlem_set_device": (cint, [cint]),
+            "mlem_device_info": (cint, [cint, C.c_char_p, sz, P(u64), P(cint)]),
+            "mlem_device_alloc": (cint, [P(vp), sz]),
+            "mlem_device_free": (cint, [vp]),
+            "mlem_memcpy_h2d": (cint, [vp, vp, sz, vp]),
+            "mlem_memcpy_d2h": (cint, [vp, vp, sz, vp]),
+            "mlem_memcpy_d2d": (cint, [vp, vp, sz, vp]),
+            "mlem_stream_create": (cint, [P(vp)]),
+            "mlem_stream_destroy": (cint, [vp]),
+            "mlem_stream_sync": (cint, [vp]),
+            "mlem_device_sync": (cint, []),
+            "mlem_event_create": (cint, [P(vp)]),
+            "mlem_event_destroy": (cint, [vp]),
+            "mlem_event_record": (cint, [vp, vp]),
+            "mlem_event_elapsed_ms": (cint, [vp, vp, P(f32)]),
+            "mlem_fill_f32": (cint, [vp, f32, u64, vp]),
+            "mlem_ratio": (cint, [vp, vp, u32, vp]),
+            "mlem_update": (cint, [vp, vp, vp, u32, vp]),
+            "mlem_merge_workspace_bytes": (sz, [u32, u64, u32]),
+            "mlem_fp_merge": (cint, [vp, vp, vp, vp, vp, u32, u32, u64, vp, sz, vp]),
+            "mlem_bp_merge": (cint, [vp, vp, vp, vp, vp, u32, u32, u64, vp, sz, vp]),
+            "mlem_fp_csr_vector": (cint, [vp, vp, vp, vp, vp, u32, u64, u32, vp]),
+            "mlem_bp_csr_vector": (cint, [vp, vp, vp, vp, vp, u32, u64, u32, vp]),
+            "mlem_bp_scatter": (cint, [vp, vp, vp, vp, vp, u32, u32, u64, u32, vp]),
+            "mlem_fp_ratio_merge": (cint, [vp, vp, vp, vp, vp, vp, u32, u32, u64, vp, sz, vp]),
+            "mlem_fp_ratio_csr_vector": (cint, [vp, vp, vp, vp, vp, vp, u32, u64, u32, vp]),
+            "mlem_bp_update_merge": (cint, [vp, vp, vp, vp, vp, vp, u32, u32, u64, vp, sz, vp]),
+            "mlem_bp_update_csr_vector": (cint, [vp, vp, vp, vp, vp, vp, u32, u64, u32, vp]),
+            "mlem_colsum_f64": (cint, [vp, vp, u32, u64, vp, vp, vp]),
+            "mlem_colsum_scatter_f64": (cint, [vp, vp, vp, u32, u32, u64, vp, vp, vp]),
+            "mlem_opts_default": (None, [P(Opts)]),
+            "mlem_matrix_create": (cint, [cint, u32, u32, u64, vp, vp, vp, vp, vp, vp, P(vp)]),
+            "mlem_matrix_build_transpose": (cint, [vp]),
+            "mlem_matrix_destroy": (cint, [vp]),
+            "mlem_matrix_info": (cint, [vp, P(u32), P(u32), P(u64), P(cint), P(u64)]),
+            "mlem_matrix_device_arrays": (cint, [vp, P(vp), P(vp), P(vp), P(vp), P(vp), P(vp)]),
+            "mlem_matrix_download_transpose": (cint, [vp, vp, vp, vp]),
+            "mlem_matrix_sensitivity": (cint, [vp, vp, vp]),
+            "mlem_recon_create": (cint, [vp, vp, vp, vp, P(Opts), P(vp)]),
+            "mlem_recon_iterate": (cint, [vp, u32]),
+            "mlem_recon_sync": (cint, [vp]),
+            "mlem_recon_get_image": (cint, [vp, vp]),
+            "mlem_recon_set_image": (cint, [vp, vp]),
+            "mlem_recon_kernel_times": (cint, [vp, P(f64), P(f64), P(u32)]),
+            "mlem_recon_reset_times": (cint, [vp]),
+            "mlem_recon_loglik": (cint, [vp, P(f64)]),
+            "mlem_recon_stream": (vp, [vp]),
+            "mlem_recon_destroy": (cint, [vp]),
+            "mlem_run": (cint, [vp, vp, vp, vp, vp, u32, P(Opts)]),
+        }
+        for name, (res, args) in sigs.items():
+            fn = getattr(lib, name)
+            fn.restype = res
+            fn.argtypes = args
+        _hip = lib
+    return _hip
+
+
+def _check(st: int):
+    if st != MLEM_OK:
+        raise MlemError(st, hip_lib().mlem_last_error().decode())
+
+
+def version() -> str:
+    return hip_lib().mlem_version().decode()
+
+
+def device_count() -> int:
+    n = cint()
+    _check(hip_lib().mlem_device_count(C.byref(n)))
+    return n.value
+
+
+def device_info(dev: int = 0):
+    name = C.create_string_buffer(256)
+    mem, cus = u64(), cint()
+    _check(hip_lib().mlem_device_info(dev, name, 256, C.byref(mem), C.byref(cus)))
+    return name.value.decode(), mem.value, cus.value
+
+
+def device_sync():
+    _check(hip_lib().mlem_device_sync())
+
+
+class Stream:
+    def __init__(self):
+        self.h = vp()
+        _check(hip_lib().mlem_stream_create(C.byref(self.h)))
+
+    def sync(self):
+        _check(hip_lib().mlem_stream_sync(self.h))
+
+    def close(self):
+        if self.h:
+            hip_lib().mlem_stream_destroy(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Event:
+    def __init__(self):
+        self.h = vp()
+        _check(hip_lib().mlem_event_create(C.byref(self.h)))
+
+    def record(self, stream=None):
+        _check(hip_lib().mlem_event_record(self.h, _sh(stream)))
+
+    def elapsed_ms(self, later: "Event") -> float:
+        ms = f32()
+        _check(hip_lib().mlem_event_elapsed_ms(self.h, later.h, C.byref(ms)))
+        return ms.value
+
+    def __del__(self):
+        try:
+            if self.h:
+                hip_lib().mlem_event_destroy(self.h)
+        except Exception:
+            pass
+
+
+def _sh(stream):
+    if stream is None:
+        return None
+    if isinstance(stream, Stream):
+        return stream.h
+    return stream
+
+
+class DeviceArray:
+    """A 1-D device buffer of a numpy dtype."""
+
+    def __init__(self, n: int, dtype):
+        self.n = int(n)
+        self.dtype = np.dtype(dtype)
+        self.nbytes = self.n * self.dtype.itemsize
+        self.h = vp()
+        _check(hip_lib().mlem_device_alloc(C.byref(self.h), self.nbytes))
+
+    @staticmethod
+    def from_numpy(a: np.ndarray) -> "DeviceArray":
+        a = np.ascontiguousarray(a)
+        d = DeviceArray(a.size, a.dtype)
+        d.upload(a)
+        return d
+
+    def upload(self, a: np.ndarray, stream=None):
+        a = np.ascontiguousarray(a, dtype=self.dtype)
+        if a.size != self.n:
+            raise ValueError("size mismatch")
+        _check(hip_lib().mlem_memcpy_h2d(self.h, _ptr(a), self.nbytes, _sh(stream)))
+
+    def numpy(self, stream=None) -> np.ndarray:
+        out = np.empty(self.n, self.dtype)
+        _check(hip_lib().mlem_memcpy_d2h(_ptr(out), self.h, self.nbytes, _sh(stream)))
+        return out
+
+    @property
+    def ptr(self):
+        return self.h
+
+    def free(self):
+        if self.h:
+            hip_lib().mlem_device_free(self.h)
+            self.h = vp()
+
+    def __del__(self):
+        try:
+            self.free()
+        except Exception:
+            pass
+
+
+def _dp(x):
+    if x is None:
+        return None
+    if isinstance(x, DeviceArray):
+        return x.h
+    return x
+
+
+def default_opts() -> Opts:
+    o = Opts()
+    hip_lib().mlem_opts_default(C.byref(o))
+    return o
+
+
+def opts(spmv=0, using_trans=0, secsize_fp=8, secsize_bp=8, fp_lanes=0, bp_lanes=0, profile=0, use_graph=0) -> Opts:
+    o = Opts()
+    o.spmv, o.using_trans, o.secsize_fp, o.secsize_bp = spmv, using_trans, secsize_fp, secsize_bp
+    o.fp_lanes, o.bp_lanes, o.profile, o.use_graph = fp_lanes, bp_lanes, profile, use_graph
+    return o
+
+
+# ---------------------------------------------------------------- kernels
+def fill_f32(dst: DeviceArray, value: float, stream=None):
+    _check(hip_lib().mlem_fill_f32(_dp(dst), value, dst.n, _sh(stream)))
+
+
+def ratio(g, fwproj, rows, stream=None):
+    _check(hip_lib().mlem_ratio(_dp(g), _dp(fwproj), rows, _sh(stream)))
+
+
+def update(f, norm, bwproj, cols, stream=None):
+    _check(hip_lib().mlem_update(_dp(f), _dp(norm), _dp(bwproj), cols, _sh(stream)))
+
+
+def merge_workspace_bytes(rows: int, nnz: int, secsize: int) -> int:
+    return int(hip_lib().mlem_merge_workspace_bytes(rows, nnz, secsize))
+
+
+def fp_merge(rowptr, col, val, f, out, secsize, rows, nnz, ws, ws_bytes, stream=None):
+    _check(hip_lib().mlem_fp_merge(_dp(rowptr), _dp(col), _dp(val), _dp(f), _dp(out), secsize, rows, nnz, _dp(ws),
+                                   ws_bytes, _sh(stream)))
+
+
+def bp_merge(rowptrT, colT, valT, correl, out, secsize, cols, nnz, ws, ws_bytes, stream=None):
+    _check(hip_lib().mlem_bp_merge(_dp(rowptrT), _dp(colT), _dp(valT), _dp(correl), _dp(out), secsize, cols, nnz,
+                                   _dp(ws), ws_bytes, _sh(stream)))
+
+
+def fp_csr_vector(rowptr, col, val, f, out, rows, nnz, lanes=0, stream=None):
+    _check(hip_lib().mlem_fp_csr_vector(_dp(rowptr), _dp(col), _dp(val), _dp(f), _dp(out), rows, nnz, lanes,
+                                        _sh(stream)))
+
+
+def bp_csr_vector(rowptrT, colT, valT, correl, out, cols, nnz, lanes=0, stream=None):
+    _check(hip_lib().mlem_bp_csr_vector(_dp(rowptrT), _dp(colT), _dp(valT), _dp(correl), _dp(out), cols, nnz, lanes,
+                                        _sh(stream)))
+
+
+def bp_scatter(rowptr, col, val, correl, out, rows, cols, nnz, lanes=0, stream=None):
+    _check(hip_lib().mlem_bp_scatter(_dp(rowptr), _dp(col), _dp(val), _dp(correl), _dp(out), rows, cols, nnz, lanes,
+                                     _sh(stream)))
+
+
+def fp_ratio_merge(rowptr, col, val, f, g, out, secsize, rows, nnz, ws, ws_bytes, stream=None):
+    _check(hip_lib().mlem_fp_ratio_merge(_dp(rowptr), _dp(col), _dp(val), _dp(f), _dp(g), _dp(out), secsize, rows,
+                                         nnz, _dp(ws), ws_bytes, _sh(stream)))
+
+
+def fp_ratio_csr_vector(rowptr, col, val, f, g, out, rows, nnz, lanes=0, stream=None):
+    _check(hip_lib().mlem_fp_ratio_csr_vector(_dp(rowptr), _dp(col), _dp(val), _dp(f), _dp(g), _dp(out), rows, nnz,
+                                              lanes, _sh(stream)))
+
+
+def bp_update_merge(rowptrT, colT, valT, ratio_, norm, f, secsize, cols, nnz, ws, ws_bytes, stream=None):
+    _check(hip_lib().mlem_bp_update_merge(_dp(rowptrT), _dp(colT), _dp(valT), _dp(ratio_), _dp(norm), _dp(f),
+                                          secsize, cols, nnz, _dp(ws), ws_bytes, _sh(stream)))
+
+
+def bp_update_csr_vector(rowptrT, colT, valT, ratio_, norm, f, cols, nnz, lanes=0, stream=None):
+    _check(hip_lib().mlem_bp_update_csr_vector(_dp(rowptrT), _dp(colT), _dp(valT), _dp(ratio_), _dp(norm), _dp(f),
+                                               cols, nnz, lanes, _sh(stream)))
+
+
+def colsum_f64(rowptrT, valT, cols, nnz, sens64=None, sens32=None, stream=None):
+    _check(hip_lib().mlem_colsum_f64(_dp(rowptrT), _dp(valT), cols, nnz, _dp(sens64), _dp(sens32), _sh(stream)))
+
+
+def colsum_scatter_f64(rowptr, col, val, rows, cols, nnz, sens64=None, sens32=None, stream=None):
+    _check(hip_lib().mlem_colsum_scatter_f64(_dp(rowptr), _dp(col), _dp(val), rows, cols, nnz, _dp(sens64),
+                                             _dp(sens32), _sh(stream)))
+
+
+# ---------------------------------------------------------------- handles
+class ResidentMatrix:
+    """A system matrix (and optionally its transpose) resident in HBM."""
+
+    def __init__(self, A: Csr, AT: Optional[Csr] = None, device: int = 0):
+        self.h = vp()
+        self.rows, self.cols, self.nnz = A.rows, A.cols, A.nnz
+        _check(hip_lib().mlem_matrix_create(
+            device, A.rows, A.cols, A.nnz, _ptr(A.rowptr), _ptr(A.col), _ptr(A.val),
+            _ptr(AT.rowptr) if AT is not None else None, _ptr(AT.col) if AT is not None else None,
+            _ptr(AT.val) if AT is not None else None, C.byref(self.h)))
+
+    def build_transpose(self):
+        _check(hip_lib().mlem_matrix_build_transpose(self.h))
+
+    def info(self):
+        r, c, n, t, b = u32(), u32(), u64(), cint(), u64()
+        _check(hip_lib().mlem_matrix_info(self.h, C.byref(r), C.byref(c), C.byref(n), C.byref(t), C.byref(b)))
+        return dict(rows=r.value, cols=c.value, nnz=n.value, has_transpose=bool(t.value), device_bytes=b.value)
+
+    def device_arrays(self):
+        ptrs = [vp() for _ in range(6)]
+        _check(hip_lib().mlem_matrix_device_arrays(self.h, *[C.byref(p) for p in ptrs]))
+        return ptrs
+
+    def download_transpose(self) -> Csr:
+        colptr = np.zeros(self.cols + 1, np.uint64)
+        rowT = np.empty(self.nnz, np.uint32)
+        valT = np.empty(self.nnz, np.float32)
+        _check(hip_lib().mlem_matrix_download_transpose(self.h, _ptr(colptr), _ptr(rowT), _ptr(valT)))
+        return Csr(self.cols, self.rows, colptr, rowT, valT)
+
+    def sensitivity(self):
+        s32 = np.zeros(self.cols, np.float32)
+        s64 = np.zeros(self.cols, np.float64)
+        _check(hip_lib().mlem_matrix_sensitivity(self.h, _ptr(s32), _ptr(s64)))
+        return s32, s64
+
+    def close(self):
+        if self.h:
+            hip_lib().mlem_matrix_destroy(self.h)
+            self.h = vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Recon:
+    """One reconstruction over a resident matrix (its own stream and buffers)."""
+
+    def __init__(self, m: ResidentMatrix, g: np.ndarray, f0: np.ndarray, sens: Optional[np.ndarray] = None,
+                 options: Optional[Opts] = None):
+        self.m = m
+        self.h = vp()
+        g = np.ascontiguousarray(g, np.int32)
+        f0 = np.ascontiguousarray(f0, np.float32)
+        sens = None if sens is None else np.ascontiguousarray(sens, np.float32)
+        if g.size != m.rows or f0.size != m.cols or (sens is not None and sens.size != m.cols):
+            raise ValueError("counts / image / sensitivity sizes do not match the matrix")
+        _check(hip_lib().mlem_recon_create(m.h, _ptr(g), _ptr(sens), _ptr(f0),
+                                           C.byref(options) if options is not None else None, C.byref(self.h)))
+
+    def iterate(self, n: int):
+        _check(hip_lib().mlem_recon_iterate(self.h, n))
+
+    def sync(self):
+        _check(hip_lib().mlem_recon_sync(self.h))
+
+    def image(self) -> np.ndarray:
+        out = np.empty(self.m.cols, np.float32)
+        _check(hip_lib().mlem_recon_get_image(self.h, _ptr(out)))
+        return out
+
+    def set_image(self, f: np.ndarray):
+        f = np.ascontiguousarray(f, np.float32)
+        _check(hip_lib().mlem_recon_set_image(self.h, _ptr(f)))
+
+    def kernel_times(self):
+        a, b, n = f64(), f64(), u32()
+        _check(hip_lib().mlem_recon_kernel_times(self.h, C.byref(a), C.byref(b), C.byref(n)))
+        return a.value, b.value, n.value
+
+    def reset_times(self):
+        _check(hip_lib().mlem_recon_reset_times(self.h))
+
+    def loglik(self) -> float:
+        v = f64()
+        _check(hip_lib().mlem_recon_loglik(self.h, C.byref(v)))
+        return v.value
+
+    @property
+    def stream(self):
+        return hip_lib().mlem_recon_stream(self.h)
+
+    def close(self):
+        if self.h:
+            hip_lib().mlem_recon_destroy(self.h)
+            self.h = vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+def mlem_run(m: ResidentMatrix, g: np.ndarray, f0: np.ndarray, iters: int, sens: Optional[np.ndarray] = None,
+             options: Optional[Opts] = None) -> np.ndarray:
+    g = np.ascontiguousarray(g, np.int32)
+    f0 = np.ascontiguousarray(f0, np.float32)
+    sens = None if sens is None else np.ascontiguousarray(sens, np.float32)
+    out = np.empty(m.cols, np.float32)
+    _check(hip_lib().mlem_run(m.h, _ptr(g), _ptr(sens), _ptr(f0), _ptr(out), iters,
+                              C.byref(options) if options is not None else None))
+    return out

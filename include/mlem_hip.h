@@ -1,0 +1,252 @@
+/*
+ * mlem_hip.h -- device-side C ABI of the MLEM engine (libmlem_hip.so, gfx950).
+ *
+ * Three layers, each replacing one interface of the reference:
+ *
+ *  1. Kernel-level entry points: one per __global__ of kernel.cuh:7-19, plus
+ *     the fused variants (FP+ratio, BP+update) and the FP64 sensitivity.
+ *     They take device pointers, 64-bit row pointers and a HIP stream
+ *     (void*, NULL = the null stream), launch asynchronously and return a
+ *     status.  Unlike the reference kernels (which atomicAdd into a
+ *     pre-zeroed output) they OVERWRITE their outputs and, apart from
+ *     mlem_bp_scatter and mlem_colsum_scatter_f64, are deterministic: the same
+ *     inputs give bitwise the same outputs on every call.
+ *  2. Resident matrix + reconstruction handles: the system matrix A and its
+ *     transpose stay in HBM; a reconstruction owns its image/ratio buffers and
+ *     one stream, so different handles may be driven from different threads.
+ *  3. mlem_run(): the whole reconstruction, replacing
+ *     void mlem(...) (main.cu:19,167-514) with const inputs and a status.
+ *
+ * Errors: every call returns an mlem_status; mlem_last_error() holds a
+ * message for the calling thread.  Every HIP call inside is checked.
+ */
+#ifndef MLEM_HIP_H
+#define MLEM_HIP_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#ifndef MLEM_STATUS_DEFINED
+#define MLEM_STATUS_DEFINED
+typedef enum mlem_status {
+    MLEM_OK = 0,
+    MLEM_ERR_INVALID_ARGUMENT = 1,
+    MLEM_ERR_HIP = 2,
+    MLEM_ERR_OUT_OF_MEMORY = 3,
+    MLEM_ERR_UNSUPPORTED = 4,
+    MLEM_ERR_NO_DEVICE = 5,
+    MLEM_ERR_IO = 6,
+    MLEM_ERR_FORMAT = 7
+} mlem_status;
+#endif
+
+const char *mlem_last_error(void);
+const char *mlem_status_string(int status);
+/* Library version string, e.g. "mlem-mi355x 0.1 (gfx950)". */
+const char *mlem_version(void);
+
+/* ------------------------------------------------------------------------ */
+/* runtime helpers (so callers need no HIP runtime of their own)            */
+/* ------------------------------------------------------------------------ */
+int mlem_device_count(int *count);            /* cudaGetDeviceCount, main.cu:57-62 */
+int mlem_set_device(int device);
+int mlem_device_info(int device, char *name, size_t name_len,
+                     uint64_t *total_mem, int *cu_count);
+int mlem_device_alloc(void **ptr, size_t bytes);
+int mlem_device_free(void *ptr);
+int mlem_memcpy_h2d(void *dst, const void *src, size_t bytes, void *stream);
+int mlem_memcpy_d2h(void *dst, const void *src, size_t bytes, void *stream);
+int mlem_memcpy_d2d(void *dst, const void *src, size_t bytes, void *stream);
+int mlem_stream_create(void **stream);
+int mlem_stream_destroy(void *stream);
+int mlem_stream_sync(void *stream);
+int mlem_device_sync(void);
+/* HIP events for timing on a given stream. */
+int mlem_event_create(void **event);
+int mlem_event_destroy(void *event);
+int mlem_event_record(void *event, void *stream);
+int mlem_event_elapsed_ms(void *start, void *stop, float *ms);
+/* Fill n floats with a value (a kernel, not hipMemset). */
+int mlem_fill_f32(float *dst, float value, uint64_t n, void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* 1. kernel-level entry points                                             */
+/* ------------------------------------------------------------------------ */
+
+/* calcCorrel (kernel.cu:11-19): fwproj[i] = (float)g[i] / fwproj[i] where
+ * fwproj[i] != 0, else left as is.  In place over fwproj f32[rows]. */
+int mlem_ratio(const int32_t *g, float *fwproj, uint32_t rows, void *stream);
+
+/* calcUpdate (kernel.cu:28-39): bwproj[j] = f[j]*bwproj[j]/norm[j], or
+ * f[j]*bwproj[j] where norm[j] == 0.  In place over bwproj f32[cols]. */
+int mlem_update(const float *f, const float *norm, float *bwproj,
+                uint32_t cols, void *stream);
+
+/* Merge-path SpMV y = A x (calcFwProj_merge_based / calcBwProj_merge_based,
+ * kernel.cu:52-105,190-286).  secsize = merge-path items per thread
+ * (argv[5]/argv[6]); each 256-thread workgroup owns 256*secsize items.
+ * Supported secsize: 1..64.  The workspace holds the tile partition and the
+ * cross-tile carries; its size comes from mlem_merge_workspace_bytes().
+ * y is f32[rows] and is overwritten (no pre-zero needed). */
+size_t mlem_merge_workspace_bytes(uint32_t rows, uint64_t nnz,
+                                  uint32_t secsize);
+int mlem_fp_merge(const uint64_t *rowptr, const uint32_t *col,
+                  const float *val, const float *f, float *fwproj,
+                  uint32_t secsize, uint32_t rows, uint64_t nnz,
+                  void *workspace, size_t workspace_bytes, void *stream);
+int mlem_bp_merge(const uint64_t *rowptrT, const uint32_t *colT,
+                  const float *valT, const float *correl, float *bwproj,
+                  uint32_t secsize, uint32_t cols, uint64_t nnz,
+                  void *workspace, size_t workspace_bytes, void *stream);
+
+/* CSR-vector SpMV (calcFwProj_csr_vector / calcBwProj_csr_vector,
+ * kernel.cu:118-153,297-336): a group of `lanes` work-items per row
+ * (lanes in {4,8,16,32,64}; 0 picks from the mean row length), a
+ * fixed-order shuffle reduction and a plain store.  y overwritten. */
+int mlem_fp_csr_vector(const uint64_t *rowptr, const uint32_t *col,
+                       const float *val, const float *f, float *fwproj,
+                       uint32_t rows, uint64_t nnz, uint32_t lanes,
+                       void *stream);
+int mlem_bp_csr_vector(const uint64_t *rowptrT, const uint32_t *colT,
+                       const float *valT, const float *correl, float *bwproj,
+                       uint32_t cols, uint64_t nnz, uint32_t lanes,
+                       void *stream);
+
+/* BP without a transpose (calcBwProj_none_trans, kernel.cu:165-177,347-370):
+ * bwproj = A^T correl by a float-atomic scatter over A's rows.  bwproj
+ * f32[cols] is zeroed first (overwrite semantics).  Not deterministic in the
+ * last bits (atomic order), like the reference. */
+int mlem_bp_scatter(const uint64_t *rowptr, const uint32_t *col,
+                    const float *val, const float *correl, float *bwproj,
+                    uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t lanes,
+                    void *stream);
+
+/* Fused FP + ratio: ratio[i] = g[i] / (A f)_i where (A f)_i != 0, else 0. */
+int mlem_fp_ratio_merge(const uint64_t *rowptr, const uint32_t *col,
+                        const float *val, const float *f, const int32_t *g,
+                        float *ratio, uint32_t secsize, uint32_t rows,
+                        uint64_t nnz, void *workspace, size_t workspace_bytes,
+                        void *stream);
+int mlem_fp_ratio_csr_vector(const uint64_t *rowptr, const uint32_t *col,
+                             const float *val, const float *f,
+                             const int32_t *g, float *ratio, uint32_t rows,
+                             uint64_t nnz, uint32_t lanes, void *stream);
+
+/* Fused BP + update on A^T: f[j] <- f[j]*bp_j/norm[j] (f[j]*bp_j where
+ * norm[j] == 0), bp = A^T ratio.  f is read and written in place. */
+int mlem_bp_update_merge(const uint64_t *rowptrT, const uint32_t *colT,
+                         const float *valT, const float *ratio,
+                         const float *norm, float *f, uint32_t secsize,
+                         uint32_t cols, uint64_t nnz, void *workspace,
+                         size_t workspace_bytes, void *stream);
+int mlem_bp_update_csr_vector(const uint64_t *rowptrT, const uint32_t *colT,
+                              const float *valT, const float *ratio,
+                              const float *norm, float *f, uint32_t cols,
+                              uint64_t nnz, uint32_t lanes, void *stream);
+
+/* Sensitivity (column sums of A, calcColumnSums main.cu:558-580) in FP64 as
+ * the row sums of A^T: deterministic.  sens64 f64[cols] and/or sens32
+ * f32[cols] (rounded from the FP64 sum); either may be NULL. */
+int mlem_colsum_f64(const uint64_t *rowptrT, const float *valT,
+                    uint32_t cols, uint64_t nnz, double *sens64, float *sens32,
+                    void *stream);
+/* The same from A itself (no transpose) with FP64 atomics. */
+int mlem_colsum_scatter_f64(const uint64_t *rowptr, const uint32_t *col,
+                            const float *val, uint32_t rows, uint32_t cols,
+                            uint64_t nnz, double *sens64, float *sens32,
+                            void *stream);
+
+/* ------------------------------------------------------------------------ */
+/* 2. resident matrix and reconstruction handles                            */
+/* ------------------------------------------------------------------------ */
+
+typedef struct mlem_matrix mlem_matrix;
+typedef struct mlem_recon mlem_recon;
+
+/* argv[5..8] of the reference CLI (main.cu:50-55) plus tuning knobs. */
+typedef struct mlem_opts {
+    uint32_t spmv;        /* argv[8]: 0 merge-based, 1 CSR-vector            */
+    uint32_t using_trans; /* argv[7]: 0 BP on A^T, 1 BP by scatter over A    */
+    uint32_t secsize_fp;  /* argv[5]: merge-path items per thread, FP        */
+    uint32_t secsize_bp;  /* argv[6]: merge-path items per thread, BP        */
+    uint32_t fp_lanes;    /* CSR-vector lanes per row for FP (0 = auto)      */
+    uint32_t bp_lanes;    /* CSR-vector lanes per row for BP (0 = auto)      */
+    uint32_t profile;     /* 1: time FP and BP with HIP events per iteration */
+    uint32_t use_graph;   /* 1: replay iterations from a captured hipGraph   */
+} mlem_opts;
+
+/* Fill opts with the reference's usual defaults (merge FP, transposed BP,
+ * secsize 8, auto lanes, no profiling, no graph). */
+void mlem_opts_default(mlem_opts *opts);
+
+/* Upload A (and A^T if rowptrT != NULL) from host arrays to `device` and keep
+ * them resident.  Replaces the per-GPU cudaMalloc/cudaMemcpy of
+ * main.cu:236-299 for one GPU with 64-bit row pointers. */
+int mlem_matrix_create(int device, uint32_t rows, uint32_t cols, uint64_t nnz,
+                       const uint64_t *rowptr, const uint32_t *col,
+                       const float *val, const uint64_t *rowptrT,
+                       const uint32_t *colT, const float *valT,
+                       mlem_matrix **out);
+/* Build A^T on the device from the resident A (stable counting sort,
+ * bit-exact with the host transpose) instead of uploading one. */
+int mlem_matrix_build_transpose(mlem_matrix *m);
+int mlem_matrix_destroy(mlem_matrix *m);
+int mlem_matrix_info(const mlem_matrix *m, uint32_t *rows, uint32_t *cols,
+                     uint64_t *nnz, int *has_transpose, uint64_t *device_bytes);
+/* Device pointers of the resident arrays (any may be NULL). */
+int mlem_matrix_device_arrays(const mlem_matrix *m, const uint64_t **rowptr,
+                              const uint32_t **col, const float **val,
+                              const uint64_t **rowptrT, const uint32_t **colT,
+                              const float **valT);
+/* Download A^T (built or uploaded) to host arrays. */
+int mlem_matrix_download_transpose(const mlem_matrix *m, uint64_t *colptrT,
+                                   uint32_t *rowT, float *valT);
+/* FP64 sensitivity on the GPU; sens32 f32[cols] and/or sens64 f64[cols]
+ * host outputs (either may be NULL). */
+int mlem_matrix_sensitivity(const mlem_matrix *m, float *sens32,
+                            double *sens64);
+
+/* A reconstruction over a resident matrix: g int32[rows] counts, sens
+ * f32[cols] (NULL: FP64 sensitivity on the GPU), f0 f32[cols] start image. */
+int mlem_recon_create(const mlem_matrix *m, const int32_t *g,
+                      const float *sens, const float *f0,
+                      const mlem_opts *opts, mlem_recon **out);
+/* Enqueue `iters` MLEM iterations (main.cu:329-449) on the handle's stream. */
+int mlem_recon_iterate(mlem_recon *r, uint32_t iters);
+int mlem_recon_sync(mlem_recon *r);
+int mlem_recon_get_image(mlem_recon *r, float *f_out);
+/* Replace the image (f32[cols]) to restart from another x. */
+int mlem_recon_set_image(mlem_recon *r, const float *f);
+/* Sum over iterations of the FP and BP kernel times (HIP events on the
+ * handle's stream; needs opts.profile) and the number of iterations timed.
+ * The FP time covers FP+ratio (+ the merge fix-up), the BP time covers
+ * BP+update (+ fix-up, or scatter+update). */
+int mlem_recon_kernel_times(mlem_recon *r, double *fp_ms, double *bp_ms,
+                            uint32_t *iters_timed);
+int mlem_recon_reset_times(mlem_recon *r);
+/* Poisson log-likelihood sum_i g_i log(Ax)_i - (Ax)_i of the current image,
+ * computed on the GPU in FP64 (rows with (Ax)_i <= 0 contribute -(Ax)_i). */
+int mlem_recon_loglik(mlem_recon *r, double *ll);
+/* The handle's stream (a hipStream_t). */
+void *mlem_recon_stream(mlem_recon *r);
+int mlem_recon_destroy(mlem_recon *r);
+
+/* ------------------------------------------------------------------------ */
+/* 3. whole reconstruction                                                  */
+/* ------------------------------------------------------------------------ */
+
+/* mlem() of main.cu:167-514 on one GPU: iterate `iters` times from f0 and
+ * write the image to f_out (host arrays; the caller keeps ownership, nothing
+ * is mutated).  sens NULL: FP64 sensitivity on the GPU. */
+int mlem_run(const mlem_matrix *m, const int32_t *g, const float *sens,
+             const float *f0, float *f_out, uint32_t iters,
+             const mlem_opts *opts);
+
+#ifdef __cplusplus
+}
+#endif
+#endif /* MLEM_HIP_H */

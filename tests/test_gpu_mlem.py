@@ -1,0 +1,136 @@
+"""Full MLEM on the GPU against the FP64 oracle (oracle_mlem_f64).
+
+Bar: relative L2 error of the FP32 image after N iterations <= 1e-5 against
+the FP64 oracle run from the same start image with the same sensitivity.
+"""
+import numpy as np
+import pytest
+
+import mlem_amd as M
+import oracle as O
+from matrices import make
+
+pytestmark = pytest.mark.gpu
+
+REL_L2 = 1e-5
+
+
+def rel_l2(a, b):
+    a = np.asarray(a, np.float64)
+    b = np.asarray(b, np.float64)
+    return np.linalg.norm(a - b) / np.linalg.norm(b)
+
+
+@pytest.fixture(scope="module")
+def resident(c1):
+    m = M.ResidentMatrix(c1["A"], c1["AT"])
+    yield m
+    m.close()
+
+
+@pytest.fixture(scope="module")
+def ref50(c1, resident):
+    A = c1["A"]
+    s32, s64 = resident.sensitivity()
+    f0 = np.ones(A.cols, np.float32)  # uniform x0 = 1
+    ref = O.mlem_f64(A.rowptr, A.col, A.val, A.rows, A.cols, c1["g"], s32.astype(np.float64), f0, 50)
+    return dict(s32=s32, s64=s64, f0=f0, ref=ref)
+
+
+@pytest.mark.parametrize("trans,spmv", [(0, 0), (0, 1), (1, 0), (1, 1)])
+def test_mlem_50_iterations_vs_fp64_oracle(c1, resident, ref50, trans, spmv):
+    print(f"mlem C1 trans={trans} spmv={spmv}", flush=True)
+    o = M.opts(spmv=spmv, using_trans=trans, secsize_fp=5, secsize_bp=5)
+    img = M.mlem_run(resident, c1["g"], ref50["f0"], 50, sens=ref50["s32"], options=o)
+    err = rel_l2(img, ref50["ref"])
+    print(f"  rel-L2 vs FP64 oracle: {err:.3e}", flush=True)
+    assert err <= REL_L2
+
+
+def test_sensitivity_fp64_matches_oracle(c1, resident, ref50):
+    A = c1["A"]
+    ref = O.colsum_f64(A.rowptr, A.col, A.val, A.rows, A.cols)
+    np.testing.assert_allclose(ref50["s64"], ref, rtol=1e-12, atol=0)
+    assert np.array_equal(ref50["s32"], ref.astype(np.float32))
+
+
+def test_gpu_sensitivity_default_path(c1, resident, ref50):
+    # sens=None: the reconstruction computes FP64 column sums itself.
+    o = M.opts(spmv=1, using_trans=0)
+    a = M.mlem_run(resident, c1["g"], ref50["f0"], 5, sens=None, options=o)
+    b = M.mlem_run(resident, c1["g"], ref50["f0"], 5, sens=ref50["s32"], options=o)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+
+
+def test_reference_init_and_scale_invariance(c1, resident, ref50):
+    A, g = c1["A"], c1["g"]
+    f0, init = M.initial_image(g, A.rows, ref50["s32"])
+    assert init == O.initial_image(g, A.rows, ref50["s32"])
+    o = M.opts(spmv=0, using_trans=0)
+    img = M.mlem_run(resident, g, f0, 50, sens=ref50["s32"], options=o)
+    # After >= 1 iteration MLEM does not depend on a uniform x0 scale.
+    assert rel_l2(img, ref50["ref"]) <= REL_L2
+
+
+def test_deterministic_and_graph_replay(c1, resident, ref50):
+    o = M.opts(spmv=0, using_trans=0, secsize_fp=8, secsize_bp=8)
+    a = M.mlem_run(resident, c1["g"], ref50["f0"], 20, sens=ref50["s32"], options=o)
+    b = M.mlem_run(resident, c1["g"], ref50["f0"], 20, sens=ref50["s32"], options=o)
+    assert np.array_equal(a.view(np.uint32), b.view(np.uint32))
+    og = M.opts(spmv=0, using_trans=0, secsize_fp=8, secsize_bp=8, use_graph=1)
+    with M.Recon(resident, c1["g"], ref50["f0"], sens=ref50["s32"], options=og) as r:
+        for _ in range(4):
+            r.iterate(5)
+            r.sync()
+            r.image()  # eager host work between replays
+        c = r.image()
+    assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+
+
+def test_recon_loglik_fp64(c1, resident, ref50):
+    A, g = c1["A"], c1["g"]
+    o = M.opts(spmv=1, using_trans=0)
+    with M.Recon(resident, g, ref50["f0"], sens=ref50["s32"], options=o) as r:
+        lls = []
+        for _ in range(6):
+            lls.append(r.loglik())
+            img = r.image()
+            ref = O.loglik_f64(A.rowptr, A.col, A.val, A.rows, g, img.astype(np.float64))
+            assert abs(lls[-1] - ref) <= 1e-6 * abs(ref)
+            r.iterate(3)
+    assert all(b >= a - 1e-9 * abs(a) for a, b in zip(lls, lls[1:]))  # EM ascent
+
+
+def test_profile_times(c1, resident, ref50):
+    o = M.opts(spmv=0, using_trans=0, profile=1)
+    with M.Recon(resident, c1["g"], ref50["f0"], sens=ref50["s32"], options=o) as r:
+        r.iterate(10)
+        r.sync()
+        fp, bp, n = r.kernel_times()
+    assert n == 10 and fp > 0 and bp > 0
+
+
+def test_edge_matrices_full_mlem():
+    for kind in ["empty_rows", "one_huge_row", "heavy", "dups"]:
+        A = make(kind, seed=5)
+        AT = M.transpose(A)
+        rng = np.random.default_rng(4)
+        g = rng.integers(0, 30, A.rows).astype(np.int32)
+        with M.ResidentMatrix(A, AT) as m:
+            s32, _ = m.sensitivity()
+            f0 = np.ones(A.cols, np.float32)
+            ref = O.mlem_f64(A.rowptr, A.col, A.val, A.rows, A.cols, g, s32.astype(np.float64), f0, 10)
+            for trans, spmv in [(0, 0), (0, 1), (1, 1)]:
+                print(f"edge {kind} trans={trans} spmv={spmv}", flush=True)
+                img = M.mlem_run(m, g, f0, 10, sens=s32, options=M.opts(spmv=spmv, using_trans=trans, secsize_fp=3,
+                                                                         secsize_bp=4))
+                assert rel_l2(img, ref) <= (REL_L2 if trans == 0 else 1e-4)
+
+
+def test_using_trans_requires_transpose(c1):
+    with M.ResidentMatrix(c1["A"]) as m:
+        with pytest.raises(M.MlemError):
+            M.Recon(m, c1["g"], np.ones(c1["A"].cols, np.float32), options=M.opts(using_trans=0))
+        # scatter path works without a transpose
+        img = M.mlem_run(m, c1["g"], np.ones(c1["A"].cols, np.float32), 2, options=M.opts(using_trans=1, spmv=1))
+        assert np.isfinite(img).all()
