@@ -16,6 +16,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <tuple>
 #include <memory>
 #include <mutex>
 #include <string>
@@ -133,6 +134,12 @@ struct Partition {
     DevBuf buf;
 };
 
+struct Tiled {
+    TilePlan plan;
+    DevBuf mem;
+    TiledMatrix view;
+};
+
 struct mlem_matrix {
     int device = 0;
     uint32_t rows = 0, cols = 0;
@@ -142,9 +149,14 @@ struct mlem_matrix {
     bool has_T = false;
     std::mutex mu;
     std::map<std::pair<int, uint32_t>, std::unique_ptr<Partition>> parts;  // (0: A / 1: A^T, secsize)
+    // Tiles keyed by (0: A / 1: A^T, width W, rows R).
+    std::map<std::tuple<int, uint32_t, uint32_t>, std::unique_ptr<Tiled>> tiles;
+    std::map<std::tuple<int, uint32_t, uint32_t>, uint32_t> tile_refused;  // build flags of refused tilings
 
     uint64_t device_bytes() const {
-        return rowptr.bytes + col.bytes + val.bytes + rowptrT.bytes + colT.bytes + valT.bytes;
+        uint64_t b = rowptr.bytes + col.bytes + val.bytes + rowptrT.bytes + colT.bytes + valT.bytes;
+        for (const auto &t : tiles) b += t.second->mem.bytes;
+        return b;
     }
 };
 
@@ -158,7 +170,12 @@ struct mlem_recon {
     hipStream_t stream = nullptr;
     DevBuf g, sens, f, ratio, bp, scratch_fp, scratch_bp, llpart;
     const Partition *pfp = nullptr, *pbp = nullptr;
+    const Tiled *tfp = nullptr, *tbp = nullptr;  // blocked LDS-tiled path, when used
+    TilePlan plan_fp, plan_bp;                   // with this reconstruction's segment grouping
+    uint32_t tile_s_fp = 8, tile_s_bp = 8;
+    DevBuf part_fp, part_bp;
     uint32_t lanes_fp = 64, lanes_bp = 64;
+    bool bp_merge = true;  // BP on A^T: merge-based or CSR-vector
     std::vector<IterEvents> events;  // pool, one set per profiled iteration
     size_t events_used = 0;
     double fp_ms = 0, bp_ms = 0;
@@ -189,6 +206,83 @@ int get_partition(mlem_matrix *m, int which, uint32_t secsize, const Partition *
     return MLEM_OK;
 }
 
+// Tiles of A (which 0) or A^T (which 1) with segment width W and R rows per
+// block, built on first use.  MLEM_ERR_UNSUPPORTED (and *flags) when the
+// matrix does not allow them (a row not sorted by column).
+int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, const Tiled **out, uint32_t *flags) {
+    std::lock_guard<std::mutex> lk(m->mu);
+    auto key = std::make_tuple(which, W, R);
+    *flags = 0;
+    auto it = m->tiles.find(key);
+    if (it != m->tiles.end()) {
+        *out = it->second.get();
+        return MLEM_OK;
+    }
+    auto ref = m->tile_refused.find(key);
+    if (ref != m->tile_refused.end()) {
+        *flags = ref->second;
+        return set_err(MLEM_ERR_UNSUPPORTED, "tiling refused (flags %u)", *flags);
+    }
+    auto t = std::make_unique<Tiled>();
+    const uint32_t nrow = which == 0 ? m->rows : m->cols, ncol = which == 0 ? m->cols : m->rows;
+    t->plan = tile_plan(nrow, ncol, m->nnz, W, R, 1);
+    HIPCHK(t->mem.alloc(tiled_matrix_bytes(t->plan)));
+    DevBuf scratch;
+    HIPCHK(scratch.alloc(tile_build_scratch_bytes(t->plan)));
+    t->view = tiled_matrix_view(t->plan, t->mem.p);
+    uint32_t fl = 0;
+    if (which == 0)
+        HIPCHK(launch_tile_build(t->plan, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(),
+                                 t->mem.p, scratch.p, &fl, nullptr));
+    else
+        HIPCHK(launch_tile_build(t->plan, m->rowptrT.as<uint64_t>(), m->colT.as<uint32_t>(), m->valT.as<float>(),
+                                 t->mem.p, scratch.p, &fl, nullptr));
+    if (fl) {
+        m->tile_refused[key] = fl;
+        *flags = fl;
+        return set_err(MLEM_ERR_UNSUPPORTED, "tiling (width %u, rows %u) refused: a row is not sorted by column", W,
+                       R);
+    }
+    *out = t.get();
+    m->tiles.emplace(key, std::move(t));
+    return MLEM_OK;
+}
+
+// Chooses the blocked (LDS-tiled) or the plain merge path for one side of
+// the iteration.  Segment groups are sized so the grid has about 2560
+// workgroups (10 per CU).
+int setup_merge_side(mlem_matrix *m, int which, uint32_t tile_width, uint32_t tile_rows, uint32_t secsize,
+                     const Tiled **tiled, TilePlan *plan, uint32_t *tile_s, DevBuf *part, const Partition **partn,
+                     DevBuf *scratch) {
+    *tiled = nullptr;
+    if (tile_width != 1) {
+        const uint32_t W = tile_width == 0 ? 2048u : tile_width;
+        const uint32_t R = tile_rows == 0 ? 1024u : tile_rows;
+        uint32_t flags = 0;
+        int st = get_tiled(m, which, W, R, tiled, &flags);
+        if (st == MLEM_OK) {
+            const TilePlan &lp = (*tiled)->plan;
+            const uint32_t want_groups = std::max(1u, (2560u + lp.nblk - 1) / std::max(1u, lp.nblk));
+            const uint32_t K = std::max(1u, (lp.nseg + want_groups - 1) / want_groups);
+            *plan = tile_plan(lp.rows, lp.cols, lp.nnz, W, R, K);
+            if (tiled_lds_bytes(*plan) > 65536) {
+                *tiled = nullptr;
+                st = set_err(MLEM_ERR_UNSUPPORTED, "tiling (width %u, rows %u) needs too much LDS", W, R);
+            }
+        }
+        if (st == MLEM_OK) {
+            *tile_s = secsize < 16 ? 8u : 16u;  // entries per work-item of a work unit
+            HIPCHK(part->alloc(std::max<size_t>(4, tiled_part_bytes(*plan))));
+            return MLEM_OK;
+        }
+        if (st != MLEM_ERR_UNSUPPORTED || tile_width != 0) return st;  // explicit width: no fallback
+        *tiled = nullptr;
+    }
+    if (int st = get_partition(m, which, secsize, partn)) return st;
+    HIPCHK(scratch->alloc(merge_scratch_bytes((*partn)->plan)));
+    return MLEM_OK;
+}
+
 // One MLEM iteration on r->stream (main.cu:331-448).
 hipError_t enqueue_iteration(mlem_recon *r, IterEvents *ev) {
     const mlem_matrix *m = r->m;
@@ -199,7 +293,10 @@ hipError_t enqueue_iteration(mlem_recon *r, IterEvents *ev) {
     EpiArgs fa{};
     fa.out = r->ratio.as<float>();
     fa.g = r->g.as<int32_t>();
-    if (r->opts.spmv == 0)
+    if (r->tfp)
+        e = launch_tiled_spmv(r->plan_fp, r->tfp->view, r->f.as<float>(), m->cols, r->tile_s_fp,
+                              r->part_fp.as<float>(), EPI_RATIO, fa, s);
+    else if (r->opts.spmv == 0)
         e = launch_merge_spmv(r->pfp->plan, r->pfp->buf.p, r->scratch_fp.p, m->rowptr.as<uint64_t>(),
                               m->col.as<uint32_t>(), m->val.as<float>(), r->f.as<float>(), EPI_RATIO, fa, s);
     else
@@ -212,7 +309,10 @@ hipError_t enqueue_iteration(mlem_recon *r, IterEvents *ev) {
     ba.norm = r->sens.as<float>();
     ba.f = r->f.as<float>();
     if (r->opts.using_trans == 0) {
-        if (r->opts.spmv == 0)
+        if (r->tbp)
+            e = launch_tiled_spmv(r->plan_bp, r->tbp->view, r->ratio.as<float>(), m->rows, r->tile_s_bp,
+                                  r->part_bp.as<float>(), EPI_UPDATE, ba, s);
+        else if (r->bp_merge)
             e = launch_merge_spmv(r->pbp->plan, r->pbp->buf.p, r->scratch_bp.p, m->rowptrT.as<uint64_t>(),
                                   m->colT.as<uint32_t>(), m->valT.as<float>(), r->ratio.as<float>(), EPI_UPDATE, ba,
                                   s);
@@ -650,6 +750,10 @@ int mlem_matrix_build_transpose(mlem_matrix *m) {
     m->has_T = true;
     for (auto it = m->parts.begin(); it != m->parts.end();)
         it = it->first.first == 1 ? m->parts.erase(it) : std::next(it);
+    for (auto it = m->tiles.begin(); it != m->tiles.end();)
+        it = std::get<0>(it->first) == 1 ? m->tiles.erase(it) : std::next(it);
+    for (auto it = m->tile_refused.begin(); it != m->tile_refused.end();)
+        it = std::get<0>(it->first) == 1 ? m->tile_refused.erase(it) : std::next(it);
     return MLEM_OK;
 }
 
@@ -722,6 +826,7 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
     ARGCHK(o.secsize_fp >= 1 && o.secsize_bp >= 1, "mlem_recon_create: section sizes must be >= 1");
     ARGCHK(o.fp_lanes == 0 || valid_lanes(o.fp_lanes), "mlem_recon_create: fp_lanes must be 0,4,8,16,32,64");
     ARGCHK(o.bp_lanes == 0 || valid_lanes(o.bp_lanes), "mlem_recon_create: bp_lanes must be 0,4,8,16,32,64");
+    ARGCHK(o.bp_spmv <= 2, "mlem_recon_create: bp_spmv must be 0 (as spmv), 1 (merge) or 2 (CSR-vector)");
     if (o.using_trans == 0 && !m->has_T)
         return set_err(MLEM_ERR_INVALID_ARGUMENT,
                        "mlem_recon_create: using_trans == 0 needs the transpose (upload it or build it)");
@@ -733,6 +838,7 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
     r->opts.secsize_bp = clamp_secsize(o.secsize_bp);
     r->lanes_fp = o.fp_lanes ? o.fp_lanes : auto_lanes(m->rows, m->nnz);
     r->lanes_bp = o.bp_lanes ? o.bp_lanes : auto_lanes(m->cols, m->nnz);
+    r->bp_merge = o.bp_spmv == 0 ? o.spmv == 0 : o.bp_spmv == 1;
     HIPCHK(hipStreamCreateWithFlags(&r->stream, hipStreamNonBlocking));
     struct StreamGuard {
         mlem_recon *r;
@@ -756,14 +862,18 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
         HIPCHK(r->bp.alloc(sizeof(float) * m->cols));
         HIPCHK(launch_fill(r->bp.as<float>(), 0.0f, m->cols, r->stream));
     }
-    if (o.spmv == 0) {
-        if (int st = get_partition(m, 0, r->opts.secsize_fp, &r->pfp)) return st;
-        HIPCHK(r->scratch_fp.alloc(merge_scratch_bytes(r->pfp->plan)));
-        if (o.using_trans == 0) {
-            if (int st = get_partition(m, 1, r->opts.secsize_bp, &r->pbp)) return st;
-            HIPCHK(r->scratch_bp.alloc(merge_scratch_bytes(r->pbp->plan)));
-        }
-    }
+    ARGCHK(o.tile_width <= 1 || valid_tile_width(o.tile_width),
+           "mlem_recon_create: tile_width must be 0 (auto), 1 (off), 2048, 4096 or 8192");
+    ARGCHK(o.tile_rows == 0 || valid_tile_rows(o.tile_rows),
+           "mlem_recon_create: tile_rows must be 0 (auto), 1024, 2048, 4096, 8192 or 16384");
+    if (o.spmv == 0)
+        if (int st = setup_merge_side(m, 0, o.tile_width, o.tile_rows, r->opts.secsize_fp, &r->tfp, &r->plan_fp,
+                                      &r->tile_s_fp, &r->part_fp, &r->pfp, &r->scratch_fp))
+            return st;
+    if (o.using_trans == 0 && r->bp_merge)
+        if (int st = setup_merge_side(m, 1, o.tile_width, o.tile_rows, r->opts.secsize_bp, &r->tbp, &r->plan_bp,
+                                      &r->tile_s_bp, &r->part_bp, &r->pbp, &r->scratch_bp))
+            return st;
     HIPCHK(r->llpart.alloc(sizeof(double) * std::max<uint32_t>(1, loglik_blocks(m->rows))));
     HIPCHK(hipStreamSynchronize(r->stream));
     guard.armed = false;

@@ -71,6 +71,41 @@ uint32_t loglik_blocks(uint32_t rows);
 hipError_t launch_loglik(const uint64_t *rowptr, const uint32_t *col, const float *val, const float *x,
                          const int32_t *g, uint32_t rows, double *part, hipStream_t s);
 
+// Blocked SpMV (tiled.hip): tiles of R rows x W columns with the W-wide
+// segment of x staged in LDS; workgroup (row block, group of K segments)
+// writes partial row sums, a reduction kernel adds them and applies the
+// epilogue.
+struct TilePlan {
+    uint32_t rows = 0, cols = 0;
+    uint64_t nnz = 0;
+    uint32_t W = 0, R = 0, nseg = 0, nblk = 0, K = 1, ngroups = 1;
+    uint64_t ntiles = 0;
+};
+struct TiledMatrix {
+    uint64_t *tile_off = nullptr;  // [ntiles+1] padded entry offsets
+    uint32_t *tile_nz = nullptr;   // [ntiles] entries per tile
+    float *tval = nullptr;         // values
+    uint16_t *tcol = nullptr;      // column - s*W
+    uint16_t *trow = nullptr;      // row - b*R
+};
+TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t K);
+bool valid_tile_width(uint32_t W);
+bool valid_tile_rows(uint32_t R);
+bool valid_tile_unit(uint32_t E);
+size_t tiled_matrix_bytes(const TilePlan &p);
+size_t tiled_part_bytes(const TilePlan &p);
+size_t tiled_lds_bytes(const TilePlan &p);
+size_t tile_build_scratch_bytes(const TilePlan &p);
+TiledMatrix tiled_matrix_view(const TilePlan &p, void *mem);
+// Builds the tiles from a CSR on the device and waits; *flags_host bit 1 =
+// a row is not sorted by column (then nothing is built).
+hipError_t launch_tile_build(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
+                             void *mem, void *scratch, uint32_t *flags_host, hipStream_t s);
+// E = entries per work-item of one work unit (8 or 16); part holds
+// tiled_part_bytes(p).
+hipError_t launch_tiled_spmv(const TilePlan &p, const TiledMatrix &tm, const float *x, uint32_t xlen, uint32_t E,
+                             float *part, Epi epi, const EpiArgs &ea, hipStream_t s);
+
 // Device transpose (stable counting sort): see transpose.hip.
 size_t transpose_scratch_bytes(uint32_t rows, uint32_t cols, uint64_t nnz);
 hipError_t launch_transpose(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr, const uint32_t *col,

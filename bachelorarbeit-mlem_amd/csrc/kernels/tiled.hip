@@ -1,0 +1,478 @@
+// Blocked SpMV with x segments staged in LDS: the high-bandwidth variant of
+// the merge-based kernels (calcFwProj_merge_based / calcBwProj_merge_based,
+// kernel.cu:52-105,190-286).
+//
+// Why: a gather x[col] that misses the 32 KiB L1 costs a 128-byte line fill,
+// and the L1 fill rate (about 64 bytes per clock per CU) -- not HBM -- caps
+// CSR kernels that gather from global memory near 2.6-3 TB/s on the 12 GB
+// scanner matrix, while streaming the same arrays runs at 6-7 TB/s.  Here the
+// matrix is cut into tiles of R rows x W columns; a tile is dense enough
+// (tens of thousands of entries) that copying its W-wide segment of x into
+// LDS once costs little, and every gather then reads LDS.
+//
+// Layout, built on the device from a CSR with column-sorted rows:
+//   tile t = (row block b, segment s), t = b*nseg + s, stored in (b, s)
+//   order, each tile padded to a multiple of 8 entries; inside a tile the
+//   entries sorted by (row, column), so every row still sums its entries in
+//   the original column order.  Per entry: value f32, column - s*W (u16),
+//   row - b*R (u16): 8 bytes, the same as CSR's f32 value + u32 column.
+//   tile_off u64[ntiles+1] (padded offsets), tile_nz u32[ntiles].
+//
+// Work: workgroup (b, g) takes the tiles (b, s) for s in segment group g.
+// A tile is cut into units of 256*E entries; work-item k owns the E
+// consecutive entries [kE, kE+E) of a unit (16-byte loads).  Row ends are
+// where the row key changes; rows split between work-items are joined by a
+// segmented scan over the workgroup, so each row's partial of a unit is
+// added exactly once, in unit order, to an LDS accumulator of the block's
+// rows.  The loads of the next unit (and, for a new tile, its x segment) are
+// issued before the current unit is computed.  Workgroup g writes its
+// partials to part[g][row]; a reduction kernel adds the groups in order and
+// applies the epilogue (ratio or update).  Bitwise reproducible.
+#include <rocprim/device/device_scan.hpp>
+
+#include <algorithm>
+
+#include "launch.hpp"
+
+namespace mlem {
+namespace {
+
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
+
+template <typename T>
+__device__ __forceinline__ T ldnt(const T *p) {
+    return __builtin_nontemporal_load(p);
+}
+
+constexpr uint32_t kNoKey = 0x10000u;  // past every u16 row key
+
+struct Unit {
+    uint32_t s, c, n;  // segment, unit in the tile, entries in the unit
+    bool valid;
+};
+
+template <int E>
+__device__ __forceinline__ Unit next_unit(const uint32_t *snz, uint32_t s_end, uint32_t s, uint32_t c) {
+    constexpr uint32_t UE = kWG * E;
+    Unit u;
+    if ((c + 1) * UE < snz[s]) {
+        u.s = s;
+        u.c = c + 1;
+    } else {
+        uint32_t t = s + 1;
+        while (t < s_end && snz[t] == 0) ++t;
+        u.s = t;
+        u.c = 0;
+    }
+    u.valid = u.s < s_end;
+    u.n = u.valid ? min(UE, snz[u.s] - u.c * UE) : 0u;
+    return u;
+}
+
+// snz / soff are indexed by segment; s_begin..s_end is this group.
+template <int E, int W>
+__global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32_t R, uint32_t nseg, uint32_t K,
+                                                           uint32_t xlen, const uint64_t *__restrict__ tile_off,
+                                                           const uint32_t *__restrict__ tile_nz,
+                                                           const float *__restrict__ tval,
+                                                           const uint16_t *__restrict__ tcol,
+                                                           const uint16_t *__restrict__ trow,
+                                                           const float *__restrict__ x, float *__restrict__ part) {
+    constexpr uint32_t UE = kWG * E;
+    constexpr int XR = W / kWG;
+    extern __shared__ __attribute__((aligned(16))) float smem[];
+    __shared__ float s_wv[kWG / 64];
+    __shared__ int s_wf[kWG / 64];
+    __shared__ uint32_t s_key0[kWG / 64 + 1];
+    float *xs = smem;                                        // [W]
+    float *acc = xs + W;                                     // [R]
+    uint32_t *snz = reinterpret_cast<uint32_t *>(acc + R);   // [K] (padded to even)
+    uint64_t *soff = reinterpret_cast<uint64_t *>(snz + ((K + 1) & ~1u));  // [K]
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint32_t ngroups = (nseg + K - 1) / K;
+    const uint32_t b = blockIdx.x / ngroups, g = blockIdx.x % ngroups;
+    const uint32_t sb = g * K, se = min(nseg, sb + K), ns = se - sb;
+    const uint64_t t0 = (uint64_t)b * nseg + sb;
+    for (uint32_t i = tid; i < R; i += kWG) acc[i] = 0.0f;
+    for (uint32_t s = tid; s < ns; s += kWG) {
+        snz[s] = tile_nz[t0 + s];
+        soff[s] = tile_off[t0 + s];
+    }
+    __syncthreads();
+
+    uint32_t s0 = 0;
+    while (s0 < ns && snz[s0] == 0) ++s0;
+    Unit cur{s0, 0, s0 < ns ? min(UE, snz[s0]) : 0u, s0 < ns};
+
+    f32x4 pv[E / 4];
+    u16x8 pc[E / 8], pr[E / 8];
+    float px[XR];
+    auto prefetch = [&](const Unit &u, uint32_t fallback_s) {
+        const uint32_t s = u.valid ? u.s : fallback_s;
+        uint64_t e = soff[s] + (uint64_t)(u.valid ? u.c : 0u) * UE + (uint64_t)tid * E;
+        if ((uint64_t)tid * E >= (u.valid ? u.n : 1u)) e = soff[s];  // stay inside the tile
+#pragma unroll
+        for (int m = 0; m < E / 4; ++m) pv[m] = ldnt(reinterpret_cast<const f32x4 *>(tval + e) + m);
+#pragma unroll
+        for (int m = 0; m < E / 8; ++m) {
+            pc[m] = ldnt(reinterpret_cast<const u16x8 *>(tcol + e) + m);
+            pr[m] = ldnt(reinterpret_cast<const u16x8 *>(trow + e) + m);
+        }
+        const uint32_t xb = (sb + s) * W;
+#pragma unroll
+        for (int k = 0; k < XR; ++k) {
+            uint32_t xi = xb + tid + kWG * k;
+            if (xi >= xlen) xi = xlen - 1;
+            px[k] = x[xi];
+        }
+    };
+
+    if (cur.valid) prefetch(cur, cur.s);
+    while (cur.valid) {
+        f32x4 cv[E / 4];
+        u16x8 cc[E / 8], cr[E / 8];
+        float cx[XR];
+#pragma unroll
+        for (int m = 0; m < E / 4; ++m) cv[m] = pv[m];
+#pragma unroll
+        for (int m = 0; m < E / 8; ++m) {
+            cc[m] = pc[m];
+            cr[m] = pr[m];
+        }
+#pragma unroll
+        for (int k = 0; k < XR; ++k) cx[k] = px[k];
+        const Unit nxt = next_unit<E>(snz, ns, cur.s, cur.c);
+        prefetch(nxt, cur.s);  // in flight while this unit is computed
+
+        // Keys of this work-item's entries; entries past n get kNoKey.
+        const uint32_t e0 = tid * E;
+        uint32_t key[E + 1];
+        float val[E];
+        uint32_t colv[E];
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            const bool ok = e0 + m < cur.n;
+            key[m] = ok ? (uint32_t)cr[m / 8][m % 8] : kNoKey;
+            val[m] = cv[m / 4][m % 4];
+            colv[m] = cc[m / 8][m % 8];
+        }
+        // First key of the next work-item (the key after my last entry).
+        const uint32_t k0 = key[0];
+        uint32_t nk = __shfl_down(k0, 1);
+        if (lane == 0) s_key0[w] = k0;
+        if (tid == 0) s_key0[kWG / 64] = kNoKey;
+        if (cur.c == 0) {  // a new tile: its x segment
+#pragma unroll
+            for (int k = 0; k < XR; ++k) xs[tid + kWG * k] = cx[k];
+        }
+        __syncthreads();
+        if (lane == 63) nk = s_key0[w + 1];
+        key[E] = nk;
+
+        // Walk my entries: a row ends where the key changes.
+        float a = 0.0f, head = 0.0f;
+        int has_end = 0;
+        uint32_t first_key = 0;
+#pragma unroll
+        for (int m = 0; m < E; ++m) {
+            if (key[m] != kNoKey) {
+                a += val[m] * xs[colv[m]];
+                if (key[m + 1] != key[m]) {
+                    if (has_end)
+                        acc[key[m]] += a;  // row started and ended inside this work-item
+                    else {
+                        head = a;
+                        first_key = key[m];
+                        has_end = 1;
+                    }
+                    a = 0.0f;
+                }
+            }
+        }
+        // Segmented scan of (has_end, carry): (f1,v1)(+)(f2,v2) = (f1|f2, f2 ? v2 : v1+v2).
+        float v = a;
+        int f = has_end;
+#pragma unroll
+        for (int o = 1; o < 64; o <<= 1) {
+            const float vu = __shfl_up(v, o);
+            const int fu = __shfl_up(f, o);
+            if (lane >= (uint32_t)o) {
+                if (!f) v = vu + v;
+                f |= fu;
+            }
+        }
+        if (lane == 63) {
+            s_wv[w] = v;
+            s_wf[w] = f;
+        }
+        __syncthreads();
+        float pvv = 0.0f;
+        for (uint32_t u = 0; u < w; ++u) pvv = s_wf[u] ? s_wv[u] : pvv + s_wv[u];
+        const float iv = f ? v : pvv + v;
+        float ev = __shfl_up(iv, 1);
+        if (lane == 0) ev = pvv;
+        if (has_end) acc[first_key] += ev + head;
+        __syncthreads();
+        cur = nxt;
+    }
+    // Partials of this segment group for the block's rows.
+    float *pg = part + (uint64_t)g * rows;
+    for (uint32_t i = tid; i < R; i += kWG) {
+        const uint64_t r = (uint64_t)b * R + i;
+        if (r < rows) pg[r] = acc[i];
+    }
+}
+
+template <int EPI>
+__global__ void blocked_reduce_kernel(const float *__restrict__ part, uint32_t ngroups, uint32_t rows, EpiArgs ea) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    float y = 0.0f;
+    for (uint32_t g = 0; g < ngroups; ++g) y += part[(uint64_t)g * rows + r];
+    if constexpr (EPI == EPI_PLAIN) {
+        ea.out[r] = y;
+    } else if constexpr (EPI == EPI_RATIO) {
+        ea.out[r] = (y != 0.0f) ? (float)ea.g[r] / y : y;
+    } else {
+        const float v = ea.f[r] * y;
+        const float n = ea.norm[r];
+        ea.f[r] = (n == 0.0f) ? v : v / n;
+    }
+}
+
+// ----------------------------------------------------------------- builder
+// cnt[r*nseg + s] = entries of row r in segment s.
+__global__ __launch_bounds__(kWG) void tile_count_kernel(const uint64_t *__restrict__ rowptr,
+                                                         const uint32_t *__restrict__ col, uint32_t rows,
+                                                         uint32_t nseg, uint32_t wshift,
+                                                         uint32_t *__restrict__ cnt) {
+    const uint32_t row = blockIdx.x * (kWG / 64) + threadIdx.x / 64;
+    const uint32_t lane = threadIdx.x % 64;
+    if (row >= rows) return;
+    for (uint64_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += 64)
+        atomicAdd(cnt + (uint64_t)row * nseg + (col[k] >> wshift), 1u);
+}
+
+// Per tile (one work-item per tile): rowpos[r*nseg+s] = entries of tile
+// (b, s) in the rows of block b before r; exact and padded tile sizes.
+__global__ void tile_rowpos_kernel(const uint32_t *__restrict__ cnt, uint32_t rows, uint32_t R, uint32_t nseg,
+                                   uint64_t ntiles, uint32_t *__restrict__ rowpos, uint32_t *__restrict__ tile_nz,
+                                   uint64_t *__restrict__ tile_padded) {
+    const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
+    if (t >= ntiles) return;
+    const uint32_t b = (uint32_t)(t / nseg), s = (uint32_t)(t % nseg);
+    uint32_t run = 0;
+    for (uint32_t i = 0; i < R; ++i) {
+        const uint64_t r = (uint64_t)b * R + i;
+        if (r >= rows) break;
+        rowpos[r * nseg + s] = run;
+        run += cnt[r * nseg + s];
+    }
+    tile_nz[t] = run;
+    tile_padded[t] = (run + 7u) & ~7u;
+}
+
+// Per row: exclusive prefix of its segment counts (in place).
+__global__ void row_segscan_kernel(uint32_t *__restrict__ cnt, uint32_t rows, uint32_t nseg) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    uint32_t *c = cnt + (uint64_t)r * nseg;
+    uint32_t run = 0;
+    for (uint32_t s = 0; s < nseg; ++s) {
+        const uint32_t n = c[s];
+        c[s] = run;
+        run += n;
+    }
+}
+
+__global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__restrict__ rowptr,
+                                                           const uint32_t *__restrict__ col,
+                                                           const float *__restrict__ val, uint32_t rows, uint32_t R,
+                                                           uint32_t nseg, uint32_t wshift,
+                                                           const uint32_t *__restrict__ segoff,
+                                                           const uint32_t *__restrict__ rowpos,
+                                                           const uint64_t *__restrict__ tile_off,
+                                                           float *__restrict__ tval, uint16_t *__restrict__ tcol,
+                                                           uint16_t *__restrict__ trow) {
+    const uint32_t row = blockIdx.x * (kWG / 64) + threadIdx.x / 64;
+    const uint32_t lane = threadIdx.x % 64;
+    if (row >= rows) return;
+    const uint32_t b = row / R, i = row % R;
+    const uint64_t rs = rowptr[row];
+    for (uint64_t k = rs + lane; k < rowptr[row + 1]; k += 64) {
+        const uint32_t c = col[k];
+        const uint32_t s = c >> wshift;
+        const uint64_t t = (uint64_t)b * nseg + s;
+        const uint64_t rsn = (uint64_t)row * nseg + s;
+        const uint64_t pos = tile_off[t] + rowpos[rsn] + (k - rs - segoff[rsn]);
+        tval[pos] = val[k];
+        tcol[pos] = (uint16_t)(c & ((1u << wshift) - 1u));
+        trow[pos] = (uint16_t)i;
+    }
+}
+
+// Rows must be sorted by column for the segment runs to be contiguous.
+__global__ __launch_bounds__(kWG) void rows_sorted_kernel(const uint64_t *__restrict__ rowptr,
+                                                          const uint32_t *__restrict__ col, uint32_t rows,
+                                                          uint32_t *__restrict__ bad) {
+    const uint32_t row = blockIdx.x * (kWG / 64) + threadIdx.x / 64;
+    const uint32_t lane = threadIdx.x % 64;
+    if (row >= rows) return;
+    for (uint64_t k = rowptr[row] + 1 + lane; k < rowptr[row + 1]; k += 64)
+        if (col[k] < col[k - 1]) atomicOr(bad, 2u);
+}
+
+inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
+constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
+
+size_t lds_bytes(uint32_t W, uint32_t R, uint32_t K) {
+    return ((size_t)W + R + (((size_t)K + 1) & ~size_t(1)) + 2 * (size_t)K) * 4;
+}
+
+template <int E, int W>
+hipError_t launch_w(const TilePlan &p, const TiledMatrix &tm, const float *x, uint32_t xlen, float *part,
+                    hipStream_t s) {
+    hipLaunchKernelGGL((blocked_spmv_kernel<E, W>), dim3(p.nblk * p.ngroups), dim3(kWG), lds_bytes(W, p.R, p.K), s,
+                       p.rows, p.R, p.nseg, p.K, xlen, tm.tile_off, tm.tile_nz, tm.tval, tm.tcol, tm.trow, x, part);
+    return hipGetLastError();
+}
+
+template <int E>
+hipError_t launch_e(const TilePlan &p, const TiledMatrix &tm, const float *x, uint32_t xlen, float *part,
+                    hipStream_t s) {
+    switch (p.W) {
+        case 2048: return launch_w<E, 2048>(p, tm, x, xlen, part, s);
+        case 4096: return launch_w<E, 4096>(p, tm, x, xlen, part, s);
+        default: return launch_w<E, 8192>(p, tm, x, xlen, part, s);
+    }
+}
+
+uint64_t padded_capacity(const TilePlan &p) { return p.nnz + 7 * p.ntiles + 16; }
+
+size_t scan_bytes(const TilePlan &p) {
+    size_t scan = 0;
+    (void)rocprim::exclusive_scan(nullptr, scan, (const uint64_t *)nullptr, (uint64_t *)nullptr, uint64_t(0),
+                                  (size_t)p.ntiles + 1, rocprim::plus<uint64_t>());
+    return scan;
+}
+
+}  // namespace
+
+TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t K) {
+    TilePlan p;
+    p.rows = rows;
+    p.cols = cols;
+    p.nnz = nnz;
+    p.W = W;
+    p.R = R;
+    p.nseg = (cols + W - 1) / W;
+    p.nblk = (rows + R - 1) / R;
+    p.K = std::max(1u, std::min(K, p.nseg));
+    p.ngroups = (p.nseg + p.K - 1) / p.K;
+    p.ntiles = (uint64_t)p.nseg * p.nblk;
+    return p;
+}
+
+bool valid_tile_width(uint32_t W) { return W == 2048 || W == 4096 || W == 8192; }
+bool valid_tile_rows(uint32_t R) { return R == 1024 || R == 2048 || R == 4096 || R == 8192 || R == 16384; }
+bool valid_tile_unit(uint32_t E) { return E == 8 || E == 16; }
+
+size_t tiled_lds_bytes(const TilePlan &p) { return lds_bytes(p.W, p.R, p.K); }
+
+size_t tiled_matrix_bytes(const TilePlan &p) {
+    const uint64_t cap = padded_capacity(p);
+    return align256(8 * (p.ntiles + 1)) + align256(4 * p.ntiles) + align256(4 * cap) + 2 * align256(2 * cap);
+}
+
+size_t tiled_part_bytes(const TilePlan &p) { return sizeof(float) * (size_t)p.ngroups * p.rows; }
+
+size_t tile_build_scratch_bytes(const TilePlan &p) {
+    return 2 * align256(4ull * p.rows * p.nseg) + align256(8 * (p.ntiles + 1)) + align256(scan_bytes(p)) + 256;
+}
+
+TiledMatrix tiled_matrix_view(const TilePlan &p, void *mem) {
+    const uint64_t cap = padded_capacity(p);
+    TiledMatrix t;
+    char *c = static_cast<char *>(mem);
+    t.tile_off = reinterpret_cast<uint64_t *>(c);
+    c += align256(8 * (p.ntiles + 1));
+    t.tile_nz = reinterpret_cast<uint32_t *>(c);
+    c += align256(4 * p.ntiles);
+    t.tval = reinterpret_cast<float *>(c);
+    c += align256(4 * cap);
+    t.tcol = reinterpret_cast<uint16_t *>(c);
+    c += align256(2 * cap);
+    t.trow = reinterpret_cast<uint16_t *>(c);
+    return t;
+}
+
+hipError_t launch_tile_build(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
+                             void *mem, void *scratch, uint32_t *flags_host, hipStream_t s) {
+    TiledMatrix tm = tiled_matrix_view(p, mem);
+    const uint64_t cap = padded_capacity(p);
+    char *c = static_cast<char *>(scratch);
+    uint32_t *cnt = reinterpret_cast<uint32_t *>(c);
+    c += align256(4ull * p.rows * p.nseg);
+    uint32_t *rowpos = reinterpret_cast<uint32_t *>(c);
+    c += align256(4ull * p.rows * p.nseg);
+    uint64_t *tile_padded = reinterpret_cast<uint64_t *>(c);
+    c += align256(8 * (p.ntiles + 1));
+    size_t sb = scan_bytes(p);
+    void *scan_tmp = c;
+    c += align256(sb);
+    uint32_t *bad = reinterpret_cast<uint32_t *>(c);
+    uint32_t wshift = 0;
+    while ((1u << wshift) < p.W) ++wshift;
+
+    hipError_t e;
+    if ((e = hipMemsetAsync(bad, 0, sizeof(uint32_t), s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(cnt, 0, 4ull * p.rows * p.nseg, s)) != hipSuccess) return e;
+    if (p.rows) {
+        hipLaunchKernelGGL(rows_sorted_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, col,
+                           p.rows, bad);
+        hipLaunchKernelGGL(tile_count_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, col,
+                           p.rows, p.nseg, wshift, cnt);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    // The scatter is only correct (and only in bounds) for sorted rows:
+    // check the flag first.
+    if ((e = hipMemcpyAsync(flags_host, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    if (*flags_host) return hipSuccess;
+    hipLaunchKernelGGL(tile_rowpos_kernel, dim3(blocks_for(p.ntiles, kWG)), dim3(kWG), 0, s, cnt, p.rows, p.R,
+                       p.nseg, p.ntiles, rowpos, tm.tile_nz, tile_padded);
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(tile_padded + p.ntiles, 0, sizeof(uint64_t), s)) != hipSuccess) return e;
+    if ((e = rocprim::exclusive_scan(scan_tmp, sb, tile_padded, tm.tile_off, uint64_t(0), (size_t)p.ntiles + 1,
+                                     rocprim::plus<uint64_t>(), s)) != hipSuccess)
+        return e;
+    // Padding entries: value 0 (their keys are masked by the entry count).
+    if ((e = hipMemsetAsync(tm.tval, 0, 4 * cap, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(tm.tcol, 0, 2 * cap, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(tm.trow, 0, 2 * cap, s)) != hipSuccess) return e;
+    if (p.rows) {
+        hipLaunchKernelGGL(row_segscan_kernel, dim3(blocks_for(p.rows, kWG)), dim3(kWG), 0, s, cnt, p.rows, p.nseg);
+        hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, col, val,
+                           p.rows, p.R, p.nseg, wshift, cnt, rowpos, tm.tile_off, tm.tval, tm.tcol, tm.trow);
+        if ((e = hipGetLastError()) != hipSuccess) return e;
+    }
+    return hipStreamSynchronize(s);
+}
+
+hipError_t launch_tiled_spmv(const TilePlan &p, const TiledMatrix &tm, const float *x, uint32_t xlen, uint32_t E,
+                             float *part, Epi epi, const EpiArgs &ea, hipStream_t s) {
+    if (p.rows == 0) return hipSuccess;
+    hipError_t e = (E == 8) ? launch_e<8>(p, tm, x, xlen, part, s) : launch_e<16>(p, tm, x, xlen, part, s);
+    if (e != hipSuccess) return e;
+    const dim3 grid(blocks_for(p.rows, kWG)), block(kWG);
+    switch (epi) {
+        case EPI_PLAIN: hipLaunchKernelGGL(blocked_reduce_kernel<EPI_PLAIN>, grid, block, 0, s, part, p.ngroups, p.rows, ea); break;
+        case EPI_RATIO: hipLaunchKernelGGL(blocked_reduce_kernel<EPI_RATIO>, grid, block, 0, s, part, p.ngroups, p.rows, ea); break;
+        default: hipLaunchKernelGGL(blocked_reduce_kernel<EPI_UPDATE>, grid, block, 0, s, part, p.ngroups, p.rows, ea); break;
+    }
+    return hipGetLastError();
+}
+
+}  // namespace mlem

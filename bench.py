@@ -1,0 +1,191 @@
+#!/usr/bin/env python3
+"""MLEM benchmark on one MI355X.
+
+A step is one MLEM iteration (forward projection + ratio, back projection +
+update) over the resident system matrix.  Default workload: the MADPET-II
+scale 3D scanner (two rings of 576 crystals, 662,976 LOR rows x 784,000
+voxels of 140x140x40, 16 jittered sub-rays per LOR), a seeded ellipsoid
+phantom and Poisson counts totalling 1e8, generated on the host from a seed
+(no data files exist for it).
+
+Prints one JSON line: iterations/s over the timed steps, the dominant
+kernel's achieved HBM bandwidth against the 8 TB/s peak (HIP events on the
+reconstruction's stream), and the FP32 OpenMP CPU loop (the oracle's CPU
+baseline) timed on a bounded number of iterations of the same workload.
+"""
+import argparse
+import json
+import os
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+for _p in (ROOT, os.path.join(ROOT, "bachelorarbeit-mlem_amd")):
+    if _p not in sys.path:
+        sys.path.insert(0, _p)
+
+import numpy as np  # noqa: E402
+
+METRIC = "MLEM iters/s + HBM GB/s (12GB matrix); batched recon/s, p99 latency"
+HBM_PEAK_GBS = 8000.0
+
+
+def log(msg):
+    print(f"[bench {time.strftime('%H:%M:%S')}] {msg}", file=sys.stderr, flush=True)
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return "unknown"
+
+
+def workload(name, threads):
+    import mlem_amd as M
+    if name == "c2":
+        geo, counts, shapes, label = M.RING3D_1152(), 1e8, 8, "3D MLEM FP32, 2 rings x 576 crystals, 140x140x40 voxels"
+    elif name == "mid":
+        geo, counts, shapes, label = M.RING3D_576_MID(), 1e7, 6, "3D MLEM FP32, 2 rings x 288 crystals, 96x96x24 voxels"
+    elif name == "c1":
+        geo, counts, shapes, label = M.RING2D_288(), 1e6, 5, "2D MLEM FP32, 288-crystal ring, 64x64 voxels"
+    else:
+        raise SystemExit(f"unknown workload {name}")
+    t = time.time()
+    A = M.gen_system_matrix(geo, seed=1, threads=threads)
+    log(f"system matrix {A.rows} x {A.cols}, nnz={A.nnz} ({A.nnz * 8 / 1e9:.2f} GB as f32+u32) in {time.time() - t:.1f} s")
+    x_true = M.gen_phantom(geo, seed=1, n_shapes=shapes, a_min=0.5, a_max=4.0)
+    g = M.gen_sinogram(A, x_true, counts, seed=1, threads=threads)
+    return geo, A, g, label
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100, help="timed MLEM iterations")
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="c2", choices=["c2", "mid", "c1"])
+    ap.add_argument("--fp", default="merge", choices=["merge", "vector"])
+    ap.add_argument("--bp", default="vector", choices=["merge", "vector"])
+    ap.add_argument("--secsize", type=int, default=8)
+    ap.add_argument("--tile", type=int, default=0, help="merge-path tiling: 0 auto, 1 off, else segment width")
+    ap.add_argument("--graph", type=int, default=0)
+    ap.add_argument("--threads", type=int, default=0, help="host threads (0: all)")
+    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
+    ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--calibrate-copy", action="store_true",
+                    help="also run a 1 GiB device-to-device copy (HBM counter calibration)")
+    args = ap.parse_args()
+
+    world = int(os.environ.get("WORLD_SIZE", "1"))
+    if args.gpus > 1 or world > 1:
+        print(f"bench.py: this engine keeps the whole system matrix resident on ONE GPU; "
+              f"--gpus {max(args.gpus, world)} is not supported (run with --gpus 1)", file=sys.stderr)
+        sys.exit(2)
+
+    import mlem_amd as M
+
+    threads = args.threads or min(16, os.cpu_count() or 1)
+    name, mem, cus = M.device_info(0)
+    log(f"device {name}, {mem / 2**30:.0f} GiB, {cus} CUs; host {cpu_model()}, {threads} threads")
+    geo, A, g, label = workload(args.workload, threads)
+
+    t = time.time()
+    m = M.ResidentMatrix(A)
+    log(f"upload A {time.time() - t:.2f} s")
+    t = time.time()
+    m.build_transpose()
+    log(f"device transpose (stable counting sort) {time.time() - t:.2f} s")
+    t = time.time()
+    s32, _ = m.sensitivity()
+    log(f"FP64 sensitivity {time.time() - t:.2f} s")
+    f0, init = M.initial_image(g, A.rows, s32)
+
+    # FP and BP kernels chosen independently: spmv selects FP, bp_* selects BP.
+    fp_merge = args.fp == "merge"
+    o = M.opts(spmv=0 if fp_merge else 1, using_trans=0, secsize_fp=args.secsize, secsize_bp=args.secsize,
+               profile=0 if args.graph else 1, use_graph=args.graph, tile_width=args.tile)
+    o.bp_spmv = 1 if args.bp == "merge" else 2
+    with M.Recon(m, g, f0, sens=s32, options=o) as r:
+        r.iterate(args.warmup)
+        r.sync()
+        r.reset_times()
+        M.device_sync()
+        t0 = time.perf_counter()
+        r.iterate(args.steps)
+        r.sync()
+        M.device_sync()
+        t1 = time.perf_counter()
+        fp_ms, bp_ms, n_timed = r.kernel_times()
+        img = r.image()
+    elapsed = t1 - t0
+    its = args.steps / elapsed
+    rows, cols, nnz = A.rows, A.cols, A.nnz
+    b_fp = 8 * nnz + 8 * (rows + 1) + 4 * cols + 8 * rows          # A, rowptr, x, g, ratio write
+    b_bp = 8 * nnz + 8 * (cols + 1) + 4 * rows + 12 * cols          # A^T, rowptr, ratio, f r/w, sens
+    b_iter = b_fp + b_bp
+    log(f"{args.steps} iterations in {elapsed * 1e3:.1f} ms: {its:.1f} it/s, {b_iter * its / 1e9:.0f} GB/s; "
+        f"sum f = {float(np.sum(img, dtype=np.float32)):.6g}")
+
+    roofline = None
+    if n_timed:
+        fp_avg, bp_avg = fp_ms / n_timed, bp_ms / n_timed
+        fp_gbs, bp_gbs = b_fp / (fp_avg * 1e-3) / 1e9, b_bp / (bp_avg * 1e-3) / 1e9
+        log(f"FP ({args.fp}+ratio) {fp_avg:.3f} ms {fp_gbs:.0f} GB/s; BP ({args.bp}+update) {bp_avg:.3f} ms "
+            f"{bp_gbs:.0f} GB/s")
+        dom_fp = fp_avg >= bp_avg
+        ach = fp_gbs if dom_fp else bp_gbs
+        roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+                    "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
+                    "kernel": (f"FP {args.fp}-path SpMV + ratio" if dom_fp else f"BP {args.bp} SpMV on A^T + update"),
+                    "algorithmic_bytes_per_launch": b_fp if dom_fp else b_bp,
+                    "avg_launch_ms": round(fp_avg if dom_fp else bp_avg, 4),
+                    "fp": {"ms": round(fp_avg, 4), "gbs": round(fp_gbs, 1), "bytes": b_fp},
+                    "bp": {"ms": round(bp_avg, 4), "gbs": round(bp_gbs, 1), "bytes": b_bp}}
+
+    if args.calibrate_copy:
+        n = 1 << 28
+        a = M.DeviceArray(n, np.float32)
+        b = M.DeviceArray(n, np.float32)
+        M.fill_f32(a, 1.0)
+        for _ in range(3):
+            M.hip_lib().mlem_memcpy_d2d(b.ptr, a.ptr, a.nbytes, None)
+        M.device_sync()
+        log("copy calibration: 3 x 1 GiB device-to-device copies")
+
+    cpu = None
+    if not args.no_cpu:
+        import oracle as O
+        AT = m.download_transpose()
+        n_it = 1
+        _, secs = O.mlem_f32_omp(A, AT, g, s32, f0, 1, threads)  # warm-up + size the sample
+        n_it = max(1, int(args.cpu_seconds / max(secs, 1e-6)))
+        _, secs = O.mlem_f32_omp(A, AT, g, s32, f0, n_it, threads)
+        cpu = {"value": round(n_it / secs, 4), "unit": "iters/s", "cores": threads, "kind": "port",
+               "sample": f"{n_it} FP32 MLEM iterations of the same workload on {threads} threads of {cpu_model()} "
+                         f"({secs:.1f} s), OpenMP row-parallel FP on A and BP on A^T (oracle_mlem_f32_omp)"}
+        log(f"CPU baseline {cpu['value']} it/s ({n_it} iterations, {secs:.1f} s)")
+    m.close()
+
+    out = {
+        "metric": METRIC, "value": round(its, 3), "unit": "iters/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 4), "higher_is_better": True,
+        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "hbm_gbs": round(b_iter * its / 1e9, 1),
+        "config": {"workload": label, "rows": rows, "cols": cols, "nnz": nnz,
+                   "matrix_gb": round(8 * nnz / 1e9, 2), "bytes_per_iteration": b_iter,
+                   "fp": f"{args.fp}" + (f" (S={args.secsize})" if args.fp == "merge" else ""),
+                   "bp": f"{args.bp} on A^T" + (f" (S={args.secsize})" if args.bp == "merge" else ""),
+                   "tile_width": args.tile, "graph": bool(args.graph),
+                   "parallelism": "single GPU, matrix and transpose resident"},
+        "roofline": roofline, "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -89,9 +89,10 @@ int mlem_update(const float *f, const float *norm, float *bwproj,
 /* Merge-path SpMV y = A x (calcFwProj_merge_based / calcBwProj_merge_based,
  * kernel.cu:52-105,190-286).  secsize = merge-path items per thread
  * (argv[5]/argv[6]); each 256-thread workgroup owns 256*secsize items.
- * Supported secsize: 1..64.  The workspace holds the tile partition and the
+ * Supported secsize: 1..32.  The workspace holds the tile partition and the
  * cross-tile carries; its size comes from mlem_merge_workspace_bytes().
- * y is f32[rows] and is overwritten (no pre-zero needed). */
+ * y is f32[rows] and is overwritten (no pre-zero needed).  col and val must
+ * be 16-byte aligned (hipMalloc'd arrays are), as for the CSR-vector calls. */
 size_t mlem_merge_workspace_bytes(uint32_t rows, uint64_t nnz,
                                   uint32_t secsize);
 int mlem_fp_merge(const uint64_t *rowptr, const uint32_t *col,
@@ -177,10 +178,23 @@ typedef struct mlem_opts {
     uint32_t bp_lanes;    /* CSR-vector lanes per row for BP (0 = auto)      */
     uint32_t profile;     /* 1: time FP and BP with HIP events per iteration */
     uint32_t use_graph;   /* 1: replay iterations from a captured hipGraph   */
+    uint32_t bp_spmv;     /* BP algorithm: 0 = as spmv (argv[8] picks both,
+                             as in the reference), 1 = merge-based,
+                             2 = CSR-vector                                  */
+    uint32_t tile_width;  /* merge-based kernels: 0 = blocked (segments of
+                             2048 columns) with x segments staged in LDS
+                             when the matrix allows it (rows sorted by
+                             column), else the plain merge path;
+                             1 = plain merge path; 2048, 4096 or 8192 =
+                             blocked with that segment width (an error if
+                             the matrix does not allow it)                   */
+    uint32_t tile_rows;   /* rows per block of the blocked kernels: 0 = 1024,
+                             or 1024, 2048, 4096, 8192, 16384                */
 } mlem_opts;
 
-/* Fill opts with the reference's usual defaults (merge FP, transposed BP,
- * secsize 8, auto lanes, no profiling, no graph). */
+/* Fill opts with the defaults: merge-based FP and BP (argv[8] = 0),
+ * transposed BP, secsize 8, auto lanes, BP as spmv, automatic tiling, no
+ * profiling, no graph. */
 void mlem_opts_default(mlem_opts *opts);
 
 /* Upload A (and A^T if rowptrT != NULL) from host arrays to `device` and keep

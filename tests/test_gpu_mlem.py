@@ -134,3 +134,68 @@ def test_using_trans_requires_transpose(c1):
         # scatter path works without a transpose
         img = M.mlem_run(m, c1["g"], np.ones(c1["A"].cols, np.float32), 2, options=M.opts(using_trans=1, spmv=1))
         assert np.isfinite(img).all()
+
+
+@pytest.mark.parametrize("width,rows", [(2048, 1024), (2048, 8192), (4096, 1024), (4096, 4096), (4096, 8192),
+                                        (8192, 1024), (8192, 4096)])
+def test_blocked_merge_path_vs_fp64_oracle(c1, resident, ref50, width, rows):
+    """Merge-based FP and BP through the LDS-blocked kernels, every segment width and block size."""
+    for secsize in (8, 16):
+        print(f"blocked W={width} R={rows} S={secsize}", flush=True)
+        o = M.opts(spmv=0, using_trans=0, secsize_fp=secsize, secsize_bp=secsize, tile_width=width, tile_rows=rows)
+        img = M.mlem_run(resident, c1["g"], ref50["f0"], 50, sens=ref50["s32"], options=o)
+        assert rel_l2(img, ref50["ref"]) <= REL_L2
+        img2 = M.mlem_run(resident, c1["g"], ref50["f0"], 50, sens=ref50["s32"], options=o)
+        assert np.array_equal(img.view(np.uint32), img2.view(np.uint32))
+
+
+def test_blocked_edge_matrices():
+    for kind in ["empty_rows", "one_huge_row", "heavy", "dups", "all_empty", "single_row"]:
+        A = make(kind, seed=6)
+        AT = M.transpose(A)
+        rng = np.random.default_rng(5)
+        g = rng.integers(0, 30, A.rows).astype(np.int32)
+        with M.ResidentMatrix(A, AT) as m:
+            s32, _ = m.sensitivity()
+            f0 = np.ones(A.cols, np.float32)
+            ref = O.mlem_f64(A.rowptr, A.col, A.val, A.rows, A.cols, g, s32.astype(np.float64), f0, 8)
+            for width, rows in ((2048, 1024), (4096, 4096)):
+                print(f"blocked edge {kind} W={width} R={rows}", flush=True)
+                o = M.opts(spmv=0, using_trans=0, secsize_fp=8, secsize_bp=16, tile_width=width, tile_rows=rows)
+                img = M.mlem_run(m, g, f0, 8, sens=s32, options=o)
+                assert rel_l2(img, ref) <= REL_L2 or np.linalg.norm(ref) == 0
+
+
+def test_blocked_lds_limit_is_an_error(c1, resident, ref50):
+    # 16384 rows x 8192 columns per tile would need 96 KiB of LDS.
+    with pytest.raises(M.MlemError):
+        M.mlem_run(resident, c1["g"], ref50["f0"], 1, options=M.opts(spmv=0, tile_width=8192, tile_rows=16384))
+
+
+def test_tiling_refused_and_automatic_fallback():
+    # Unsorted rows: tiles are refused; automatic mode falls back to the
+    # plain merge path, an explicit width is an error.
+    A = make("ragged", seed=2)
+    col = A.col.copy()
+    for r in range(0, A.rows, 3):
+        b, e = int(A.rowptr[r]), int(A.rowptr[r + 1])
+        col[b:e] = col[b:e][::-1]
+    U = M.Csr(A.rows, A.cols, A.rowptr, col, A.val)
+    g = np.random.default_rng(1).integers(0, 20, A.rows).astype(np.int32)
+    f0 = np.ones(A.cols, np.float32)
+    with M.ResidentMatrix(U, M.transpose(U)) as m:
+        s32, _ = m.sensitivity()
+        ref = O.mlem_f64(U.rowptr, U.col, U.val, U.rows, U.cols, g, s32.astype(np.float64), f0, 5)
+        img = M.mlem_run(m, g, f0, 5, sens=s32, options=M.opts(spmv=0, tile_width=0))
+        assert rel_l2(img, ref) <= REL_L2
+        with pytest.raises(M.MlemError):
+            M.mlem_run(m, g, f0, 5, sens=s32, options=M.opts(spmv=0, bp_spmv=2, tile_width=4096))
+    # A dense block (tiles of more than 65536 entries) is fine.
+    D = __import__("matrices").from_lengths(np.full(1024, 700), 2048, seed=3)
+    g = np.random.default_rng(2).integers(1, 9, D.rows).astype(np.int32)
+    with M.ResidentMatrix(D, M.transpose(D)) as m:
+        s32, _ = m.sensitivity()
+        f0 = np.ones(D.cols, np.float32)
+        ref = O.mlem_f64(D.rowptr, D.col, D.val, D.rows, D.cols, g, s32.astype(np.float64), f0, 3)
+        img = M.mlem_run(m, g, f0, 3, sens=s32, options=M.opts(spmv=0, tile_width=2048, tile_rows=1024))
+        assert rel_l2(img, ref) <= REL_L2
