@@ -265,7 +265,10 @@ int setup_merge_side(mlem_matrix *m, int which, uint32_t tile_width, uint32_t ti
             const uint32_t want_groups = std::max(1u, (2560u + lp.nblk - 1) / std::max(1u, lp.nblk));
             const uint32_t K = std::max(1u, (lp.nseg + want_groups - 1) / want_groups);
             *plan = tile_plan(lp.rows, lp.cols, lp.nnz, W, R, K);
-            if (tiled_lds_bytes(*plan) > 65536) {
+            int dev = 0, lds_max = 65536;
+            if (hipGetDevice(&dev) == hipSuccess)
+                (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+            if (tiled_lds_bytes(*plan) + 4096 > (size_t)lds_max) {
                 *tiled = nullptr;
                 st = set_err(MLEM_ERR_UNSUPPORTED, "tiling (width %u, rows %u) needs too much LDS", W, R);
             }
@@ -847,9 +850,14 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
             if (armed && r->stream) (void)hipStreamDestroy(r->stream);
         }
     } guard{r.get()};
+    // f and ratio are the gathered vectors of FP / BP: padded with zeros to
+    // whole segments of the blocked kernels.
+    auto padded = [](size_t n) { return (n + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad; };
     HIPCHK(r->g.alloc(sizeof(int32_t) * std::max<size_t>(1, m->rows)));
-    HIPCHK(r->ratio.alloc(sizeof(float) * std::max<size_t>(1, m->rows)));
-    HIPCHK(r->f.alloc(sizeof(float) * m->cols));
+    HIPCHK(r->ratio.alloc(sizeof(float) * padded(m->rows)));
+    HIPCHK(r->f.alloc(sizeof(float) * padded(m->cols)));
+    HIPCHK(launch_fill(r->ratio.as<float>(), 0.0f, padded(m->rows), r->stream));
+    HIPCHK(launch_fill(r->f.as<float>(), 0.0f, padded(m->cols), r->stream));
     HIPCHK(r->sens.alloc(sizeof(float) * m->cols));
     if (m->rows) HIPCHK(hipMemcpyAsync(r->g.p, g, sizeof(int32_t) * m->rows, hipMemcpyHostToDevice, r->stream));
     HIPCHK(hipMemcpyAsync(r->f.p, f0, sizeof(float) * m->cols, hipMemcpyHostToDevice, r->stream));
@@ -863,9 +871,9 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
         HIPCHK(launch_fill(r->bp.as<float>(), 0.0f, m->cols, r->stream));
     }
     ARGCHK(o.tile_width <= 1 || valid_tile_width(o.tile_width),
-           "mlem_recon_create: tile_width must be 0 (auto), 1 (off), 2048, 4096 or 8192");
+           "mlem_recon_create: tile_width must be 0 (auto), 1 (off), 1024, 2048, 4096 or 8192");
     ARGCHK(o.tile_rows == 0 || valid_tile_rows(o.tile_rows),
-           "mlem_recon_create: tile_rows must be 0 (auto), 1024, 2048, 4096, 8192 or 16384");
+           "mlem_recon_create: tile_rows must be 0 (auto), 512, 1024, 2048, 4096, 8192 or 16384");
     if (o.spmv == 0)
         if (int st = setup_merge_side(m, 0, o.tile_width, o.tile_rows, r->opts.secsize_fp, &r->tfp, &r->plan_fp,
                                       &r->tile_s_fp, &r->part_fp, &r->pfp, &r->scratch_fp))

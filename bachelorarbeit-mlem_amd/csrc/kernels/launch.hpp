@@ -75,6 +75,7 @@ hipError_t launch_loglik(const uint64_t *rowptr, const uint32_t *col, const floa
 // segment of x staged in LDS; workgroup (row block, group of K segments)
 // writes partial row sums, a reduction kernel adds them and applies the
 // epilogue.
+constexpr uint32_t kTileXPad = 8192;  // pad gathered vectors to a multiple of this
 struct TilePlan {
     uint32_t rows = 0, cols = 0;
     uint64_t nnz = 0;
@@ -102,7 +103,8 @@ TiledMatrix tiled_matrix_view(const TilePlan &p, void *mem);
 hipError_t launch_tile_build(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
                              void *mem, void *scratch, uint32_t *flags_host, hipStream_t s);
 // E = entries per work-item of one work unit (8 or 16); part holds
-// tiled_part_bytes(p).
+// tiled_part_bytes(p).  x must be readable (and finite) up to nseg * W
+// floats: callers pad their vectors to whole segments (kTileXPad).
 hipError_t launch_tiled_spmv(const TilePlan &p, const TiledMatrix &tm, const float *x, uint32_t xlen, uint32_t E,
                              float *part, Epi epi, const EpiArgs &ea, hipStream_t s);
 

@@ -21,10 +21,11 @@
 // Work: workgroup (b, g) takes the tiles (b, s) for s in segment group g.
 // A tile is cut into units of 256*E entries; work-item k owns the E
 // consecutive entries [kE, kE+E) of a unit (16-byte loads).  Row ends are
-// where the row key changes; rows split between work-items are joined by a
-// segmented scan over the workgroup, so each row's partial of a unit is
-// added exactly once, in unit order, to an LDS accumulator of the block's
-// rows.  The loads of the next unit (and, for a new tile, its x segment) are
+// where the row key changes.  A row split between work-items is finished by
+// the work-item holding its last entry, which adds the carries of the
+// work-items before it (nearest first, back to the previous row end) from
+// LDS; so each row's partial of a unit is added exactly once, in unit order,
+// to an LDS accumulator of the block's rows.  The loads of the next unit (and, for a new tile, its x segment) are
 // issued before the current unit is computed.  Workgroup g writes its
 // partials to part[g][row]; a reduction kernel adds the groups in order and
 // applies the epilogue (ratio or update).  Bitwise reproducible.
@@ -82,15 +83,15 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32
     constexpr uint32_t UE = kWG * E;
     constexpr int XR = W / kWG;
     extern __shared__ __attribute__((aligned(16))) float smem[];
-    __shared__ float s_wv[kWG / 64];
-    __shared__ int s_wf[kWG / 64];
-    __shared__ uint32_t s_key0[kWG / 64 + 1];
+    __shared__ uint32_t s_k0[kWG + 1];  // first key of every work-item (+ sentinel)
+    __shared__ float s_cf[kWG];         // carry after the last row end
+    __shared__ uint32_t s_hf[kWG];      // work-item saw a row end
     float *xs = smem;                                        // [W]
     float *acc = xs + W;                                     // [R]
     uint32_t *snz = reinterpret_cast<uint32_t *>(acc + R);   // [K] (padded to even)
     uint64_t *soff = reinterpret_cast<uint64_t *>(snz + ((K + 1) & ~1u));  // [K]
 
-    const uint32_t tid = threadIdx.x, lane = tid & 63u, w = tid >> 6;
+    const uint32_t tid = threadIdx.x;
     const uint32_t ngroups = (nseg + K - 1) / K;
     const uint32_t b = blockIdx.x / ngroups, g = blockIdx.x % ngroups;
     const uint32_t sb = g * K, se = min(nseg, sb + K), ns = se - sb;
@@ -100,6 +101,7 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32
         snz[s] = tile_nz[t0 + s];
         soff[s] = tile_off[t0 + s];
     }
+    if (tid == 0) s_k0[kWG] = kNoKey;
     __syncthreads();
 
     uint32_t s0 = 0;
@@ -108,7 +110,7 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32
 
     f32x4 pv[E / 4];
     u16x8 pc[E / 8], pr[E / 8];
-    float px[XR];
+    f32x4 px[XR / 4];
     auto prefetch = [&](const Unit &u, uint32_t fallback_s) {
         const uint32_t s = u.valid ? u.s : fallback_s;
         uint64_t e = soff[s] + (uint64_t)(u.valid ? u.c : 0u) * UE + (uint64_t)tid * E;
@@ -120,20 +122,17 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32
             pc[m] = ldnt(reinterpret_cast<const u16x8 *>(tcol + e) + m);
             pr[m] = ldnt(reinterpret_cast<const u16x8 *>(trow + e) + m);
         }
-        const uint32_t xb = (sb + s) * W;
+        // x is padded to whole segments (xlen >= nseg * W): no clamping.
+        const float *xb = x + (uint64_t)(sb + s) * W;
 #pragma unroll
-        for (int k = 0; k < XR; ++k) {
-            uint32_t xi = xb + tid + kWG * k;
-            if (xi >= xlen) xi = xlen - 1;
-            px[k] = x[xi];
-        }
+        for (int k = 0; k < XR / 4; ++k) px[k] = *(reinterpret_cast<const f32x4 *>(xb) + tid + kWG * k);
     };
 
     if (cur.valid) prefetch(cur, cur.s);
     while (cur.valid) {
         f32x4 cv[E / 4];
         u16x8 cc[E / 8], cr[E / 8];
-        float cx[XR];
+        f32x4 cx[XR / 4];
 #pragma unroll
         for (int m = 0; m < E / 4; ++m) cv[m] = pv[m];
 #pragma unroll
@@ -142,81 +141,62 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32
             cr[m] = pr[m];
         }
 #pragma unroll
-        for (int k = 0; k < XR; ++k) cx[k] = px[k];
+        for (int k = 0; k < XR / 4; ++k) cx[k] = px[k];
         const Unit nxt = next_unit<E>(snz, ns, cur.s, cur.c);
         prefetch(nxt, cur.s);  // in flight while this unit is computed
 
         // Keys of this work-item's entries; entries past n get kNoKey.
         const uint32_t e0 = tid * E;
         uint32_t key[E + 1];
-        float val[E];
-        uint32_t colv[E];
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const bool ok = e0 + m < cur.n;
-            key[m] = ok ? (uint32_t)cr[m / 8][m % 8] : kNoKey;
-            val[m] = cv[m / 4][m % 4];
-            colv[m] = cc[m / 8][m % 8];
-        }
-        // First key of the next work-item (the key after my last entry).
-        const uint32_t k0 = key[0];
-        uint32_t nk = __shfl_down(k0, 1);
-        if (lane == 0) s_key0[w] = k0;
-        if (tid == 0) s_key0[kWG / 64] = kNoKey;
+        for (int m = 0; m < E; ++m) key[m] = (e0 + m < cur.n) ? (uint32_t)cr[m / 8][m % 8] : kNoKey;
+        s_k0[tid] = key[0];
         if (cur.c == 0) {  // a new tile: its x segment
 #pragma unroll
-            for (int k = 0; k < XR; ++k) xs[tid + kWG * k] = cx[k];
+            for (int k = 0; k < XR / 4; ++k) *(reinterpret_cast<f32x4 *>(xs) + tid + kWG * k) = cx[k];
         }
         __syncthreads();
-        if (lane == 63) nk = s_key0[w + 1];
-        key[E] = nk;
+        key[E] = s_k0[tid + 1];
 
-        // Walk my entries: a row ends where the key changes.
-        float a = 0.0f, head = 0.0f;
-        int has_end = 0;
-        uint32_t first_key = 0;
+        // Products first (all LDS gathers in flight together; padding and
+        // masked entries read a valid column and are zeroed).
+        float prod[E];
 #pragma unroll
         for (int m = 0; m < E; ++m) {
-            if (key[m] != kNoKey) {
-                a += val[m] * xs[colv[m]];
-                if (key[m + 1] != key[m]) {
-                    if (has_end)
-                        acc[key[m]] += a;  // row started and ended inside this work-item
-                    else {
-                        head = a;
-                        first_key = key[m];
-                        has_end = 1;
-                    }
-                    a = 0.0f;
-                }
-            }
+            const float p = cv[m / 4][m % 4] * xs[cc[m / 8][m % 8]];
+            prod[m] = key[m] != kNoKey ? p : 0.0f;
         }
-        // Segmented scan of (has_end, carry): (f1,v1)(+)(f2,v2) = (f1|f2, f2 ? v2 : v1+v2).
-        float v = a;
-        int f = has_end;
+        // Walk my entries: a row ends where the key changes.
+        float a = 0.0f, head = 0.0f;
+        uint32_t has_end = 0, first_key = 0;
 #pragma unroll
-        for (int o = 1; o < 64; o <<= 1) {
-            const float vu = __shfl_up(v, o);
-            const int fu = __shfl_up(f, o);
-            if (lane >= (uint32_t)o) {
-                if (!f) v = vu + v;
-                f |= fu;
+        for (int m = 0; m < E; ++m) {
+            a += prod[m];
+            const bool end = (key[m] != key[m + 1]) && (key[m] != kNoKey);
+            if (end && has_end) atomicAdd(acc + key[m], a);  // the row's only contribution in this unit
+            const bool first = end && !has_end;
+            head = first ? a : head;
+            first_key = first ? key[m] : first_key;
+            has_end |= end ? 1u : 0u;
+            a = end ? 0.0f : a;
+        }
+        s_cf[tid] = a;
+        s_hf[tid] = has_end;
+        __syncthreads();
+        // The first row ending here may have started in earlier work-items:
+        // add their carries, nearest first, back to the last one with a row
+        // end (whose carry belongs to this row too).
+        if (has_end) {
+            float cin = 0.0f;
+            for (int jj = (int)tid - 1; jj >= 0; --jj) {
+                cin += s_cf[jj];
+                if (s_hf[jj]) break;
             }
+            atomicAdd(acc + first_key, cin + head);
         }
-        if (lane == 63) {
-            s_wv[w] = v;
-            s_wf[w] = f;
-        }
-        __syncthreads();
-        float pvv = 0.0f;
-        for (uint32_t u = 0; u < w; ++u) pvv = s_wf[u] ? s_wv[u] : pvv + s_wv[u];
-        const float iv = f ? v : pvv + v;
-        float ev = __shfl_up(iv, 1);
-        if (lane == 0) ev = pvv;
-        if (has_end) acc[first_key] += ev + head;
-        __syncthreads();
         cur = nxt;
     }
+    __syncthreads();
     // Partials of this segment group for the block's rows.
     float *pg = part + (uint64_t)g * rows;
     for (uint32_t i = tid; i < R; i += kWG) {
@@ -343,6 +323,7 @@ template <int E>
 hipError_t launch_e(const TilePlan &p, const TiledMatrix &tm, const float *x, uint32_t xlen, float *part,
                     hipStream_t s) {
     switch (p.W) {
+        case 1024: return launch_w<E, 1024>(p, tm, x, xlen, part, s);
         case 2048: return launch_w<E, 2048>(p, tm, x, xlen, part, s);
         case 4096: return launch_w<E, 4096>(p, tm, x, xlen, part, s);
         default: return launch_w<E, 8192>(p, tm, x, xlen, part, s);
@@ -375,8 +356,8 @@ TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint3
     return p;
 }
 
-bool valid_tile_width(uint32_t W) { return W == 2048 || W == 4096 || W == 8192; }
-bool valid_tile_rows(uint32_t R) { return R == 1024 || R == 2048 || R == 4096 || R == 8192 || R == 16384; }
+bool valid_tile_width(uint32_t W) { return W == 1024 || W == 2048 || W == 4096 || W == 8192; }
+bool valid_tile_rows(uint32_t R) { return R == 512 || R == 1024 || R == 2048 || R == 4096 || R == 8192 || R == 16384; }
 bool valid_tile_unit(uint32_t E) { return E == 8 || E == 16; }
 
 size_t tiled_lds_bytes(const TilePlan &p) { return lds_bytes(p.W, p.R, p.K); }

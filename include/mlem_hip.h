@@ -185,11 +185,11 @@ typedef struct mlem_opts {
                              2048 columns) with x segments staged in LDS
                              when the matrix allows it (rows sorted by
                              column), else the plain merge path;
-                             1 = plain merge path; 2048, 4096 or 8192 =
+                             1 = plain merge path; 1024, 2048, 4096, 8192 =
                              blocked with that segment width (an error if
                              the matrix does not allow it)                   */
     uint32_t tile_rows;   /* rows per block of the blocked kernels: 0 = 1024,
-                             or 1024, 2048, 4096, 8192, 16384                */
+                             or 512, 1024, 2048, 4096, 8192, 16384           */
 } mlem_opts;
 
 /* Fill opts with the defaults: merge-based FP and BP (argv[8] = 0),

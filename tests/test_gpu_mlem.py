@@ -136,8 +136,8 @@ def test_using_trans_requires_transpose(c1):
         assert np.isfinite(img).all()
 
 
-@pytest.mark.parametrize("width,rows", [(2048, 1024), (2048, 8192), (4096, 1024), (4096, 4096), (4096, 8192),
-                                        (8192, 1024), (8192, 4096)])
+@pytest.mark.parametrize("width,rows", [(1024, 1024), (2048, 1024), (2048, 8192), (4096, 1024), (4096, 4096),
+                                        (4096, 8192), (8192, 1024), (8192, 4096)])
 def test_blocked_merge_path_vs_fp64_oracle(c1, resident, ref50, width, rows):
     """Merge-based FP and BP through the LDS-blocked kernels, every segment width and block size."""
     for secsize in (8, 16):
@@ -166,10 +166,11 @@ def test_blocked_edge_matrices():
                 assert rel_l2(img, ref) <= REL_L2 or np.linalg.norm(ref) == 0
 
 
-def test_blocked_lds_limit_is_an_error(c1, resident, ref50):
-    # 16384 rows x 8192 columns per tile would need 96 KiB of LDS.
-    with pytest.raises(M.MlemError):
-        M.mlem_run(resident, c1["g"], ref50["f0"], 1, options=M.opts(spmv=0, tile_width=8192, tile_rows=16384))
+def test_blocked_largest_tiles(c1, resident, ref50):
+    # 16384 rows x 8192 columns per tile: 96 KiB of LDS per workgroup.
+    o = M.opts(spmv=0, tile_width=8192, tile_rows=16384, secsize_fp=16, secsize_bp=16)
+    img = M.mlem_run(resident, c1["g"], ref50["f0"], 50, sens=ref50["s32"], options=o)
+    assert rel_l2(img, ref50["ref"]) <= REL_L2
 
 
 def test_tiling_refused_and_automatic_fallback():
