@@ -134,10 +134,18 @@ struct Partition {
     DevBuf buf;
 };
 
+// A work list of a tiled matrix on the device (one per unit size and
+// target workgroup count).
+struct TileWorkDev {
+    TileWork view;
+    DevBuf mem;
+};
+
 struct Tiled {
     TilePlan plan;
     DevBuf mem;
     TiledMatrix view;
+    std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<TileWorkDev>> works;  // (E, target workgroups)
 };
 
 struct mlem_matrix {
@@ -170,9 +178,8 @@ struct mlem_recon {
     hipStream_t stream = nullptr;
     DevBuf g, sens, f, ratio, bp, scratch_fp, scratch_bp, llpart;
     const Partition *pfp = nullptr, *pbp = nullptr;
-    const Tiled *tfp = nullptr, *tbp = nullptr;  // blocked LDS-tiled path, when used
-    TilePlan plan_fp, plan_bp;                   // with this reconstruction's segment grouping
-    uint32_t tile_s_fp = 8, tile_s_bp = 8;
+    const Tiled *tfp = nullptr, *tbp = nullptr;        // blocked LDS-tiled path, when used
+    const TileWorkDev *wfp = nullptr, *wbp = nullptr;  // and its work lists
     DevBuf part_fp, part_bp;
     uint32_t lanes_fp = 64, lanes_bp = 64;
     bool bp_merge = true;  // BP on A^T: merge-based or CSR-vector
@@ -225,7 +232,7 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, const Tiled **o
     }
     auto t = std::make_unique<Tiled>();
     const uint32_t nrow = which == 0 ? m->rows : m->cols, ncol = which == 0 ? m->cols : m->rows;
-    t->plan = tile_plan(nrow, ncol, m->nnz, W, R, 1);
+    t->plan = tile_plan(nrow, ncol, m->nnz, W, R);
     HIPCHK(t->mem.alloc(tiled_matrix_bytes(t->plan)));
     DevBuf scratch;
     HIPCHK(scratch.alloc(tile_build_scratch_bytes(t->plan)));
@@ -248,38 +255,78 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, const Tiled **o
     return MLEM_OK;
 }
 
+// Work list of tiled matrix t for unit size E and about `target`
+// workgroups, built on first use from the tile sizes.
+int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t E, uint32_t target, const TileWorkDev **out) {
+    std::lock_guard<std::mutex> lk(m->mu);
+    auto key = std::make_pair(E, target);
+    auto it = t->works.find(key);
+    if (it != t->works.end()) {
+        *out = it->second.get();
+        return MLEM_OK;
+    }
+    const TilePlan &p = t->plan;
+    std::vector<uint32_t> nz(p.ntiles);
+    std::vector<uint64_t> off(p.ntiles + 1);
+    if (p.ntiles) {
+        HIPCHK(hipMemcpy(nz.data(), t->view.tile_nz, 4 * p.ntiles, hipMemcpyDeviceToHost));
+        HIPCHK(hipMemcpy(off.data(), t->view.tile_off, 8 * (p.ntiles + 1), hipMemcpyDeviceToHost));
+    }
+    const TileWorkList L = tile_work_list(p, nz.data(), off.data(), E, target);
+    auto w = std::make_unique<TileWorkDev>();
+    const size_t b_wl = 4 * L.wl.size(), b_tl = 4 * L.tiles.size(), b_first = 4 * L.wl_first.size();
+    const size_t o_tl = (b_wl + 255) & ~size_t(255), o_first = o_tl + ((b_tl + 255) & ~size_t(255));
+    HIPCHK(w->mem.alloc(std::max<size_t>(256, o_first + b_first)));
+    char *base = static_cast<char *>(w->mem.p);
+    if (b_wl) HIPCHK(hipMemcpy(base, L.wl.data(), b_wl, hipMemcpyHostToDevice));
+    if (b_tl) HIPCHK(hipMemcpy(base + o_tl, L.tiles.data(), b_tl, hipMemcpyHostToDevice));
+    HIPCHK(hipMemcpy(base + o_first, L.wl_first.data(), b_first, hipMemcpyHostToDevice));
+    w->view.E = L.E;
+    w->view.span = L.span;
+    w->view.nwg = L.nwg;
+    w->view.ntl = L.ntl;
+    w->view.wl = reinterpret_cast<const uint4 *>(base);
+    w->view.tiles = reinterpret_cast<const uint4 *>(base + o_tl);
+    w->view.wl_first = reinterpret_cast<const uint32_t *>(base + o_first);
+    *out = w.get();
+    t->works.emplace(key, std::move(w));
+    return MLEM_OK;
+}
+
 // Chooses the blocked (LDS-tiled) or the plain merge path for one side of
-// the iteration.  Segment groups are sized so the grid has about 2560
-// workgroups (10 per CU).
-int setup_merge_side(mlem_matrix *m, int which, uint32_t tile_width, uint32_t tile_rows, uint32_t secsize,
-                     const Tiled **tiled, TilePlan *plan, uint32_t *tile_s, DevBuf *part, const Partition **partn,
-                     DevBuf *scratch) {
+// the iteration.  Work units of 256*E entries (E = 8 below secsize 16, else
+// 16); the blocked path runs about tile_groups workgroups (0: 32 per CU).
+int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t secsize, const Tiled **tiled,
+                     const TileWorkDev **work, DevBuf *part, const Partition **partn, DevBuf *scratch) {
     *tiled = nullptr;
-    if (tile_width != 1) {
-        const uint32_t W = tile_width == 0 ? 2048u : tile_width;
-        const uint32_t R = tile_rows == 0 ? 1024u : tile_rows;
+    *work = nullptr;
+    if (o.tile_width != 1) {
+        const uint32_t W = o.tile_width == 0 ? 1024u : o.tile_width;
+        const uint32_t R = o.tile_rows == 0 ? 4096u : o.tile_rows;
         uint32_t flags = 0;
         int st = get_tiled(m, which, W, R, tiled, &flags);
-        if (st == MLEM_OK) {
-            const TilePlan &lp = (*tiled)->plan;
-            const uint32_t want_groups = std::max(1u, (2560u + lp.nblk - 1) / std::max(1u, lp.nblk));
-            const uint32_t K = std::max(1u, (lp.nseg + want_groups - 1) / want_groups);
-            *plan = tile_plan(lp.rows, lp.cols, lp.nnz, W, R, K);
-            int dev = 0, lds_max = 65536;
-            if (hipGetDevice(&dev) == hipSuccess)
-                (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-            if (tiled_lds_bytes(*plan) + 4096 > (size_t)lds_max) {
-                *tiled = nullptr;
-                st = set_err(MLEM_ERR_UNSUPPORTED, "tiling (width %u, rows %u) needs too much LDS", W, R);
-            }
+        int dev = 0, lds_max = 65536, cus = 256;
+        if (hipGetDevice(&dev) == hipSuccess) {
+            (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         }
         if (st == MLEM_OK) {
-            *tile_s = secsize < 16 ? 8u : 16u;  // entries per work-item of a work unit
-            HIPCHK(part->alloc(std::max<size_t>(4, tiled_part_bytes(*plan))));
+            const uint32_t E = secsize < 16 ? 8u : 16u;
+            const uint32_t target = o.tile_groups ? o.tile_groups : 32u * (uint32_t)std::max(1, cus);
+            st = get_tile_work(m, const_cast<Tiled *>(*tiled), E, target, work);
+        }
+        if (st == MLEM_OK && tiled_lds_bytes((*tiled)->plan, (*work)->view.span) + 4096 > (size_t)lds_max) {
+            *tiled = nullptr;
+            *work = nullptr;
+            st = set_err(MLEM_ERR_UNSUPPORTED, "tiling (width %u, rows %u) needs too much LDS", W, R);
+        }
+        if (st == MLEM_OK) {
+            HIPCHK(part->alloc(std::max<size_t>(4, tiled_part_bytes((*tiled)->plan, (*work)->view.nwg))));
             return MLEM_OK;
         }
-        if (st != MLEM_ERR_UNSUPPORTED || tile_width != 0) return st;  // explicit width: no fallback
+        if (st != MLEM_ERR_UNSUPPORTED || o.tile_width != 0) return st;  // explicit width: no fallback
         *tiled = nullptr;
+        *work = nullptr;
     }
     if (int st = get_partition(m, which, secsize, partn)) return st;
     HIPCHK(scratch->alloc(merge_scratch_bytes((*partn)->plan)));
@@ -297,8 +344,8 @@ hipError_t enqueue_iteration(mlem_recon *r, IterEvents *ev) {
     fa.out = r->ratio.as<float>();
     fa.g = r->g.as<int32_t>();
     if (r->tfp)
-        e = launch_tiled_spmv(r->plan_fp, r->tfp->view, r->f.as<float>(), m->cols, r->tile_s_fp,
-                              r->part_fp.as<float>(), EPI_RATIO, fa, s);
+        e = launch_tiled_spmv(r->tfp->plan, r->tfp->view, r->wfp->view, r->f.as<float>(), r->part_fp.as<float>(),
+                              EPI_RATIO, fa, s);
     else if (r->opts.spmv == 0)
         e = launch_merge_spmv(r->pfp->plan, r->pfp->buf.p, r->scratch_fp.p, m->rowptr.as<uint64_t>(),
                               m->col.as<uint32_t>(), m->val.as<float>(), r->f.as<float>(), EPI_RATIO, fa, s);
@@ -313,7 +360,7 @@ hipError_t enqueue_iteration(mlem_recon *r, IterEvents *ev) {
     ba.f = r->f.as<float>();
     if (r->opts.using_trans == 0) {
         if (r->tbp)
-            e = launch_tiled_spmv(r->plan_bp, r->tbp->view, r->ratio.as<float>(), m->rows, r->tile_s_bp,
+            e = launch_tiled_spmv(r->tbp->plan, r->tbp->view, r->wbp->view, r->ratio.as<float>(),
                                   r->part_bp.as<float>(), EPI_UPDATE, ba, s);
         else if (r->bp_merge)
             e = launch_merge_spmv(r->pbp->plan, r->pbp->buf.p, r->scratch_bp.p, m->rowptrT.as<uint64_t>(),
@@ -875,12 +922,12 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
     ARGCHK(o.tile_rows == 0 || valid_tile_rows(o.tile_rows),
            "mlem_recon_create: tile_rows must be 0 (auto), 512, 1024, 2048, 4096, 8192 or 16384");
     if (o.spmv == 0)
-        if (int st = setup_merge_side(m, 0, o.tile_width, o.tile_rows, r->opts.secsize_fp, &r->tfp, &r->plan_fp,
-                                      &r->tile_s_fp, &r->part_fp, &r->pfp, &r->scratch_fp))
+        if (int st = setup_merge_side(m, 0, o, r->opts.secsize_fp, &r->tfp, &r->wfp, &r->part_fp, &r->pfp,
+                                      &r->scratch_fp))
             return st;
     if (o.using_trans == 0 && r->bp_merge)
-        if (int st = setup_merge_side(m, 1, o.tile_width, o.tile_rows, r->opts.secsize_bp, &r->tbp, &r->plan_bp,
-                                      &r->tile_s_bp, &r->part_bp, &r->pbp, &r->scratch_bp))
+        if (int st = setup_merge_side(m, 1, o, r->opts.secsize_bp, &r->tbp, &r->wbp, &r->part_bp, &r->pbp,
+                                      &r->scratch_bp))
             return st;
     HIPCHK(r->llpart.alloc(sizeof(double) * std::max<uint32_t>(1, loglik_blocks(m->rows))));
     HIPCHK(hipStreamSynchronize(r->stream));

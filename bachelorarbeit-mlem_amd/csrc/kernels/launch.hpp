@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <vector>
 
 namespace mlem {
 
@@ -72,14 +73,16 @@ hipError_t launch_loglik(const uint64_t *rowptr, const uint32_t *col, const floa
                          const int32_t *g, uint32_t rows, double *part, hipStream_t s);
 
 // Blocked SpMV (tiled.hip): tiles of R rows x W columns with the W-wide
-// segment of x staged in LDS; workgroup (row block, group of K segments)
-// writes partial row sums, a reduction kernel adds them and applies the
-// epilogue.
-constexpr uint32_t kTileXPad = 8192;  // pad gathered vectors to a multiple of this
+// segment of x staged in LDS.  The units (256*E entries) of each row block
+// are split into runs of nearly equal length, one run per workgroup; a
+// workgroup writes the partial sums of its block's rows, a reduction kernel
+// adds the runs of each block in order and applies the epilogue.
+constexpr uint32_t kTileXPad = 8192;     // pad gathered vectors to a multiple of this
+constexpr uint32_t kTileSpanMax = 256;   // most tiles one run may touch
 struct TilePlan {
     uint32_t rows = 0, cols = 0;
     uint64_t nnz = 0;
-    uint32_t W = 0, R = 0, nseg = 0, nblk = 0, K = 1, ngroups = 1;
+    uint32_t W = 0, R = 0, nseg = 0, nblk = 0;
     uint64_t ntiles = 0;
 };
 struct TiledMatrix {
@@ -89,23 +92,42 @@ struct TiledMatrix {
     uint16_t *tcol = nullptr;      // column - s*W
     uint16_t *trow = nullptr;      // row - b*R
 };
-TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t K);
+// Work list on the host: 4 u32 per run {row block, first tile (index into
+// tiles), first unit in that tile, units}; 4 u32 per non-empty tile
+// {segment, entries, padded offset lo, hi}; wl_first[b] = first run of
+// row block b (nblk+1 entries).
+struct TileWorkList {
+    uint32_t E = 8, span = 1, nwg = 0, ntl = 0;
+    std::vector<uint32_t> wl, tiles, wl_first;
+};
+// The same on the device.
+struct TileWork {
+    uint32_t E = 8, span = 1, nwg = 0, ntl = 0;
+    const uint4 *wl = nullptr;
+    const uint4 *tiles = nullptr;
+    const uint32_t *wl_first = nullptr;
+};
+TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R);
 bool valid_tile_width(uint32_t W);
 bool valid_tile_rows(uint32_t R);
 bool valid_tile_unit(uint32_t E);
 size_t tiled_matrix_bytes(const TilePlan &p);
-size_t tiled_part_bytes(const TilePlan &p);
-size_t tiled_lds_bytes(const TilePlan &p);
+size_t tiled_part_bytes(const TilePlan &p, uint32_t nwg);
+size_t tiled_lds_bytes(const TilePlan &p, uint32_t span);
 size_t tile_build_scratch_bytes(const TilePlan &p);
 TiledMatrix tiled_matrix_view(const TilePlan &p, void *mem);
 // Builds the tiles from a CSR on the device and waits; *flags_host bit 1 =
 // a row is not sorted by column (then nothing is built).
 hipError_t launch_tile_build(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
                              void *mem, void *scratch, uint32_t *flags_host, hipStream_t s);
-// E = entries per work-item of one work unit (8 or 16); part holds
-// tiled_part_bytes(p).  x must be readable (and finite) up to nseg * W
-// floats: callers pad their vectors to whole segments (kTileXPad).
-hipError_t launch_tiled_spmv(const TilePlan &p, const TiledMatrix &tm, const float *x, uint32_t xlen, uint32_t E,
+// Runs for about target_wgs workgroups from the host copies of tile_nz and
+// tile_off; E = entries per work-item of a unit (8 or 16).
+TileWorkList tile_work_list(const TilePlan &p, const uint32_t *tile_nz, const uint64_t *tile_off, uint32_t E,
+                            uint32_t target_wgs);
+// part holds tiled_part_bytes(p, wk.nwg).  x must be readable (and finite)
+// up to nseg * W floats: callers pad their vectors to whole segments
+// (kTileXPad).
+hipError_t launch_tiled_spmv(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x,
                              float *part, Epi epi, const EpiArgs &ea, hipStream_t s);
 
 // Device transpose (stable counting sort): see transpose.hip.

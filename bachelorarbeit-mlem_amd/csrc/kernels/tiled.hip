@@ -7,8 +7,8 @@
 // CSR kernels that gather from global memory near 2.6-3 TB/s on the 12 GB
 // scanner matrix, while streaming the same arrays runs at 6-7 TB/s.  Here the
 // matrix is cut into tiles of R rows x W columns; a tile is dense enough
-// (tens of thousands of entries) that copying its W-wide segment of x into
-// LDS once costs little, and every gather then reads LDS.
+// (thousands of entries) that copying its W-wide segment of x into LDS once
+// costs little, and every gather then reads LDS.
 //
 // Layout, built on the device from a CSR with column-sorted rows:
 //   tile t = (row block b, segment s), t = b*nseg + s, stored in (b, s)
@@ -18,17 +18,22 @@
 //   row - b*R (u16): 8 bytes, the same as CSR's f32 value + u32 column.
 //   tile_off u64[ntiles+1] (padded offsets), tile_nz u32[ntiles].
 //
-// Work: workgroup (b, g) takes the tiles (b, s) for s in segment group g.
-// A tile is cut into units of 256*E entries; work-item k owns the E
-// consecutive entries [kE, kE+E) of a unit (16-byte loads).  Row ends are
-// where the row key changes.  A row split between work-items is finished by
-// the work-item holding its last entry, which adds the carries of the
-// work-items before it (nearest first, back to the previous row end) from
-// LDS; so each row's partial of a unit is added exactly once, in unit order,
-// to an LDS accumulator of the block's rows.  The loads of the next unit (and, for a new tile, its x segment) are
-// issued before the current unit is computed.  Workgroup g writes its
-// partials to part[g][row]; a reduction kernel adds the groups in order and
-// applies the epilogue (ratio or update).  Bitwise reproducible.
+// Work: a tile is cut into units of 256*E entries; work-item k owns the E
+// consecutive entries [kE, kE+E) of a unit (16-byte loads).  The units of a
+// row block (its non-empty tiles in segment order) are split into runs of
+// nearly equal length, one run per workgroup, so every workgroup streams
+// about the same number of bytes whatever the tiles' fill (a work list built
+// on the host from the tile sizes).  Row ends are where the row key changes.
+// A row split between work-items is finished by the work-item holding its
+// last entry, which adds the carries of the work-items before it (nearest
+// first, back to the previous row end) from LDS; so each row that has
+// entries in a unit is updated by exactly one work-item per unit, and the
+// LDS accumulator of the block's rows needs no atomics (a read and a write,
+// ordered by the unit's barriers).  The loads of the next unit (and, for a
+// new tile, its x segment) are issued before the current unit is computed.
+// Workgroup w writes its partials to part[w][0..R); a reduction kernel adds
+// the workgroups of each row block in order and applies the epilogue (ratio
+// or update).  Bitwise reproducible.
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
@@ -48,34 +53,31 @@ __device__ __forceinline__ T ldnt(const T *p) {
 
 constexpr uint32_t kNoKey = 0x10000u;  // past every u16 row key
 
+// A run position: staged tile ti (index into the workgroup's tile list),
+// unit c of that tile, n entries in the unit.
 struct Unit {
-    uint32_t s, c, n;  // segment, unit in the tile, entries in the unit
+    uint32_t ti, c, n;
     bool valid;
 };
 
 template <int E>
-__device__ __forceinline__ Unit next_unit(const uint32_t *snz, uint32_t s_end, uint32_t s, uint32_t c) {
+__device__ __forceinline__ Unit make_unit(const uint4 *tl, uint32_t ti, uint32_t c, bool valid) {
     constexpr uint32_t UE = kWG * E;
     Unit u;
-    if ((c + 1) * UE < snz[s]) {
-        u.s = s;
-        u.c = c + 1;
-    } else {
-        uint32_t t = s + 1;
-        while (t < s_end && snz[t] == 0) ++t;
-        u.s = t;
-        u.c = 0;
-    }
-    u.valid = u.s < s_end;
-    u.n = u.valid ? min(UE, snz[u.s] - u.c * UE) : 0u;
+    u.ti = ti;
+    u.c = c;
+    u.valid = valid;
+    const uint32_t tj = valid ? ti : 0u;  // ti < span whenever valid
+    u.n = valid ? min(UE, tl[tj].y - c * UE) : 0u;
     return u;
 }
 
-// snz / soff are indexed by segment; s_begin..s_end is this group.
+// wl[w] = {row block, first tile (index into tiles), first unit in it, units}
+// tiles[i] = {segment, entries, padded offset lo, hi} of the non-empty tiles.
 template <int E, int W>
-__global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32_t R, uint32_t nseg, uint32_t K,
-                                                           uint32_t xlen, const uint64_t *__restrict__ tile_off,
-                                                           const uint32_t *__restrict__ tile_nz,
+__global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t R, uint32_t span, uint32_t ntl,
+                                                           const uint4 *__restrict__ wl,
+                                                           const uint4 *__restrict__ tiles,
                                                            const float *__restrict__ tval,
                                                            const uint16_t *__restrict__ tcol,
                                                            const uint16_t *__restrict__ trow,
@@ -86,35 +88,31 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32
     __shared__ uint32_t s_k0[kWG + 1];  // first key of every work-item (+ sentinel)
     __shared__ float s_cf[kWG];         // carry after the last row end
     __shared__ uint32_t s_hf[kWG];      // work-item saw a row end
-    float *xs = smem;                                        // [W]
-    float *acc = xs + W;                                     // [R]
-    uint32_t *snz = reinterpret_cast<uint32_t *>(acc + R);   // [K] (padded to even)
-    uint64_t *soff = reinterpret_cast<uint64_t *>(snz + ((K + 1) & ~1u));  // [K]
+    float *xs = smem;                                      // [W]
+    float *acc = xs + W;                                   // [R]
+    uint4 *tl = reinterpret_cast<uint4 *>(acc + R);        // [span]
 
     const uint32_t tid = threadIdx.x;
-    const uint32_t ngroups = (nseg + K - 1) / K;
-    const uint32_t b = blockIdx.x / ngroups, g = blockIdx.x % ngroups;
-    const uint32_t sb = g * K, se = min(nseg, sb + K), ns = se - sb;
-    const uint64_t t0 = (uint64_t)b * nseg + sb;
+    const uint4 w = wl[blockIdx.x];
     for (uint32_t i = tid; i < R; i += kWG) acc[i] = 0.0f;
-    for (uint32_t s = tid; s < ns; s += kWG) {
-        snz[s] = tile_nz[t0 + s];
-        soff[s] = tile_off[t0 + s];
-    }
+    for (uint32_t i = tid; i < span; i += kWG)
+        if (w.y + i < ntl) tl[i] = tiles[w.y + i];
     if (tid == 0) s_k0[kWG] = kNoKey;
     __syncthreads();
 
-    uint32_t s0 = 0;
-    while (s0 < ns && snz[s0] == 0) ++s0;
-    Unit cur{s0, 0, s0 < ns ? min(UE, snz[s0]) : 0u, s0 < ns};
+    uint32_t left = w.w;  // units of the run not yet started
+    Unit cur = make_unit<E>(tl, 0, w.z, left > 0);
 
     f32x4 pv[E / 4];
     u16x8 pc[E / 8], pr[E / 8];
     f32x4 px[XR / 4];
-    auto prefetch = [&](const Unit &u, uint32_t fallback_s) {
-        const uint32_t s = u.valid ? u.s : fallback_s;
-        uint64_t e = soff[s] + (uint64_t)(u.valid ? u.c : 0u) * UE + (uint64_t)tid * E;
-        if ((uint64_t)tid * E >= (u.valid ? u.n : 1u)) e = soff[s];  // stay inside the tile
+    // Loads of unit u; an invalid u re-reads the start of the current tile
+    // (every address stays inside the arrays, so the wait counts are exact).
+    auto prefetch = [&](const Unit &u, uint32_t fallback_ti) {
+        const uint4 t = tl[u.valid ? u.ti : fallback_ti];
+        const uint64_t off = ((uint64_t)t.w << 32) | t.z;
+        uint64_t e = off + (uint64_t)(u.valid ? u.c : 0u) * UE + (uint64_t)tid * E;
+        if ((uint64_t)tid * E >= (u.valid ? u.n : 1u)) e = off;  // stay inside the tile
 #pragma unroll
         for (int m = 0; m < E / 4; ++m) pv[m] = ldnt(reinterpret_cast<const f32x4 *>(tval + e) + m);
 #pragma unroll
@@ -123,12 +121,13 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32
             pr[m] = ldnt(reinterpret_cast<const u16x8 *>(trow + e) + m);
         }
         // x is padded to whole segments (xlen >= nseg * W): no clamping.
-        const float *xb = x + (uint64_t)(sb + s) * W;
+        const float *xb = x + (uint64_t)t.x * W;
 #pragma unroll
         for (int k = 0; k < XR / 4; ++k) px[k] = *(reinterpret_cast<const f32x4 *>(xb) + tid + kWG * k);
     };
 
-    if (cur.valid) prefetch(cur, cur.s);
+    if (cur.valid) prefetch(cur, 0);
+    bool fresh = true;  // first unit of the run: its x segment is not staged yet
     while (cur.valid) {
         f32x4 cv[E / 4];
         u16x8 cc[E / 8], cr[E / 8];
@@ -142,8 +141,11 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32
         }
 #pragma unroll
         for (int k = 0; k < XR / 4; ++k) cx[k] = px[k];
-        const Unit nxt = next_unit<E>(snz, ns, cur.s, cur.c);
-        prefetch(nxt, cur.s);  // in flight while this unit is computed
+        --left;
+        const bool more_in_tile = (cur.c + 1) * UE < tl[cur.ti].y;
+        const Unit nxt = make_unit<E>(tl, more_in_tile ? cur.ti : cur.ti + 1, more_in_tile ? cur.c + 1 : 0u,
+                                      left > 0);
+        prefetch(nxt, cur.ti);  // in flight while this unit is computed
 
         // Keys of this work-item's entries; entries past n get kNoKey.
         const uint32_t e0 = tid * E;
@@ -151,10 +153,11 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32
 #pragma unroll
         for (int m = 0; m < E; ++m) key[m] = (e0 + m < cur.n) ? (uint32_t)cr[m / 8][m % 8] : kNoKey;
         s_k0[tid] = key[0];
-        if (cur.c == 0) {  // a new tile: its x segment
+        if (cur.c == 0 || fresh) {  // a new tile: its x segment
 #pragma unroll
             for (int k = 0; k < XR / 4; ++k) *(reinterpret_cast<f32x4 *>(xs) + tid + kWG * k) = cx[k];
         }
+        fresh = false;
         __syncthreads();
         key[E] = s_k0[tid + 1];
 
@@ -166,14 +169,16 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32
             const float p = cv[m / 4][m % 4] * xs[cc[m / 8][m % 8]];
             prod[m] = key[m] != kNoKey ? p : 0.0f;
         }
-        // Walk my entries: a row ends where the key changes.
+        // Walk my entries: a row ends where the key changes.  A row that
+        // starts and ends here is complete: this work-item is its only
+        // writer in this unit.
         float a = 0.0f, head = 0.0f;
         uint32_t has_end = 0, first_key = 0;
 #pragma unroll
         for (int m = 0; m < E; ++m) {
             a += prod[m];
             const bool end = (key[m] != key[m + 1]) && (key[m] != kNoKey);
-            if (end && has_end) atomicAdd(acc + key[m], a);  // the row's only contribution in this unit
+            if (end && has_end) acc[key[m]] += a;
             const bool first = end && !has_end;
             head = first ? a : head;
             first_key = first ? key[m] : first_key;
@@ -192,25 +197,25 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t rows, uint32
                 cin += s_cf[jj];
                 if (s_hf[jj]) break;
             }
-            atomicAdd(acc + first_key, cin + head);
+            acc[first_key] += cin + head;
         }
         cur = nxt;
     }
     __syncthreads();
-    // Partials of this segment group for the block's rows.
-    float *pg = part + (uint64_t)g * rows;
-    for (uint32_t i = tid; i < R; i += kWG) {
-        const uint64_t r = (uint64_t)b * R + i;
-        if (r < rows) pg[r] = acc[i];
-    }
+    float *pw = part + (uint64_t)blockIdx.x * R;
+    for (uint32_t i = tid; i < R; i += kWG) pw[i] = acc[i];
 }
 
+// Row r of block b = r / R sums the partials of the block's workgroups
+// wl_first[b] .. wl_first[b+1]-1 in order, then applies the epilogue.
 template <int EPI>
-__global__ void blocked_reduce_kernel(const float *__restrict__ part, uint32_t ngroups, uint32_t rows, EpiArgs ea) {
+__global__ void blocked_reduce_kernel(const float *__restrict__ part, const uint32_t *__restrict__ wl_first,
+                                      uint32_t rshift, uint32_t rows, EpiArgs ea) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
+    const uint32_t b = r >> rshift, i = r & ((1u << rshift) - 1u);
     float y = 0.0f;
-    for (uint32_t g = 0; g < ngroups; ++g) y += part[(uint64_t)g * rows + r];
+    for (uint32_t w = wl_first[b]; w < wl_first[b + 1]; ++w) y += part[((uint64_t)w << rshift) + i];
     if constexpr (EPI == EPI_PLAIN) {
         ea.out[r] = y;
     } else if constexpr (EPI == EPI_RATIO) {
@@ -307,26 +312,24 @@ __global__ __launch_bounds__(kWG) void rows_sorted_kernel(const uint64_t *__rest
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-size_t lds_bytes(uint32_t W, uint32_t R, uint32_t K) {
-    return ((size_t)W + R + (((size_t)K + 1) & ~size_t(1)) + 2 * (size_t)K) * 4;
-}
+size_t lds_bytes(uint32_t W, uint32_t R, uint32_t span) { return ((size_t)W + R) * 4 + (size_t)span * 16; }
 
 template <int E, int W>
-hipError_t launch_w(const TilePlan &p, const TiledMatrix &tm, const float *x, uint32_t xlen, float *part,
+hipError_t launch_w(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
                     hipStream_t s) {
-    hipLaunchKernelGGL((blocked_spmv_kernel<E, W>), dim3(p.nblk * p.ngroups), dim3(kWG), lds_bytes(W, p.R, p.K), s,
-                       p.rows, p.R, p.nseg, p.K, xlen, tm.tile_off, tm.tile_nz, tm.tval, tm.tcol, tm.trow, x, part);
+    hipLaunchKernelGGL((blocked_spmv_kernel<E, W>), dim3(wk.nwg), dim3(kWG), lds_bytes(W, p.R, wk.span), s, p.R,
+                       wk.span, wk.ntl, wk.wl, wk.tiles, tm.tval, tm.tcol, tm.trow, x, part);
     return hipGetLastError();
 }
 
 template <int E>
-hipError_t launch_e(const TilePlan &p, const TiledMatrix &tm, const float *x, uint32_t xlen, float *part,
+hipError_t launch_e(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
                     hipStream_t s) {
     switch (p.W) {
-        case 1024: return launch_w<E, 1024>(p, tm, x, xlen, part, s);
-        case 2048: return launch_w<E, 2048>(p, tm, x, xlen, part, s);
-        case 4096: return launch_w<E, 4096>(p, tm, x, xlen, part, s);
-        default: return launch_w<E, 8192>(p, tm, x, xlen, part, s);
+        case 1024: return launch_w<E, 1024>(p, tm, wk, x, part, s);
+        case 2048: return launch_w<E, 2048>(p, tm, wk, x, part, s);
+        case 4096: return launch_w<E, 4096>(p, tm, wk, x, part, s);
+        default: return launch_w<E, 8192>(p, tm, wk, x, part, s);
     }
 }
 
@@ -341,7 +344,7 @@ size_t scan_bytes(const TilePlan &p) {
 
 }  // namespace
 
-TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t K) {
+TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R) {
     TilePlan p;
     p.rows = rows;
     p.cols = cols;
@@ -350,8 +353,6 @@ TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint3
     p.R = R;
     p.nseg = (cols + W - 1) / W;
     p.nblk = (rows + R - 1) / R;
-    p.K = std::max(1u, std::min(K, p.nseg));
-    p.ngroups = (p.nseg + p.K - 1) / p.K;
     p.ntiles = (uint64_t)p.nseg * p.nblk;
     return p;
 }
@@ -360,14 +361,69 @@ bool valid_tile_width(uint32_t W) { return W == 1024 || W == 2048 || W == 4096 |
 bool valid_tile_rows(uint32_t R) { return R == 512 || R == 1024 || R == 2048 || R == 4096 || R == 8192 || R == 16384; }
 bool valid_tile_unit(uint32_t E) { return E == 8 || E == 16; }
 
-size_t tiled_lds_bytes(const TilePlan &p) { return lds_bytes(p.W, p.R, p.K); }
+size_t tiled_lds_bytes(const TilePlan &p, uint32_t span) { return lds_bytes(p.W, p.R, span); }
 
 size_t tiled_matrix_bytes(const TilePlan &p) {
     const uint64_t cap = padded_capacity(p);
     return align256(8 * (p.ntiles + 1)) + align256(4 * p.ntiles) + align256(4 * cap) + 2 * align256(2 * cap);
 }
 
-size_t tiled_part_bytes(const TilePlan &p) { return sizeof(float) * (size_t)p.ngroups * p.rows; }
+size_t tiled_part_bytes(const TilePlan &p, uint32_t nwg) { return sizeof(float) * (size_t)nwg * p.R; }
+
+TileWorkList tile_work_list(const TilePlan &p, const uint32_t *nz, const uint64_t *off, uint32_t E,
+                            uint32_t target_wgs) {
+    const uint64_t UE = (uint64_t)kWG * E;
+    TileWorkList L;
+    L.E = E;
+    uint64_t total = 0;
+    for (uint64_t t = 0; t < p.ntiles; ++t) total += (nz[t] + UE - 1) / UE;
+    const uint64_t per_run = std::max<uint64_t>(1, (total + std::max(1u, target_wgs) - 1) / std::max(1u, target_wgs));
+    L.wl_first.assign((size_t)p.nblk + 1, 0);
+    std::vector<uint64_t> units;  // per non-empty tile of the current block
+    for (uint32_t b = 0; b < p.nblk; ++b) {
+        L.wl_first[b] = L.nwg;
+        const uint32_t first = L.ntl;
+        uint64_t ub = 0;
+        units.clear();
+        for (uint32_t sg = 0; sg < p.nseg; ++sg) {
+            const uint64_t t = (uint64_t)b * p.nseg + sg;
+            if (!nz[t]) continue;
+            L.tiles.insert(L.tiles.end(), {sg, nz[t], (uint32_t)off[t], (uint32_t)(off[t] >> 32)});
+            ++L.ntl;
+            units.push_back((nz[t] + UE - 1) / UE);
+            ub += units.back();
+        }
+        // Runs of nearly equal length over the block's units, none touching
+        // more than kTileSpanMax tiles.
+        uint64_t runs = (ub + per_run - 1) / per_run, left = ub;
+        uint32_t ti = 0;
+        uint64_t c = 0;
+        while (left) {
+            const uint64_t want = (left + runs - 1) / runs;
+            const uint32_t t0 = ti;
+            const uint64_t c0 = c;
+            uint64_t got = 0;
+            uint32_t touched = 0;
+            while (got < want && touched < kTileSpanMax) {
+                ++touched;
+                const uint64_t take = std::min(want - got, units[ti] - c);
+                got += take;
+                c += take;
+                if (c == units[ti]) {
+                    ++ti;
+                    c = 0;
+                }
+            }
+            L.wl.insert(L.wl.end(), {b, first + t0, (uint32_t)c0, (uint32_t)got});
+            ++L.nwg;
+            L.span = std::max(L.span, touched);
+            left -= got;
+            runs = runs > 1 ? runs - 1 : 1;
+        }
+    }
+    L.wl_first[p.nblk] = L.nwg;
+    return L;
+}
 
 size_t tile_build_scratch_bytes(const TilePlan &p) {
     return 2 * align256(4ull * p.rows * p.nseg) + align256(8 * (p.ntiles + 1)) + align256(scan_bytes(p)) + 256;
@@ -442,16 +498,27 @@ hipError_t launch_tile_build(const TilePlan &p, const uint64_t *rowptr, const ui
     return hipStreamSynchronize(s);
 }
 
-hipError_t launch_tiled_spmv(const TilePlan &p, const TiledMatrix &tm, const float *x, uint32_t xlen, uint32_t E,
+hipError_t launch_tiled_spmv(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x,
                              float *part, Epi epi, const EpiArgs &ea, hipStream_t s) {
     if (p.rows == 0) return hipSuccess;
-    hipError_t e = (E == 8) ? launch_e<8>(p, tm, x, xlen, part, s) : launch_e<16>(p, tm, x, xlen, part, s);
-    if (e != hipSuccess) return e;
+    if (wk.nwg) {
+        hipError_t e = (wk.E == 8) ? launch_e<8>(p, tm, wk, x, part, s) : launch_e<16>(p, tm, wk, x, part, s);
+        if (e != hipSuccess) return e;
+    }
+    uint32_t rshift = 0;
+    while ((1u << rshift) < p.R) ++rshift;
     const dim3 grid(blocks_for(p.rows, kWG)), block(kWG);
     switch (epi) {
-        case EPI_PLAIN: hipLaunchKernelGGL(blocked_reduce_kernel<EPI_PLAIN>, grid, block, 0, s, part, p.ngroups, p.rows, ea); break;
-        case EPI_RATIO: hipLaunchKernelGGL(blocked_reduce_kernel<EPI_RATIO>, grid, block, 0, s, part, p.ngroups, p.rows, ea); break;
-        default: hipLaunchKernelGGL(blocked_reduce_kernel<EPI_UPDATE>, grid, block, 0, s, part, p.ngroups, p.rows, ea); break;
+        case EPI_PLAIN:
+            hipLaunchKernelGGL(blocked_reduce_kernel<EPI_PLAIN>, grid, block, 0, s, part, wk.wl_first, rshift, p.rows, ea);
+            break;
+        case EPI_RATIO:
+            hipLaunchKernelGGL(blocked_reduce_kernel<EPI_RATIO>, grid, block, 0, s, part, wk.wl_first, rshift, p.rows, ea);
+            break;
+        default:
+            hipLaunchKernelGGL(blocked_reduce_kernel<EPI_UPDATE>, grid, block, 0, s, part, wk.wl_first, rshift, p.rows,
+                               ea);
+            break;
     }
     return hipGetLastError();
 }

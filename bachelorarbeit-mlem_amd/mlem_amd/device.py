@@ -223,13 +223,15 @@ def default_opts() -> Opts:
 
 
 def opts(spmv=0, using_trans=0, secsize_fp=8, secsize_bp=8, fp_lanes=0, bp_lanes=0, profile=0, use_graph=0,
-         bp_spmv=0, tile_width=0, tile_rows=0) -> Opts:
+         bp_spmv=0, tile_width=0, tile_rows=0, tile_groups=0) -> Opts:
     """bp_spmv: 0 = BP uses the spmv algorithm too, 1 = merge-based BP, 2 = CSR-vector BP.
-    tile_width: 0 = automatic LDS tiling of the merge-based kernels, 1 = plain merge path, else the segment width."""
+    tile_width: 0 = automatic LDS tiling of the merge-based kernels, 1 = plain merge path, else the segment width.
+    tile_rows: rows per tile block (0 = default); tile_groups: workgroups per blocked SpMV (0 = default)."""
     o = Opts()
     o.bp_spmv = bp_spmv
     o.tile_width = tile_width
     o.tile_rows = tile_rows
+    o.tile_groups = tile_groups
     o.spmv, o.using_trans, o.secsize_fp, o.secsize_bp = spmv, using_trans, secsize_fp, secsize_bp
     o.fp_lanes, o.bp_lanes, o.profile, o.use_graph = fp_lanes, bp_lanes, profile, use_graph
     return o

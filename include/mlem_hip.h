@@ -182,14 +182,17 @@ typedef struct mlem_opts {
                              as in the reference), 1 = merge-based,
                              2 = CSR-vector                                  */
     uint32_t tile_width;  /* merge-based kernels: 0 = blocked (segments of
-                             2048 columns) with x segments staged in LDS
+                             1024 columns) with x segments staged in LDS
                              when the matrix allows it (rows sorted by
                              column), else the plain merge path;
                              1 = plain merge path; 1024, 2048, 4096, 8192 =
                              blocked with that segment width (an error if
                              the matrix does not allow it)                   */
-    uint32_t tile_rows;   /* rows per block of the blocked kernels: 0 = 1024,
+    uint32_t tile_rows;   /* rows per block of the blocked kernels: 0 = 4096,
                              or 512, 1024, 2048, 4096, 8192, 16384           */
+    uint32_t tile_groups; /* workgroups per blocked SpMV (runs of equal
+                             length over each row block's tiles):
+                             0 = 32 per CU                                   */
 } mlem_opts;
 
 /* Fill opts with the defaults: merge-based FP and BP (argv[8] = 0),

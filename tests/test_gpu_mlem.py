@@ -166,6 +166,15 @@ def test_blocked_edge_matrices():
                 assert rel_l2(img, ref) <= REL_L2 or np.linalg.norm(ref) == 0
 
 
+@pytest.mark.parametrize("groups", [1, 7, 64, 100000])
+def test_blocked_work_split(c1, resident, ref50, groups):
+    """Any number of runs per row block (one run per block up to one unit per run) gives the same image."""
+    print(f"blocked groups={groups}", flush=True)
+    o = M.opts(spmv=0, secsize_fp=8, secsize_bp=16, tile_width=1024, tile_rows=512, tile_groups=groups)
+    img = M.mlem_run(resident, c1["g"], ref50["f0"], 50, sens=ref50["s32"], options=o)
+    assert rel_l2(img, ref50["ref"]) <= REL_L2
+
+
 def test_blocked_largest_tiles(c1, resident, ref50):
     # 16384 rows x 8192 columns per tile: 96 KiB of LDS per workgroup.
     o = M.opts(spmv=0, tile_width=8192, tile_rows=16384, secsize_fp=16, secsize_bp=16)
