@@ -145,7 +145,7 @@ struct Tiled {
     TilePlan plan;
     DevBuf mem;
     TiledMatrix view;
-    std::map<std::pair<uint32_t, uint32_t>, std::unique_ptr<TileWorkDev>> works;  // (E, target workgroups)
+    std::map<uint32_t, std::unique_ptr<TileWorkDev>> works;  // by target workgroup count
 };
 
 struct mlem_matrix {
@@ -233,33 +233,37 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, const Tiled **o
     auto t = std::make_unique<Tiled>();
     const uint32_t nrow = which == 0 ? m->rows : m->cols, ncol = which == 0 ? m->cols : m->rows;
     t->plan = tile_plan(nrow, ncol, m->nnz, W, R);
-    HIPCHK(t->mem.alloc(tiled_matrix_bytes(t->plan)));
+    const uint64_t *rp = which == 0 ? m->rowptr.as<uint64_t>() : m->rowptrT.as<uint64_t>();
+    const uint32_t *ci = which == 0 ? m->col.as<uint32_t>() : m->colT.as<uint32_t>();
+    const float *va = which == 0 ? m->val.as<float>() : m->valT.as<float>();
     DevBuf scratch;
-    HIPCHK(scratch.alloc(tile_build_scratch_bytes(t->plan)));
-    t->view = tiled_matrix_view(t->plan, t->mem.p);
+    HIPCHK(scratch.alloc(tile_count_scratch_bytes(t->plan)));
     uint32_t fl = 0;
-    if (which == 0)
-        HIPCHK(launch_tile_build(t->plan, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(),
-                                 t->mem.p, scratch.p, &fl, nullptr));
-    else
-        HIPCHK(launch_tile_build(t->plan, m->rowptrT.as<uint64_t>(), m->colT.as<uint32_t>(), m->valT.as<float>(),
-                                 t->mem.p, scratch.p, &fl, nullptr));
+    HIPCHK(launch_tile_count(t->plan, rp, ci, scratch.p, &fl, nullptr));
     if (fl) {
         m->tile_refused[key] = fl;
         *flags = fl;
         return set_err(MLEM_ERR_UNSUPPORTED, "tiling (width %u, rows %u) refused: a row is not sorted by column", W,
                        R);
     }
+    HIPCHK(t->mem.alloc(tiled_matrix_bytes(t->plan)));
+    t->view = tiled_matrix_view(t->plan, t->mem.p);
+    {
+        DevBuf fill_scratch;
+        HIPCHK(fill_scratch.alloc(tile_fill_scratch_bytes(t->plan)));
+        HIPCHK(launch_tile_fill(t->plan, rp, ci, va, t->mem.p, scratch.p, fill_scratch.p, &fl, nullptr));
+    }
+    if (fl) return set_err(MLEM_ERR_FORMAT, "tiling (width %u, rows %u): layout check failed (flags %u)", W, R, fl);
     *out = t.get();
     m->tiles.emplace(key, std::move(t));
     return MLEM_OK;
 }
 
-// Work list of tiled matrix t for unit size E and about `target`
-// workgroups, built on first use from the tile sizes.
-int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t E, uint32_t target, const TileWorkDev **out) {
+// Work list of tiled matrix t for about `target` workgroups, built on first
+// use from the tile sizes.
+int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t target, const TileWorkDev **out) {
     std::lock_guard<std::mutex> lk(m->mu);
-    auto key = std::make_pair(E, target);
+    const uint32_t key = target;
     auto it = t->works.find(key);
     if (it != t->works.end()) {
         *out = it->second.get();
@@ -272,7 +276,7 @@ int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t E, uint32_t target, const T
         HIPCHK(hipMemcpy(nz.data(), t->view.tile_nz, 4 * p.ntiles, hipMemcpyDeviceToHost));
         HIPCHK(hipMemcpy(off.data(), t->view.tile_off, 8 * (p.ntiles + 1), hipMemcpyDeviceToHost));
     }
-    const TileWorkList L = tile_work_list(p, nz.data(), off.data(), E, target);
+    const TileWorkList L = tile_work_list(p, nz.data(), off.data(), target);
     auto w = std::make_unique<TileWorkDev>();
     const size_t b_wl = 4 * L.wl.size(), b_tl = 4 * L.tiles.size(), b_first = 4 * L.wl_first.size();
     const size_t o_tl = (b_wl + 255) & ~size_t(255), o_first = o_tl + ((b_tl + 255) & ~size_t(255));
@@ -281,7 +285,6 @@ int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t E, uint32_t target, const T
     if (b_wl) HIPCHK(hipMemcpy(base, L.wl.data(), b_wl, hipMemcpyHostToDevice));
     if (b_tl) HIPCHK(hipMemcpy(base + o_tl, L.tiles.data(), b_tl, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(base + o_first, L.wl_first.data(), b_first, hipMemcpyHostToDevice));
-    w->view.E = L.E;
     w->view.span = L.span;
     w->view.nwg = L.nwg;
     w->view.ntl = L.ntl;
@@ -294,8 +297,8 @@ int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t E, uint32_t target, const T
 }
 
 // Chooses the blocked (LDS-tiled) or the plain merge path for one side of
-// the iteration.  Work units of 256*E entries (E = 8 below secsize 16, else
-// 16); the blocked path runs about tile_groups workgroups (0: 32 per CU).
+// the iteration.  The blocked path runs about tile_groups workgroups (0: 32
+// per CU); secsize applies to the plain merge path only.
 int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t secsize, const Tiled **tiled,
                      const TileWorkDev **work, DevBuf *part, const Partition **partn, DevBuf *scratch) {
     *tiled = nullptr;
@@ -311,9 +314,8 @@ int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t sec
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         }
         if (st == MLEM_OK) {
-            const uint32_t E = secsize < 16 ? 8u : 16u;
             const uint32_t target = o.tile_groups ? o.tile_groups : 32u * (uint32_t)std::max(1, cus);
-            st = get_tile_work(m, const_cast<Tiled *>(*tiled), E, target, work);
+            st = get_tile_work(m, const_cast<Tiled *>(*tiled), target, work);
         }
         if (st == MLEM_OK && tiled_lds_bytes((*tiled)->plan, (*work)->view.span) + 4096 > (size_t)lds_max) {
             *tiled = nullptr;
@@ -918,7 +920,7 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
         HIPCHK(launch_fill(r->bp.as<float>(), 0.0f, m->cols, r->stream));
     }
     ARGCHK(o.tile_width <= 1 || valid_tile_width(o.tile_width),
-           "mlem_recon_create: tile_width must be 0 (auto), 1 (off), 1024, 2048, 4096 or 8192");
+           "mlem_recon_create: tile_width must be 0 (auto), 1 (off), 1024 or 2048");
     ARGCHK(o.tile_rows == 0 || valid_tile_rows(o.tile_rows),
            "mlem_recon_create: tile_rows must be 0 (auto), 512, 1024, 2048, 4096, 8192 or 16384");
     if (o.spmv == 0)

@@ -73,36 +73,38 @@ hipError_t launch_loglik(const uint64_t *rowptr, const uint32_t *col, const floa
                          const int32_t *g, uint32_t rows, double *part, hipStream_t s);
 
 // Blocked SpMV (tiled.hip): tiles of R rows x W columns with the W-wide
-// segment of x staged in LDS.  The units (256*E entries) of each row block
-// are split into runs of nearly equal length, one run per workgroup; a
-// workgroup writes the partial sums of its block's rows, a reduction kernel
-// adds the runs of each block in order and applies the epilogue.
+// segment of x staged in LDS, entries packed to 6 bytes in units of
+// kTileUnit.  The units of each row block are split into runs of nearly
+// equal length, one run per workgroup; a workgroup writes the partial sums
+// of its block's rows, a reduction kernel adds the runs of each block in
+// order and applies the epilogue.
 constexpr uint32_t kTileXPad = 8192;     // pad gathered vectors to a multiple of this
 constexpr uint32_t kTileSpanMax = 256;   // most tiles one run may touch
+constexpr uint32_t kTileE = 8;           // entries per work-item of a unit
+constexpr uint32_t kTileUnit = kWG * kTileE;
 struct TilePlan {
     uint32_t rows = 0, cols = 0;
     uint64_t nnz = 0;
     uint32_t W = 0, R = 0, nseg = 0, nblk = 0;
     uint64_t ntiles = 0;
+    uint64_t cap = 0;  // entries including padding (set by launch_tile_count)
 };
 struct TiledMatrix {
-    uint64_t *tile_off = nullptr;  // [ntiles+1] padded entry offsets
-    uint32_t *tile_nz = nullptr;   // [ntiles] entries per tile
-    float *tval = nullptr;         // values
-    uint16_t *tcol = nullptr;      // column - s*W
-    uint16_t *trow = nullptr;      // row - b*R
+    uint64_t *tile_off = nullptr;  // [ntiles+1] entry offsets (padded to 8)
+    uint32_t *tile_nz = nullptr;   // [ntiles] entries including padding
+    uint8_t *tdata = nullptr;      // 6 bytes per entry: per unit, values then group words
 };
 // Work list on the host: 4 u32 per run {row block, first tile (index into
 // tiles), first unit in that tile, units}; 4 u32 per non-empty tile
-// {segment, entries, padded offset lo, hi}; wl_first[b] = first run of
+// {segment, entries, entry offset lo, hi}; wl_first[b] = first run of
 // row block b (nblk+1 entries).
 struct TileWorkList {
-    uint32_t E = 8, span = 1, nwg = 0, ntl = 0;
+    uint32_t span = 1, nwg = 0, ntl = 0;
     std::vector<uint32_t> wl, tiles, wl_first;
 };
 // The same on the device.
 struct TileWork {
-    uint32_t E = 8, span = 1, nwg = 0, ntl = 0;
+    uint32_t span = 1, nwg = 0, ntl = 0;
     const uint4 *wl = nullptr;
     const uint4 *tiles = nullptr;
     const uint32_t *wl_first = nullptr;
@@ -110,19 +112,24 @@ struct TileWork {
 TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R);
 bool valid_tile_width(uint32_t W);
 bool valid_tile_rows(uint32_t R);
-bool valid_tile_unit(uint32_t E);
 size_t tiled_matrix_bytes(const TilePlan &p);
 size_t tiled_part_bytes(const TilePlan &p, uint32_t nwg);
 size_t tiled_lds_bytes(const TilePlan &p, uint32_t span);
-size_t tile_build_scratch_bytes(const TilePlan &p);
+size_t tile_count_scratch_bytes(const TilePlan &p);
+size_t tile_fill_scratch_bytes(const TilePlan &p);  // after the count (uses p.cap)
 TiledMatrix tiled_matrix_view(const TilePlan &p, void *mem);
-// Builds the tiles from a CSR on the device and waits; *flags_host bit 1 =
-// a row is not sorted by column (then nothing is built).
-hipError_t launch_tile_build(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
-                             void *mem, void *scratch, uint32_t *flags_host, hipStream_t s);
+// Building the tiles from a CSR on the device takes two calls, each waits.
+// Count: sizes and offsets into `scratch`, sets p.cap; *flags_host bit 1 =
+// a row is not sorted by column (then p.cap is not set and the tiles must
+// not be filled).  Fill: the tiled arrays into `mem` (tiled_matrix_bytes(p))
+// with the same scratch; *flags_host bit 4 = internal layout error.
+hipError_t launch_tile_count(TilePlan &p, const uint64_t *rowptr, const uint32_t *col, void *scratch,
+                             uint32_t *flags_host, hipStream_t s);
+hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
+                            void *mem, void *scratch, void *fill_scratch, uint32_t *flags_host, hipStream_t s);
 // Runs for about target_wgs workgroups from the host copies of tile_nz and
-// tile_off; E = entries per work-item of a unit (8 or 16).
-TileWorkList tile_work_list(const TilePlan &p, const uint32_t *tile_nz, const uint64_t *tile_off, uint32_t E,
+// tile_off.
+TileWorkList tile_work_list(const TilePlan &p, const uint32_t *tile_nz, const uint64_t *tile_off,
                             uint32_t target_wgs);
 // part holds tiled_part_bytes(p, wk.nwg).  x must be readable (and finite)
 // up to nseg * W floats: callers pad their vectors to whole segments

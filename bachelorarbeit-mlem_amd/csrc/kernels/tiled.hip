@@ -12,28 +12,37 @@
 //
 // Layout, built on the device from a CSR with column-sorted rows:
 //   tile t = (row block b, segment s), t = b*nseg + s, stored in (b, s)
-//   order, each tile padded to a multiple of 8 entries; inside a tile the
-//   entries sorted by (row, column), so every row still sums its entries in
-//   the original column order.  Per entry: value f32, column - s*W (u16),
-//   row - b*R (u16): 8 bytes, the same as CSR's f32 value + u32 column.
-//   tile_off u64[ntiles+1] (padded offsets), tile_nz u32[ntiles].
+//   order.  Inside a tile the entries are sorted by (row, column), so every
+//   row still sums its entries in the original column order.  Entries come
+//   in groups of 8; a group never holds a row step (key difference between
+//   neighbours) above 3: a row that starts further away starts a new group
+//   and the rest of the old one is padding (value 0, column W, whose LDS
+//   slot holds 0).  A tile is cut into units of 2048 entries (the last one
+//   shorter, a multiple of 8); unit u of n entries is stored contiguously as
+//     n f32 values, then n/8 group words of 16 bytes:
+//       bits  0-15  row - b*R of the group's first entry
+//       bits 16-29  2-bit row steps of entries 1..7
+//       bits 32-..  8 columns - s*W, 11 bits (W = 1024) or 12 bits (W = 2048)
+//   so an entry costs 6 bytes (CSR: 8) and a unit is one 12 KiB stream.
+//   tile_off u64[ntiles+1] (entry offsets, padded to 8), tile_nz u32[ntiles]
+//   (entries including padding).
 //
-// Work: a tile is cut into units of 256*E entries; work-item k owns the E
-// consecutive entries [kE, kE+E) of a unit (16-byte loads).  The units of a
-// row block (its non-empty tiles in segment order) are split into runs of
-// nearly equal length, one run per workgroup, so every workgroup streams
-// about the same number of bytes whatever the tiles' fill (a work list built
-// on the host from the tile sizes).  Row ends are where the row key changes.
-// A row split between work-items is finished by the work-item holding its
-// last entry, which adds the carries of the work-items before it (nearest
-// first, back to the previous row end) from LDS; so each row that has
-// entries in a unit is updated by exactly one work-item per unit, and the
-// LDS accumulator of the block's rows needs no atomics (a read and a write,
-// ordered by the unit's barriers).  The loads of the next unit (and, for a
-// new tile, its x segment) are issued before the current unit is computed.
-// Workgroup w writes its partials to part[w][0..R); a reduction kernel adds
-// the workgroups of each row block in order and applies the epilogue (ratio
-// or update).  Bitwise reproducible.
+// Work: work-item k owns entries [8k, 8k+8) of a unit (two 16-byte value
+// loads, one group word).  The units of a row block (its non-empty tiles in
+// segment order) are split into runs of nearly equal length, one run per
+// workgroup, so every workgroup streams about the same number of bytes
+// whatever the tiles' fill (a work list built on the host from the tile
+// sizes).  Row ends are where the row key changes.  A row split between
+// work-items is finished by the work-item holding its last entry, which adds
+// the carries of the work-items before it (nearest first, back to the
+// previous row end) from LDS; so each row that has entries in a unit is
+// updated by exactly one work-item per unit, and the LDS accumulator of the
+// block's rows needs no atomics (a read and a write, ordered by the unit's
+// barriers).  The loads of the next unit (and, for a new tile, its x
+// segment) are issued before the current unit is computed.  Workgroup w
+// writes its partials to part[w][0..R); a reduction kernel adds the
+// workgroups of each row block in order and applies the epilogue (ratio or
+// update).  Bitwise reproducible.
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
@@ -44,6 +53,7 @@ namespace mlem {
 namespace {
 
 typedef float f32x4 __attribute__((ext_vector_type(4)));
+typedef uint32_t u32x4 __attribute__((ext_vector_type(4)));
 typedef uint16_t u16x8 __attribute__((ext_vector_type(8)));
 
 template <typename T>
@@ -51,7 +61,32 @@ __device__ __forceinline__ T ldnt(const T *p) {
     return __builtin_nontemporal_load(p);
 }
 
-constexpr uint32_t kNoKey = 0x10000u;  // past every u16 row key
+constexpr uint32_t kE = kTileE;          // entries per work-item of a unit
+constexpr uint32_t kUE = kTileUnit;      // entries per unit
+constexpr uint32_t kNoKey = 0x10000u;    // past every u16 row key
+constexpr uint32_t kMaxRowStep = 3;      // largest key step inside a group of 8 entries
+constexpr uint16_t kPadRow = 0xffffu;    // build scratch: position holds padding
+static_assert(kE == 8 && kUE == kWG * kE, "one group word per work-item");
+
+template <int W>
+constexpr int col_bits() {
+    return W == 1024 ? 11 : 12;  // columns 0..W (W = padding)
+}
+
+// Column j of a group word: bits [32 + j*CB, 32 + (j+1)*CB).
+template <int CB>
+__device__ __forceinline__ uint32_t group_col(const u32x4 &w, int j) {
+    const int bit = j * CB;
+    const uint64_t lo = (uint64_t)w.y | ((uint64_t)w.z << 32);
+    uint64_t v;
+    if (bit + CB <= 64)
+        v = lo >> bit;
+    else if (bit >= 64)
+        v = (uint64_t)w.w >> (bit - 64);
+    else
+        v = (lo >> bit) | ((uint64_t)w.w << (64 - bit));
+    return (uint32_t)v & ((1u << CB) - 1u);
+}
 
 // A run position: staged tile ti (index into the workgroup's tile list),
 // unit c of that tile, n entries in the unit.
@@ -60,36 +95,32 @@ struct Unit {
     bool valid;
 };
 
-template <int E>
 __device__ __forceinline__ Unit make_unit(const uint4 *tl, uint32_t ti, uint32_t c, bool valid) {
-    constexpr uint32_t UE = kWG * E;
     Unit u;
     u.ti = ti;
     u.c = c;
     u.valid = valid;
     const uint32_t tj = valid ? ti : 0u;  // ti < span whenever valid
-    u.n = valid ? min(UE, tl[tj].y - c * UE) : 0u;
+    u.n = valid ? min(kUE, tl[tj].y - c * kUE) : 0u;
     return u;
 }
 
 // wl[w] = {row block, first tile (index into tiles), first unit in it, units}
-// tiles[i] = {segment, entries, padded offset lo, hi} of the non-empty tiles.
-template <int E, int W>
+// tiles[i] = {segment, entries, entry offset lo, hi} of the non-empty tiles.
+template <int W>
 __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t R, uint32_t span, uint32_t ntl,
                                                            const uint4 *__restrict__ wl,
                                                            const uint4 *__restrict__ tiles,
-                                                           const float *__restrict__ tval,
-                                                           const uint16_t *__restrict__ tcol,
-                                                           const uint16_t *__restrict__ trow,
+                                                           const uint8_t *__restrict__ tdata,
                                                            const float *__restrict__ x, float *__restrict__ part) {
-    constexpr uint32_t UE = kWG * E;
     constexpr int XR = W / kWG;
+    constexpr int CB = col_bits<W>();
     extern __shared__ __attribute__((aligned(16))) float smem[];
     __shared__ uint32_t s_k0[kWG + 1];  // first key of every work-item (+ sentinel)
     __shared__ float s_cf[kWG];         // carry after the last row end
     __shared__ uint32_t s_hf[kWG];      // work-item saw a row end
-    float *xs = smem;                                      // [W]
-    float *acc = xs + W;                                   // [R]
+    float *xs = smem;                                      // [W + 4]; xs[W] = 0 for padding entries
+    float *acc = xs + W + 4;                               // [R]
     uint4 *tl = reinterpret_cast<uint4 *>(acc + R);        // [span]
 
     const uint32_t tid = threadIdx.x;
@@ -97,29 +128,31 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t R, uint32_t 
     for (uint32_t i = tid; i < R; i += kWG) acc[i] = 0.0f;
     for (uint32_t i = tid; i < span; i += kWG)
         if (w.y + i < ntl) tl[i] = tiles[w.y + i];
-    if (tid == 0) s_k0[kWG] = kNoKey;
+    if (tid == 0) {
+        s_k0[kWG] = kNoKey;
+        xs[W] = 0.0f;
+    }
     __syncthreads();
 
     uint32_t left = w.w;  // units of the run not yet started
-    Unit cur = make_unit<E>(tl, 0, w.z, left > 0);
+    Unit cur = make_unit(tl, 0, w.z, left > 0);
 
-    f32x4 pv[E / 4];
-    u16x8 pc[E / 8], pr[E / 8];
+    f32x4 pv[2];
+    u32x4 pg;
     f32x4 px[XR / 4];
     // Loads of unit u; an invalid u re-reads the start of the current tile
-    // (every address stays inside the arrays, so the wait counts are exact).
+    // and lanes past the unit's end re-read lane 0's entries (every address
+    // stays inside the unit, so the wait counts are exact).
     auto prefetch = [&](const Unit &u, uint32_t fallback_ti) {
         const uint4 t = tl[u.valid ? u.ti : fallback_ti];
         const uint64_t off = ((uint64_t)t.w << 32) | t.z;
-        uint64_t e = off + (uint64_t)(u.valid ? u.c : 0u) * UE + (uint64_t)tid * E;
-        if ((uint64_t)tid * E >= (u.valid ? u.n : 1u)) e = off;  // stay inside the tile
-#pragma unroll
-        for (int m = 0; m < E / 4; ++m) pv[m] = ldnt(reinterpret_cast<const f32x4 *>(tval + e) + m);
-#pragma unroll
-        for (int m = 0; m < E / 8; ++m) {
-            pc[m] = ldnt(reinterpret_cast<const u16x8 *>(tcol + e) + m);
-            pr[m] = ldnt(reinterpret_cast<const u16x8 *>(trow + e) + m);
-        }
+        const uint32_t c = u.valid ? u.c : 0u;
+        const uint32_t n8 = (min(kUE, t.y - c * kUE) + 7u) & ~7u;
+        const uint8_t *base = tdata + 6 * (off + (uint64_t)c * kUE);
+        const uint32_t le = (tid * kE < (u.valid ? u.n : 1u)) ? tid * kE : 0u;
+        pv[0] = ldnt(reinterpret_cast<const f32x4 *>(base) + le / 4);
+        pv[1] = ldnt(reinterpret_cast<const f32x4 *>(base) + le / 4 + 1);
+        pg = ldnt(reinterpret_cast<const u32x4 *>(base + 4 * n8) + le / 8);
         // x is padded to whole segments (xlen >= nseg * W): no clamping.
         const float *xb = x + (uint64_t)t.x * W;
 #pragma unroll
@@ -129,29 +162,32 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t R, uint32_t 
     if (cur.valid) prefetch(cur, 0);
     bool fresh = true;  // first unit of the run: its x segment is not staged yet
     while (cur.valid) {
-        f32x4 cv[E / 4];
-        u16x8 cc[E / 8], cr[E / 8];
+        const f32x4 cv0 = pv[0], cv1 = pv[1];
+        const u32x4 cg = pg;
         f32x4 cx[XR / 4];
-#pragma unroll
-        for (int m = 0; m < E / 4; ++m) cv[m] = pv[m];
-#pragma unroll
-        for (int m = 0; m < E / 8; ++m) {
-            cc[m] = pc[m];
-            cr[m] = pr[m];
-        }
 #pragma unroll
         for (int k = 0; k < XR / 4; ++k) cx[k] = px[k];
         --left;
-        const bool more_in_tile = (cur.c + 1) * UE < tl[cur.ti].y;
-        const Unit nxt = make_unit<E>(tl, more_in_tile ? cur.ti : cur.ti + 1, more_in_tile ? cur.c + 1 : 0u,
-                                      left > 0);
+        const bool more_in_tile = (cur.c + 1) * kUE < tl[cur.ti].y;
+        const Unit nxt = make_unit(tl, more_in_tile ? cur.ti : cur.ti + 1, more_in_tile ? cur.c + 1 : 0u, left > 0);
         prefetch(nxt, cur.ti);  // in flight while this unit is computed
 
-        // Keys of this work-item's entries; entries past n get kNoKey.
-        const uint32_t e0 = tid * E;
-        uint32_t key[E + 1];
+        // Keys of this work-item's entries; entries past n get kNoKey.  (The
+        // clamps to R-1 and W only keep LDS addresses inside the block's
+        // arrays; a well-formed group word never needs them.)
+        const uint32_t e0 = tid * kE;
+        uint32_t key[kE + 1];
+        {
+            uint32_t k = cg.x & 0xffffu;
+            key[0] = min(k, R - 1);
 #pragma unroll
-        for (int m = 0; m < E; ++m) key[m] = (e0 + m < cur.n) ? (uint32_t)cr[m / 8][m % 8] : kNoKey;
+            for (int j = 1; j < 8; ++j) {
+                k += (cg.x >> (14 + 2 * j)) & 3u;
+                key[j] = min(k, R - 1);
+            }
+        }
+#pragma unroll
+        for (int m = 0; m < (int)kE; ++m) key[m] = (e0 + m < cur.n) ? key[m] : kNoKey;
         s_k0[tid] = key[0];
         if (cur.c == 0 || fresh) {  // a new tile: its x segment
 #pragma unroll
@@ -159,14 +195,15 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t R, uint32_t 
         }
         fresh = false;
         __syncthreads();
-        key[E] = s_k0[tid + 1];
+        key[kE] = s_k0[tid + 1];
 
-        // Products first (all LDS gathers in flight together; padding and
-        // masked entries read a valid column and are zeroed).
-        float prod[E];
+        // Products first (all LDS gathers in flight together; masked
+        // entries read a valid column and are zeroed).
+        const float v[8] = {cv0.x, cv0.y, cv0.z, cv0.w, cv1.x, cv1.y, cv1.z, cv1.w};
+        float prod[kE];
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
-            const float p = cv[m / 4][m % 4] * xs[cc[m / 8][m % 8]];
+        for (int m = 0; m < (int)kE; ++m) {
+            const float p = v[m] * xs[min(group_col<CB>(cg, m), (uint32_t)W)];
             prod[m] = key[m] != kNoKey ? p : 0.0f;
         }
         // Walk my entries: a row ends where the key changes.  A row that
@@ -175,7 +212,7 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t R, uint32_t 
         float a = 0.0f, head = 0.0f;
         uint32_t has_end = 0, first_key = 0;
 #pragma unroll
-        for (int m = 0; m < E; ++m) {
+        for (int m = 0; m < (int)kE; ++m) {
             a += prod[m];
             const bool end = (key[m] != key[m + 1]) && (key[m] != kNoKey);
             if (end && has_end) acc[key[m]] += a;
@@ -228,6 +265,17 @@ __global__ void blocked_reduce_kernel(const float *__restrict__ part, const uint
 }
 
 // ----------------------------------------------------------------- builder
+// Rows must be sorted by column for the segment runs to be contiguous.
+__global__ __launch_bounds__(kWG) void rows_sorted_kernel(const uint64_t *__restrict__ rowptr,
+                                                          const uint32_t *__restrict__ col, uint32_t rows,
+                                                          uint32_t *__restrict__ bad) {
+    const uint32_t row = blockIdx.x * (kWG / 64) + threadIdx.x / 64;
+    const uint32_t lane = threadIdx.x % 64;
+    if (row >= rows) return;
+    for (uint64_t k = rowptr[row] + 1 + lane; k < rowptr[row + 1]; k += 64)
+        if (col[k] < col[k - 1]) atomicOr(bad, 2u);
+}
+
 // cnt[r*nseg + s] = entries of row r in segment s.
 __global__ __launch_bounds__(kWG) void tile_count_kernel(const uint64_t *__restrict__ rowptr,
                                                          const uint32_t *__restrict__ col, uint32_t rows,
@@ -240,20 +288,30 @@ __global__ __launch_bounds__(kWG) void tile_count_kernel(const uint64_t *__restr
         atomicAdd(cnt + (uint64_t)row * nseg + (col[k] >> wshift), 1u);
 }
 
-// Per tile (one work-item per tile): rowpos[r*nseg+s] = entries of tile
-// (b, s) in the rows of block b before r; exact and padded tile sizes.
+// Per tile (one work-item per tile): rowpos[r*nseg+s] = position of row r's
+// run in tile (b, s); tile_nz = entries including step padding, padded to 8
+// in tile_padded.  A row whose key exceeds the previous non-empty row's by
+// more than kMaxRowStep starts a new group of 8 (the rest of the group is
+// padding), so every key inside a group is its predecessor plus 0..3.
 __global__ void tile_rowpos_kernel(const uint32_t *__restrict__ cnt, uint32_t rows, uint32_t R, uint32_t nseg,
                                    uint64_t ntiles, uint32_t *__restrict__ rowpos, uint32_t *__restrict__ tile_nz,
                                    uint64_t *__restrict__ tile_padded) {
     const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
     const uint32_t b = (uint32_t)(t / nseg), s = (uint32_t)(t % nseg);
-    uint32_t run = 0;
+    uint32_t run = 0, last = 0;
+    bool any = false;
     for (uint32_t i = 0; i < R; ++i) {
         const uint64_t r = (uint64_t)b * R + i;
         if (r >= rows) break;
+        const uint32_t n = cnt[r * nseg + s];
+        if (n) {
+            if (any && i - last > kMaxRowStep) run = (run + 7u) & ~7u;
+            any = true;
+            last = i;
+        }
         rowpos[r * nseg + s] = run;
-        run += cnt[r * nseg + s];
+        run += n;
     }
     tile_nz[t] = run;
     tile_padded[t] = (run + 7u) & ~7u;
@@ -272,6 +330,8 @@ __global__ void row_segscan_kernel(uint32_t *__restrict__ cnt, uint32_t rows, ui
     }
 }
 
+// Places every entry: its value straight into the unit it falls in, its
+// local column and row key into scratch (by entry offset) for the packing.
 __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__restrict__ rowptr,
                                                            const uint32_t *__restrict__ col,
                                                            const float *__restrict__ val, uint32_t rows, uint32_t R,
@@ -279,8 +339,8 @@ __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__res
                                                            const uint32_t *__restrict__ segoff,
                                                            const uint32_t *__restrict__ rowpos,
                                                            const uint64_t *__restrict__ tile_off,
-                                                           float *__restrict__ tval, uint16_t *__restrict__ tcol,
-                                                           uint16_t *__restrict__ trow) {
+                                                           uint8_t *__restrict__ tdata, uint16_t *__restrict__ ctmp,
+                                                           uint16_t *__restrict__ rtmp) {
     const uint32_t row = blockIdx.x * (kWG / 64) + threadIdx.x / 64;
     const uint32_t lane = threadIdx.x % 64;
     if (row >= rows) return;
@@ -291,55 +351,129 @@ __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__res
         const uint32_t s = c >> wshift;
         const uint64_t t = (uint64_t)b * nseg + s;
         const uint64_t rsn = (uint64_t)row * nseg + s;
-        const uint64_t pos = tile_off[t] + rowpos[rsn] + (k - rs - segoff[rsn]);
-        tval[pos] = val[k];
-        tcol[pos] = (uint16_t)(c & ((1u << wshift) - 1u));
-        trow[pos] = (uint16_t)i;
+        const uint64_t off = tile_off[t];
+        const uint64_t p = rowpos[rsn] + (k - rs - segoff[rsn]);  // position in the tile
+        const uint64_t u = p / kUE, q = p % kUE;
+        reinterpret_cast<float *>(tdata + 6 * (off + u * kUE))[q] = val[k];
+        ctmp[off + p] = (uint16_t)(c & ((1u << wshift) - 1u));
+        rtmp[off + p] = (uint16_t)i;
     }
 }
 
-// Rows must be sorted by column for the segment runs to be contiguous.
-__global__ __launch_bounds__(kWG) void rows_sorted_kernel(const uint64_t *__restrict__ rowptr,
-                                                          const uint32_t *__restrict__ col, uint32_t rows,
-                                                          uint32_t *__restrict__ bad) {
-    const uint32_t row = blockIdx.x * (kWG / 64) + threadIdx.x / 64;
-    const uint32_t lane = threadIdx.x % 64;
-    if (row >= rows) return;
-    for (uint64_t k = rowptr[row] + 1 + lane; k < rowptr[row + 1]; k += 64)
-        if (col[k] < col[k - 1]) atomicOr(bad, 2u);
+// One work-item per group of 8 entries (entry offset 8*q): finds its tile,
+// packs the group word (layout at the top of this file) into its unit.
+// Padding (rtmp = kPadRow) repeats the previous key with column W.  bit 4
+// of *bad: a step above kMaxRowStep (the layout guarantees none).
+template <int W>
+__global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restrict__ ctmp,
+                                                        const uint16_t *__restrict__ rtmp,
+                                                        const uint64_t *__restrict__ tile_off, uint64_t ntiles,
+                                                        uint8_t *__restrict__ tdata, uint32_t *__restrict__ bad) {
+    constexpr int CB = col_bits<W>();
+    const uint64_t q = blockIdx.x * (uint64_t)kWG + threadIdx.x;
+    const uint64_t L = 8 * q;
+    if (L >= tile_off[ntiles]) return;
+    // Last tile t with tile_off[t] <= L (empty tiles share their successor's
+    // offset and come first, so this is the non-empty tile holding L).
+    uint64_t lo = 0, hi = ntiles;  // tile_off[lo] <= L < tile_off[hi]
+    while (hi - lo > 1) {
+        const uint64_t mid = (lo + hi) / 2;
+        if (tile_off[mid] <= L)
+            lo = mid;
+        else
+            hi = mid;
+    }
+    const uint64_t off = tile_off[lo], n8 = tile_off[lo + 1] - off;
+    const uint64_t p = L - off, u = p / kUE, e = p % kUE;
+    const uint64_t n8u = std::min<uint64_t>(kUE, n8 - u * kUE);
+
+    const u16x8 r = *reinterpret_cast<const u16x8 *>(rtmp + L);
+    const u16x8 c = *reinterpret_cast<const u16x8 *>(ctmp + L);
+    uint32_t prev = r[0] == kPadRow ? 0u : r[0];
+    uint32_t w0 = prev, big = 0;
+    uint32_t cols[8];
+    cols[0] = r[0] == kPadRow ? (uint32_t)W : c[0];
+#pragma unroll
+    for (int j = 1; j < 8; ++j) {
+        const uint32_t v = r[j];
+        const uint32_t d = v == kPadRow ? 0u : v - prev;
+        prev = v == kPadRow ? prev : v;
+        big |= d > kMaxRowStep ? 1u : 0u;
+        w0 |= (d & 3u) << (14 + 2 * j);
+        cols[j] = v == kPadRow ? (uint32_t)W : c[j];
+    }
+    uint64_t lo64 = 0;
+    uint32_t w3 = 0;
+#pragma unroll
+    for (int j = 0; j < 8; ++j) {
+        const int bit = j * CB;
+        const uint64_t cv = cols[j];
+        if (bit < 64) lo64 |= cv << bit;
+        if (bit + CB > 64) w3 |= (uint32_t)(bit >= 64 ? cv << (bit - 64) : cv >> (64 - bit));
+    }
+    u32x4 word;
+    word.x = w0;
+    word.y = (uint32_t)lo64;
+    word.z = (uint32_t)(lo64 >> 32);
+    word.w = w3;
+    *reinterpret_cast<u32x4 *>(tdata + 6 * (off + u * kUE) + 4 * n8u + 2 * e) = word;
+    if (big) atomicOr(bad, 4u);
+}
+
+__global__ void fill_u16_kernel(uint16_t *__restrict__ p, uint16_t v, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        p[i] = v;
 }
 
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-size_t lds_bytes(uint32_t W, uint32_t R, uint32_t span) { return ((size_t)W + R) * 4 + (size_t)span * 16; }
+size_t lds_bytes(uint32_t W, uint32_t R, uint32_t span) { return ((size_t)W + 4 + R) * 4 + (size_t)span * 16; }
 
-template <int E, int W>
+template <int W>
 hipError_t launch_w(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
                     hipStream_t s) {
-    hipLaunchKernelGGL((blocked_spmv_kernel<E, W>), dim3(wk.nwg), dim3(kWG), lds_bytes(W, p.R, wk.span), s, p.R,
-                       wk.span, wk.ntl, wk.wl, wk.tiles, tm.tval, tm.tcol, tm.trow, x, part);
+    hipLaunchKernelGGL((blocked_spmv_kernel<W>), dim3(wk.nwg), dim3(kWG), lds_bytes(W, p.R, wk.span), s, p.R,
+                       wk.span, wk.ntl, wk.wl, wk.tiles, tm.tdata, x, part);
     return hipGetLastError();
 }
-
-template <int E>
-hipError_t launch_e(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
-                    hipStream_t s) {
-    switch (p.W) {
-        case 1024: return launch_w<E, 1024>(p, tm, wk, x, part, s);
-        case 2048: return launch_w<E, 2048>(p, tm, wk, x, part, s);
-        case 4096: return launch_w<E, 4096>(p, tm, wk, x, part, s);
-        default: return launch_w<E, 8192>(p, tm, wk, x, part, s);
-    }
-}
-
-uint64_t padded_capacity(const TilePlan &p) { return p.nnz + 7 * p.ntiles + 16; }
 
 size_t scan_bytes(const TilePlan &p) {
     size_t scan = 0;
     (void)rocprim::exclusive_scan(nullptr, scan, (const uint64_t *)nullptr, (uint64_t *)nullptr, uint64_t(0),
                                   (size_t)p.ntiles + 1, rocprim::plus<uint64_t>());
     return scan;
+}
+
+// Count-phase scratch: cnt and rowpos [rows*nseg], tile_padded, tile_off and
+// tile_nz of the count, the scan's temporary storage, the flag word.
+struct CountScratch {
+    uint32_t *cnt, *rowpos;
+    uint64_t *tile_padded, *tile_off;
+    uint32_t *tile_nz;
+    void *scan_tmp;
+    size_t scan_size;
+    uint32_t *bad;
+};
+
+CountScratch count_scratch(const TilePlan &p, void *scratch) {
+    CountScratch c;
+    char *q = static_cast<char *>(scratch);
+    c.cnt = reinterpret_cast<uint32_t *>(q);
+    q += align256(4ull * p.rows * p.nseg);
+    c.rowpos = reinterpret_cast<uint32_t *>(q);
+    q += align256(4ull * p.rows * p.nseg);
+    c.tile_padded = reinterpret_cast<uint64_t *>(q);
+    q += align256(8 * (p.ntiles + 1));
+    c.tile_off = reinterpret_cast<uint64_t *>(q);
+    q += align256(8 * (p.ntiles + 1));
+    c.tile_nz = reinterpret_cast<uint32_t *>(q);
+    q += align256(4 * p.ntiles);
+    c.scan_size = scan_bytes(p);
+    c.scan_tmp = q;
+    q += align256(c.scan_size);
+    c.bad = reinterpret_cast<uint32_t *>(q);
+    return c;
 }
 
 }  // namespace
@@ -357,24 +491,20 @@ TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint3
     return p;
 }
 
-bool valid_tile_width(uint32_t W) { return W == 1024 || W == 2048 || W == 4096 || W == 8192; }
+bool valid_tile_width(uint32_t W) { return W == 1024 || W == 2048; }
 bool valid_tile_rows(uint32_t R) { return R == 512 || R == 1024 || R == 2048 || R == 4096 || R == 8192 || R == 16384; }
-bool valid_tile_unit(uint32_t E) { return E == 8 || E == 16; }
 
 size_t tiled_lds_bytes(const TilePlan &p, uint32_t span) { return lds_bytes(p.W, p.R, span); }
 
 size_t tiled_matrix_bytes(const TilePlan &p) {
-    const uint64_t cap = padded_capacity(p);
-    return align256(8 * (p.ntiles + 1)) + align256(4 * p.ntiles) + align256(4 * cap) + 2 * align256(2 * cap);
+    return align256(8 * (p.ntiles + 1)) + align256(4 * p.ntiles) + align256(6 * p.cap);
 }
 
 size_t tiled_part_bytes(const TilePlan &p, uint32_t nwg) { return sizeof(float) * (size_t)nwg * p.R; }
 
-TileWorkList tile_work_list(const TilePlan &p, const uint32_t *nz, const uint64_t *off, uint32_t E,
-                            uint32_t target_wgs) {
-    const uint64_t UE = (uint64_t)kWG * E;
+TileWorkList tile_work_list(const TilePlan &p, const uint32_t *nz, const uint64_t *off, uint32_t target_wgs) {
+    const uint64_t UE = kUE;
     TileWorkList L;
-    L.E = E;
     uint64_t total = 0;
     for (uint64_t t = 0; t < p.ntiles; ++t) total += (nz[t] + UE - 1) / UE;
     const uint64_t per_run = std::max<uint64_t>(1, (total + std::max(1u, target_wgs) - 1) / std::max(1u, target_wgs));
@@ -425,76 +555,98 @@ TileWorkList tile_work_list(const TilePlan &p, const uint32_t *nz, const uint64_
     return L;
 }
 
-size_t tile_build_scratch_bytes(const TilePlan &p) {
-    return 2 * align256(4ull * p.rows * p.nseg) + align256(8 * (p.ntiles + 1)) + align256(scan_bytes(p)) + 256;
+size_t tile_count_scratch_bytes(const TilePlan &p) {
+    return 2 * align256(4ull * p.rows * p.nseg) + 2 * align256(8 * (p.ntiles + 1)) + align256(4 * p.ntiles) +
+           align256(scan_bytes(p)) + 256;
 }
 
+size_t tile_fill_scratch_bytes(const TilePlan &p) { return 2 * align256(2 * p.cap); }
+
 TiledMatrix tiled_matrix_view(const TilePlan &p, void *mem) {
-    const uint64_t cap = padded_capacity(p);
     TiledMatrix t;
     char *c = static_cast<char *>(mem);
     t.tile_off = reinterpret_cast<uint64_t *>(c);
     c += align256(8 * (p.ntiles + 1));
     t.tile_nz = reinterpret_cast<uint32_t *>(c);
     c += align256(4 * p.ntiles);
-    t.tval = reinterpret_cast<float *>(c);
-    c += align256(4 * cap);
-    t.tcol = reinterpret_cast<uint16_t *>(c);
-    c += align256(2 * cap);
-    t.trow = reinterpret_cast<uint16_t *>(c);
+    t.tdata = reinterpret_cast<uint8_t *>(c);
     return t;
 }
 
-hipError_t launch_tile_build(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
-                             void *mem, void *scratch, uint32_t *flags_host, hipStream_t s) {
-    TiledMatrix tm = tiled_matrix_view(p, mem);
-    const uint64_t cap = padded_capacity(p);
-    char *c = static_cast<char *>(scratch);
-    uint32_t *cnt = reinterpret_cast<uint32_t *>(c);
-    c += align256(4ull * p.rows * p.nseg);
-    uint32_t *rowpos = reinterpret_cast<uint32_t *>(c);
-    c += align256(4ull * p.rows * p.nseg);
-    uint64_t *tile_padded = reinterpret_cast<uint64_t *>(c);
-    c += align256(8 * (p.ntiles + 1));
-    size_t sb = scan_bytes(p);
-    void *scan_tmp = c;
-    c += align256(sb);
-    uint32_t *bad = reinterpret_cast<uint32_t *>(c);
+hipError_t launch_tile_count(TilePlan &p, const uint64_t *rowptr, const uint32_t *col, void *scratch,
+                             uint32_t *flags_host, hipStream_t s) {
+    CountScratch c = count_scratch(p, scratch);
     uint32_t wshift = 0;
     while ((1u << wshift) < p.W) ++wshift;
-
     hipError_t e;
-    if ((e = hipMemsetAsync(bad, 0, sizeof(uint32_t), s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(cnt, 0, 4ull * p.rows * p.nseg, s)) != hipSuccess) return e;
+    *flags_host = 0;
+    if ((e = hipMemsetAsync(c.bad, 0, sizeof(uint32_t), s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(c.cnt, 0, 4ull * p.rows * p.nseg, s)) != hipSuccess) return e;
     if (p.rows) {
         hipLaunchKernelGGL(rows_sorted_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, col,
-                           p.rows, bad);
+                           p.rows, c.bad);
         hipLaunchKernelGGL(tile_count_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, col,
-                           p.rows, p.nseg, wshift, cnt);
+                           p.rows, p.nseg, wshift, c.cnt);
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
-    // The scatter is only correct (and only in bounds) for sorted rows:
-    // check the flag first.
-    if ((e = hipMemcpyAsync(flags_host, bad, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
+    // The layout is only correct (and the fill only in bounds) for sorted
+    // rows: check the flag first.
+    if ((e = hipMemcpyAsync(flags_host, c.bad, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
     if (*flags_host) return hipSuccess;
-    hipLaunchKernelGGL(tile_rowpos_kernel, dim3(blocks_for(p.ntiles, kWG)), dim3(kWG), 0, s, cnt, p.rows, p.R,
-                       p.nseg, p.ntiles, rowpos, tm.tile_nz, tile_padded);
-    if ((e = hipGetLastError()) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(tile_padded + p.ntiles, 0, sizeof(uint64_t), s)) != hipSuccess) return e;
-    if ((e = rocprim::exclusive_scan(scan_tmp, sb, tile_padded, tm.tile_off, uint64_t(0), (size_t)p.ntiles + 1,
-                                     rocprim::plus<uint64_t>(), s)) != hipSuccess)
-        return e;
-    // Padding entries: value 0 (their keys are masked by the entry count).
-    if ((e = hipMemsetAsync(tm.tval, 0, 4 * cap, s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(tm.tcol, 0, 2 * cap, s)) != hipSuccess) return e;
-    if ((e = hipMemsetAsync(tm.trow, 0, 2 * cap, s)) != hipSuccess) return e;
-    if (p.rows) {
-        hipLaunchKernelGGL(row_segscan_kernel, dim3(blocks_for(p.rows, kWG)), dim3(kWG), 0, s, cnt, p.rows, p.nseg);
-        hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, col, val,
-                           p.rows, p.R, p.nseg, wshift, cnt, rowpos, tm.tile_off, tm.tval, tm.tcol, tm.trow);
+    if (p.ntiles) {
+        hipLaunchKernelGGL(tile_rowpos_kernel, dim3(blocks_for(p.ntiles, kWG)), dim3(kWG), 0, s, c.cnt, p.rows, p.R,
+                           p.nseg, p.ntiles, c.rowpos, c.tile_nz, c.tile_padded);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
+    if ((e = hipMemsetAsync(c.tile_padded + p.ntiles, 0, sizeof(uint64_t), s)) != hipSuccess) return e;
+    size_t sb = c.scan_size;
+    if ((e = rocprim::exclusive_scan(c.scan_tmp, sb, c.tile_padded, c.tile_off, uint64_t(0), (size_t)p.ntiles + 1,
+                                     rocprim::plus<uint64_t>(), s)) != hipSuccess)
+        return e;
+    uint64_t total = 0;
+    if ((e = hipMemcpyAsync(&total, c.tile_off + p.ntiles, sizeof(uint64_t), hipMemcpyDeviceToHost, s)) != hipSuccess)
+        return e;
+    if ((e = hipStreamSynchronize(s)) != hipSuccess) return e;
+    p.cap = total + 8;  // a multiple of 8, at least one group
+    return hipSuccess;
+}
+
+hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
+                            void *mem, void *scratch, void *fill_scratch, uint32_t *flags_host, hipStream_t s) {
+    CountScratch c = count_scratch(p, scratch);
+    TiledMatrix tm = tiled_matrix_view(p, mem);
+    uint16_t *ctmp = static_cast<uint16_t *>(fill_scratch);
+    uint16_t *rtmp = reinterpret_cast<uint16_t *>(static_cast<char *>(fill_scratch) + align256(2 * p.cap));
+    uint32_t wshift = 0;
+    while ((1u << wshift) < p.W) ++wshift;
+    hipError_t e;
+    *flags_host = 0;
+    if ((e = hipMemcpyAsync(tm.tile_off, c.tile_off, 8 * (p.ntiles + 1), hipMemcpyDeviceToDevice, s)) != hipSuccess)
+        return e;
+    if (p.ntiles &&
+        (e = hipMemcpyAsync(tm.tile_nz, c.tile_nz, 4 * p.ntiles, hipMemcpyDeviceToDevice, s)) != hipSuccess)
+        return e;
+    // Padding entries: value 0 (and, in the packing, column W).
+    if ((e = hipMemsetAsync(tm.tdata, 0, 6 * p.cap, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(rtmp, 0xff, 2 * p.cap, s)) != hipSuccess) return e;
+    hipLaunchKernelGGL(fill_u16_kernel, dim3(1024), dim3(kWG), 0, s, ctmp, (uint16_t)p.W, p.cap);
+    if (p.rows) {
+        hipLaunchKernelGGL(row_segscan_kernel, dim3(blocks_for(p.rows, kWG)), dim3(kWG), 0, s, c.cnt, p.rows, p.nseg);
+        hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, col, val,
+                           p.rows, p.R, p.nseg, wshift, c.cnt, c.rowpos, tm.tile_off, tm.tdata, ctmp, rtmp);
+    }
+    if (p.ntiles) {
+        const dim3 grid(blocks_for(p.cap / 8, kWG));
+        if (p.W == 1024)
+            hipLaunchKernelGGL(tile_pack_kernel<1024>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles,
+                               tm.tdata, c.bad);
+        else
+            hipLaunchKernelGGL(tile_pack_kernel<2048>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles,
+                               tm.tdata, c.bad);
+    }
+    if ((e = hipGetLastError()) != hipSuccess) return e;
+    if ((e = hipMemcpyAsync(flags_host, c.bad, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
     return hipStreamSynchronize(s);
 }
 
@@ -502,7 +654,8 @@ hipError_t launch_tiled_spmv(const TilePlan &p, const TiledMatrix &tm, const Til
                              float *part, Epi epi, const EpiArgs &ea, hipStream_t s) {
     if (p.rows == 0) return hipSuccess;
     if (wk.nwg) {
-        hipError_t e = (wk.E == 8) ? launch_e<8>(p, tm, wk, x, part, s) : launch_e<16>(p, tm, wk, x, part, s);
+        const hipError_t e =
+            p.W == 1024 ? launch_w<1024>(p, tm, wk, x, part, s) : launch_w<2048>(p, tm, wk, x, part, s);
         if (e != hipSuccess) return e;
     }
     uint32_t rshift = 0;
@@ -510,10 +663,12 @@ hipError_t launch_tiled_spmv(const TilePlan &p, const TiledMatrix &tm, const Til
     const dim3 grid(blocks_for(p.rows, kWG)), block(kWG);
     switch (epi) {
         case EPI_PLAIN:
-            hipLaunchKernelGGL(blocked_reduce_kernel<EPI_PLAIN>, grid, block, 0, s, part, wk.wl_first, rshift, p.rows, ea);
+            hipLaunchKernelGGL(blocked_reduce_kernel<EPI_PLAIN>, grid, block, 0, s, part, wk.wl_first, rshift, p.rows,
+                               ea);
             break;
         case EPI_RATIO:
-            hipLaunchKernelGGL(blocked_reduce_kernel<EPI_RATIO>, grid, block, 0, s, part, wk.wl_first, rshift, p.rows, ea);
+            hipLaunchKernelGGL(blocked_reduce_kernel<EPI_RATIO>, grid, block, 0, s, part, wk.wl_first, rshift, p.rows,
+                               ea);
             break;
         default:
             hipLaunchKernelGGL(blocked_reduce_kernel<EPI_UPDATE>, grid, block, 0, s, part, wk.wl_first, rshift, p.rows,

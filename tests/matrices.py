@@ -48,7 +48,15 @@ def make(kind, seed=0):
         return from_lengths(rng.integers(1, 30, 800), 20, seed, dup=True)
     if kind == "tiny":
         return from_lengths([2, 0, 3], 4, seed)
+    if kind == "row_gaps":
+        # Non-empty rows separated by 0..12 empty ones (row steps 1..13 inside
+        # a tile), rows of 1..20 entries over 5000 columns (several segments).
+        gaps = rng.integers(0, 13, 3000)
+        lens = np.zeros(int(gaps.sum()) + 3000, np.int64)
+        pos = np.cumsum(gaps + 1) - 1
+        lens[pos] = rng.integers(1, 21, 3000)
+        return from_lengths(lens, 5000, seed)
     raise ValueError(kind)
 
 
-KINDS = ["ragged", "heavy", "empty_rows", "one_huge_row", "all_empty", "single_row", "dups", "tiny"]
+KINDS = ["ragged", "heavy", "empty_rows", "one_huge_row", "all_empty", "single_row", "dups", "tiny", "row_gaps"]
