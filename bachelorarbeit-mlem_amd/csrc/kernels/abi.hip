@@ -168,8 +168,10 @@ struct mlem_matrix {
     }
 };
 
+// Per profiled iteration: FP start, FP SpMV kernel done, FP done, BP SpMV
+// kernel done, BP done.
 struct IterEvents {
-    hipEvent_t e[3];
+    hipEvent_t e[5];
 };
 
 struct mlem_recon {
@@ -185,7 +187,7 @@ struct mlem_recon {
     bool bp_merge = true;  // BP on A^T: merge-based or CSR-vector
     std::vector<IterEvents> events;  // pool, one set per profiled iteration
     size_t events_used = 0;
-    double fp_ms = 0, bp_ms = 0;
+    double phase_ms[4] = {0, 0, 0, 0};  // FP SpMV, FP epilogue, BP SpMV, BP epilogue
     uint32_t timed = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -297,15 +299,16 @@ int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t target, const TileWorkDev *
 }
 
 // Chooses the blocked (LDS-tiled) or the plain merge path for one side of
-// the iteration.  The blocked path runs about tile_groups workgroups (0: 32
-// per CU); secsize applies to the plain merge path only.
+// the iteration.  The blocked path (default tiles of 2048 rows x 1024
+// columns) runs about tile_groups workgroups (0: 20 per CU); secsize applies
+// to the plain merge path only.
 int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t secsize, const Tiled **tiled,
                      const TileWorkDev **work, DevBuf *part, const Partition **partn, DevBuf *scratch) {
     *tiled = nullptr;
     *work = nullptr;
     if (o.tile_width != 1) {
         const uint32_t W = o.tile_width == 0 ? 1024u : o.tile_width;
-        const uint32_t R = o.tile_rows == 0 ? 4096u : o.tile_rows;
+        const uint32_t R = o.tile_rows == 0 ? 2048u : o.tile_rows;
         uint32_t flags = 0;
         int st = get_tiled(m, which, W, R, tiled, &flags);
         int dev = 0, lds_max = 65536, cus = 256;
@@ -314,7 +317,7 @@ int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t sec
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         }
         if (st == MLEM_OK) {
-            const uint32_t target = o.tile_groups ? o.tile_groups : 32u * (uint32_t)std::max(1, cus);
+            const uint32_t target = o.tile_groups ? o.tile_groups : 20u * (uint32_t)std::max(1, cus);
             st = get_tile_work(m, const_cast<Tiled *>(*tiled), target, work);
         }
         if (st == MLEM_OK && tiled_lds_bytes((*tiled)->plan, (*work)->view.span) + 4096 > (size_t)lds_max) {
@@ -340,57 +343,69 @@ hipError_t enqueue_iteration(mlem_recon *r, IterEvents *ev) {
     const mlem_matrix *m = r->m;
     hipStream_t s = r->stream;
     hipError_t e;
-    if (ev && (e = hipEventRecord(ev->e[0], s)) != hipSuccess) return e;
+    auto mark = [&](int i) { return ev ? hipEventRecord(ev->e[i], s) : hipSuccess; };
+    if ((e = mark(0)) != hipSuccess) return e;
     // FP + ratio
     EpiArgs fa{};
     fa.out = r->ratio.as<float>();
     fa.g = r->g.as<int32_t>();
-    if (r->tfp)
-        e = launch_tiled_spmv(r->tfp->plan, r->tfp->view, r->wfp->view, r->f.as<float>(), r->part_fp.as<float>(),
-                              EPI_RATIO, fa, s);
-    else if (r->opts.spmv == 0)
-        e = launch_merge_spmv(r->pfp->plan, r->pfp->buf.p, r->scratch_fp.p, m->rowptr.as<uint64_t>(),
-                              m->col.as<uint32_t>(), m->val.as<float>(), r->f.as<float>(), EPI_RATIO, fa, s);
-    else
-        e = launch_csr_vector(r->lanes_fp, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(),
-                              r->f.as<float>(), m->rows, EPI_RATIO, fa, s);
+    if (r->tfp) {
+        if ((e = launch_tiled_partials(r->tfp->plan, r->tfp->view, r->wfp->view, r->f.as<float>(),
+                                       r->part_fp.as<float>(), s)) != hipSuccess)
+            return e;
+        if ((e = mark(1)) != hipSuccess) return e;
+        e = launch_tiled_reduce(r->tfp->plan, r->wfp->view, r->part_fp.as<float>(), EPI_RATIO, fa, s);
+    } else {
+        if (r->opts.spmv == 0)
+            e = launch_merge_spmv(r->pfp->plan, r->pfp->buf.p, r->scratch_fp.p, m->rowptr.as<uint64_t>(),
+                                  m->col.as<uint32_t>(), m->val.as<float>(), r->f.as<float>(), EPI_RATIO, fa, s);
+        else
+            e = launch_csr_vector(r->lanes_fp, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(),
+                                  r->f.as<float>(), m->rows, EPI_RATIO, fa, s);
+        if (e == hipSuccess) e = mark(1);
+    }
     if (e != hipSuccess) return e;
-    if (ev && (e = hipEventRecord(ev->e[1], s)) != hipSuccess) return e;
+    if ((e = mark(2)) != hipSuccess) return e;
     // BP + update
     EpiArgs ba{};
     ba.norm = r->sens.as<float>();
     ba.f = r->f.as<float>();
     if (r->opts.using_trans == 0) {
-        if (r->tbp)
-            e = launch_tiled_spmv(r->tbp->plan, r->tbp->view, r->wbp->view, r->ratio.as<float>(),
-                                  r->part_bp.as<float>(), EPI_UPDATE, ba, s);
-        else if (r->bp_merge)
-            e = launch_merge_spmv(r->pbp->plan, r->pbp->buf.p, r->scratch_bp.p, m->rowptrT.as<uint64_t>(),
-                                  m->colT.as<uint32_t>(), m->valT.as<float>(), r->ratio.as<float>(), EPI_UPDATE, ba,
-                                  s);
-        else
-            e = launch_csr_vector(r->lanes_bp, m->rowptrT.as<uint64_t>(), m->colT.as<uint32_t>(),
-                                  m->valT.as<float>(), r->ratio.as<float>(), m->cols, EPI_UPDATE, ba, s);
-        if (e != hipSuccess) return e;
+        if (r->tbp) {
+            if ((e = launch_tiled_partials(r->tbp->plan, r->tbp->view, r->wbp->view, r->ratio.as<float>(),
+                                           r->part_bp.as<float>(), s)) != hipSuccess)
+                return e;
+            if ((e = mark(3)) != hipSuccess) return e;
+            e = launch_tiled_reduce(r->tbp->plan, r->wbp->view, r->part_bp.as<float>(), EPI_UPDATE, ba, s);
+        } else {
+            if (r->bp_merge)
+                e = launch_merge_spmv(r->pbp->plan, r->pbp->buf.p, r->scratch_bp.p, m->rowptrT.as<uint64_t>(),
+                                      m->colT.as<uint32_t>(), m->valT.as<float>(), r->ratio.as<float>(), EPI_UPDATE,
+                                      ba, s);
+            else
+                e = launch_csr_vector(r->lanes_bp, m->rowptrT.as<uint64_t>(), m->colT.as<uint32_t>(),
+                                      m->valT.as<float>(), r->ratio.as<float>(), m->cols, EPI_UPDATE, ba, s);
+            if (e == hipSuccess) e = mark(3);
+        }
     } else {
         e = launch_scatter(r->lanes_fp, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(),
                            r->ratio.as<float>(), r->bp.as<float>(), m->rows, s);
-        if (e != hipSuccess) return e;
-        e = launch_update_image(r->f.as<float>(), r->sens.as<float>(), r->bp.as<float>(), m->cols, s);
-        if (e != hipSuccess) return e;
+        if (e == hipSuccess) e = mark(3);
+        if (e == hipSuccess)
+            e = launch_update_image(r->f.as<float>(), r->sens.as<float>(), r->bp.as<float>(), m->cols, s);
     }
-    if (ev && (e = hipEventRecord(ev->e[2], s)) != hipSuccess) return e;
-    return hipSuccess;
+    if (e != hipSuccess) return e;
+    return mark(4);
 }
 
 int collect_times(mlem_recon *r) {
     for (size_t i = 0; i < r->events_used; ++i) {
-        float a = 0, b = 0;
-        HIPCHK(hipEventSynchronize(r->events[i].e[2]));
-        HIPCHK(hipEventElapsedTime(&a, r->events[i].e[0], r->events[i].e[1]));
-        HIPCHK(hipEventElapsedTime(&b, r->events[i].e[1], r->events[i].e[2]));
-        r->fp_ms += a;
-        r->bp_ms += b;
+        HIPCHK(hipEventSynchronize(r->events[i].e[4]));
+        for (int k = 0; k < 4; ++k) {
+            float ms = 0;
+            HIPCHK(hipEventElapsedTime(&ms, r->events[i].e[k], r->events[i].e[k + 1]));
+            r->phase_ms[k] += ms;
+        }
         r->timed++;
     }
     r->events_used = 0;
@@ -563,6 +578,13 @@ int mlem_event_elapsed_ms(void *start, void *stop, float *ms) {
 int mlem_fill_f32(float *dst, float value, uint64_t n, void *stream) {
     ARGCHK(dst || n == 0, "mlem_fill_f32: null dst");
     HIPCHK(launch_fill(dst, value, n, S(stream)));
+    return MLEM_OK;
+}
+
+int mlem_copy_f32(float *dst, const float *src, uint64_t n, void *stream) {
+    ARGCHK(n == 0 || (dst && src), "mlem_copy_f32: null pointer");
+    ARGCHK(aligned16(dst) && aligned16(src), "mlem_copy_f32: pointers must be 16-byte aligned");
+    HIPCHK(launch_copy(dst, src, n, S(stream)));
     return MLEM_OK;
 }
 
@@ -1000,8 +1022,16 @@ int mlem_recon_set_image(mlem_recon *r, const float *f) {
 int mlem_recon_kernel_times(mlem_recon *r, double *fp_ms, double *bp_ms, uint32_t *iters_timed) {
     ARGCHK(r, "mlem_recon_kernel_times: null handle");
     if (int st = collect_times(r)) return st;
-    if (fp_ms) *fp_ms = r->fp_ms;
-    if (bp_ms) *bp_ms = r->bp_ms;
+    if (fp_ms) *fp_ms = r->phase_ms[0] + r->phase_ms[1];
+    if (bp_ms) *bp_ms = r->phase_ms[2] + r->phase_ms[3];
+    if (iters_timed) *iters_timed = r->timed;
+    return MLEM_OK;
+}
+
+int mlem_recon_phase_times(mlem_recon *r, double ms[4], uint32_t *iters_timed) {
+    ARGCHK(r && ms, "mlem_recon_phase_times: null argument");
+    if (int st = collect_times(r)) return st;
+    for (int k = 0; k < 4; ++k) ms[k] = r->phase_ms[k];
     if (iters_timed) *iters_timed = r->timed;
     return MLEM_OK;
 }
@@ -1009,7 +1039,7 @@ int mlem_recon_kernel_times(mlem_recon *r, double *fp_ms, double *bp_ms, uint32_
 int mlem_recon_reset_times(mlem_recon *r) {
     ARGCHK(r, "mlem_recon_reset_times: null handle");
     if (int st = collect_times(r)) return st;
-    r->fp_ms = r->bp_ms = 0;
+    for (double &v : r->phase_ms) v = 0;
     r->timed = 0;
     return MLEM_OK;
 }

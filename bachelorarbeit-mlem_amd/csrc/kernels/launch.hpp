@@ -61,6 +61,8 @@ hipError_t launch_update(const float *f, const float *norm, float *bwproj, uint3
 hipError_t launch_update_image(float *f, const float *norm, float *bp, uint32_t cols, hipStream_t s);
 hipError_t launch_fill(float *dst, float v, uint64_t n, hipStream_t s);
 hipError_t launch_fill_f64(double *dst, double v, uint64_t n, hipStream_t s);
+// dst = src (16-byte aligned), non-temporal streaming copy.
+hipError_t launch_copy(float *dst, const float *src, uint64_t n, hipStream_t s);
 hipError_t launch_colsum_rows_f64(const uint64_t *rowptrT, const float *valT, uint32_t cols, double *s64, float *s32,
                                   hipStream_t s);
 hipError_t launch_colsum_scatter_f64(const uint64_t *rowptr, const uint32_t *col, const float *val, uint32_t rows,
@@ -131,11 +133,14 @@ hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uin
 // tile_off.
 TileWorkList tile_work_list(const TilePlan &p, const uint32_t *tile_nz, const uint64_t *tile_off,
                             uint32_t target_wgs);
-// part holds tiled_part_bytes(p, wk.nwg).  x must be readable (and finite)
-// up to nseg * W floats: callers pad their vectors to whole segments
-// (kTileXPad).
-hipError_t launch_tiled_spmv(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x,
-                             float *part, Epi epi, const EpiArgs &ea, hipStream_t s);
+// y = A x in two launches: the partial row sums of every run into part
+// (tiled_part_bytes(p, wk.nwg)), then their reduction with the epilogue.
+// x must be readable (and finite) up to nseg * W floats: callers pad their
+// vectors to whole segments (kTileXPad).
+hipError_t launch_tiled_partials(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x,
+                                 float *part, hipStream_t s);
+hipError_t launch_tiled_reduce(const TilePlan &p, const TileWork &wk, const float *part, Epi epi, const EpiArgs &ea,
+                               hipStream_t s);
 
 // Device transpose (stable counting sort): see transpose.hip.
 size_t transpose_scratch_bytes(uint32_t rows, uint32_t cols, uint64_t nnz);

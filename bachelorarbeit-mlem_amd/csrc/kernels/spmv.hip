@@ -15,6 +15,8 @@
 //  * scatter_kernel<G>: BP without a transpose (calcBwProj_none_trans,
 //    kernel.cu:165-177,347-370), float atomics into L2.
 //  * elementwise ratio / update / fill, FP64 column sums, FP64 log-likelihood.
+#include <algorithm>
+
 #include "launch.hpp"
 
 namespace mlem {
@@ -339,6 +341,17 @@ __global__ void fill_kernel(T *__restrict__ dst, T v, uint64_t n) {
         dst[i] = v;
 }
 
+// dst = src over n floats, 16 bytes per work-item per step.
+__global__ __launch_bounds__(kWG) void copy_kernel(float *__restrict__ dst, const float *__restrict__ src,
+                                                   uint64_t n) {
+    typedef float f4 __attribute__((ext_vector_type(4)));
+    const uint64_t n4 = n / 4, stride = (uint64_t)gridDim.x * kWG;
+    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; i < n4; i += stride)
+        __builtin_nontemporal_store(__builtin_nontemporal_load(reinterpret_cast<const f4 *>(src) + i),
+                                    reinterpret_cast<f4 *>(dst) + i);
+    for (uint64_t i = 4 * n4 + blockIdx.x * (uint64_t)kWG + threadIdx.x; i < n; i += stride) dst[i] = src[i];
+}
+
 __global__ void f64_to_f32_kernel(const double *__restrict__ a, float *__restrict__ b, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
@@ -546,6 +559,14 @@ hipError_t launch_fill(float *dst, float v, uint64_t n, hipStream_t s) {
     uint64_t blocks = (n + kWG - 1) / kWG;
     if (blocks > 65536) blocks = 65536;
     hipLaunchKernelGGL(fill_kernel<float>, dim3((uint32_t)blocks), dim3(kWG), 0, s, dst, v, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy(float *dst, const float *src, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n / 4 + kWG - 1) / kWG;
+    blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, 8192));
+    hipLaunchKernelGGL(copy_kernel, dim3((uint32_t)blocks), dim3(kWG), 0, s, dst, src, n);
     return hipGetLastError();
 }
 

@@ -650,14 +650,15 @@ hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uin
     return hipStreamSynchronize(s);
 }
 
-hipError_t launch_tiled_spmv(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x,
-                             float *part, Epi epi, const EpiArgs &ea, hipStream_t s) {
+hipError_t launch_tiled_partials(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x,
+                                 float *part, hipStream_t s) {
+    if (p.rows == 0 || wk.nwg == 0) return hipSuccess;
+    return p.W == 1024 ? launch_w<1024>(p, tm, wk, x, part, s) : launch_w<2048>(p, tm, wk, x, part, s);
+}
+
+hipError_t launch_tiled_reduce(const TilePlan &p, const TileWork &wk, const float *part, Epi epi, const EpiArgs &ea,
+                               hipStream_t s) {
     if (p.rows == 0) return hipSuccess;
-    if (wk.nwg) {
-        const hipError_t e =
-            p.W == 1024 ? launch_w<1024>(p, tm, wk, x, part, s) : launch_w<2048>(p, tm, wk, x, part, s);
-        if (e != hipSuccess) return e;
-    }
     uint32_t rshift = 0;
     while ((1u << rshift) < p.R) ++rshift;
     const dim3 grid(blocks_for(p.rows, kWG)), block(kWG);

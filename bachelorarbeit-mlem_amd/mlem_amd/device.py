@@ -15,7 +15,8 @@ from . import (HIP_LIB_PATH, MLEM_OK, Csr, MlemError, Opts, _load, _ptr, cint, f
 __all__ = ["hip_lib", "device_count", "device_info", "DeviceArray", "Stream", "Event", "ResidentMatrix", "Recon",
            "mlem_run", "default_opts", "opts", "merge_workspace_bytes", "fp_merge", "bp_merge", "fp_csr_vector",
            "bp_csr_vector", "bp_scatter", "fp_ratio_merge", "fp_ratio_csr_vector", "bp_update_merge",
-           "bp_update_csr_vector", "ratio", "update", "colsum_f64", "colsum_scatter_f64", "fill_f32", "device_sync",
+           "bp_update_csr_vector", "ratio", "update", "colsum_f64", "colsum_scatter_f64", "fill_f32", "copy_f32",
+           "device_sync",
            "version"]
 
 _hip = None
@@ -47,6 +48,7 @@ def hip_lib() -> C.CDLL:
             "mlem_event_record": (cint, [vp, vp]),
             "mlem_event_elapsed_ms": (cint, [vp, vp, P(f32)]),
             "mlem_fill_f32": (cint, [vp, f32, u64, vp]),
+            "mlem_copy_f32": (cint, [vp, vp, u64, vp]),
             "mlem_ratio": (cint, [vp, vp, u32, vp]),
             "mlem_update": (cint, [vp, vp, vp, u32, vp]),
             "mlem_merge_workspace_bytes": (sz, [u32, u64, u32]),
@@ -75,6 +77,7 @@ def hip_lib() -> C.CDLL:
             "mlem_recon_get_image": (cint, [vp, vp]),
             "mlem_recon_set_image": (cint, [vp, vp]),
             "mlem_recon_kernel_times": (cint, [vp, P(f64), P(f64), P(u32)]),
+            "mlem_recon_phase_times": (cint, [vp, P(f64), P(u32)]),
             "mlem_recon_reset_times": (cint, [vp]),
             "mlem_recon_loglik": (cint, [vp, P(f64)]),
             "mlem_recon_stream": (vp, [vp]),
@@ -242,6 +245,13 @@ def fill_f32(dst: DeviceArray, value: float, stream=None):
     _check(hip_lib().mlem_fill_f32(_dp(dst), value, dst.n, _sh(stream)))
 
 
+def copy_f32(dst: DeviceArray, src: DeviceArray, stream=None):
+    """dst[:src.n] = src with the streaming copy kernel."""
+    if dst.n < src.n:
+        raise MlemError(1, "copy_f32: destination too small")
+    _check(hip_lib().mlem_copy_f32(_dp(dst), _dp(src), src.n, _sh(stream)))
+
+
 def ratio(g, fwproj, rows, stream=None):
     _check(hip_lib().mlem_ratio(_dp(g), _dp(fwproj), rows, _sh(stream)))
 
@@ -398,6 +408,12 @@ class Recon:
         a, b, n = f64(), f64(), u32()
         _check(hip_lib().mlem_recon_kernel_times(self.h, C.byref(a), C.byref(b), C.byref(n)))
         return a.value, b.value, n.value
+
+    def phase_times(self):
+        """Summed ms of (FP SpMV kernel, FP epilogue, BP SpMV kernel, BP epilogue) and the iterations timed."""
+        ms, n = (f64 * 4)(), u32()
+        _check(hip_lib().mlem_recon_phase_times(self.h, ms, C.byref(n)))
+        return tuple(ms), n.value
 
     def reset_times(self):
         _check(hip_lib().mlem_recon_reset_times(self.h))

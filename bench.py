@@ -12,6 +12,16 @@ Prints one JSON line: iterations/s over the timed steps, the dominant
 kernel's achieved HBM bandwidth against the 8 TB/s peak (HIP events on the
 reconstruction's stream), and the FP32 OpenMP CPU loop (the oracle's CPU
 baseline) timed on a bounded number of iterations of the same workload.
+
+Algorithmic bytes (f32 values, u32 indices, u64 row pointers):
+  FP SpMV kernel  8*nnz + 8*(rows+1) + 4*cols      (A and x)
+  BP SpMV kernel  8*nnz + 8*(cols+1) + 4*rows      (A^T and the ratio)
+  iteration       16*nnz + 20*rows + 24*cols + 16  (both, with g, sens, f r/w)
+The blocked kernels read 6 bytes per stored entry (plus padding and the x
+segments), so their algorithmic rate can exceed the HBM rate they draw;
+`roofline.traffic` holds the measured HBM bytes per launch (rocprofv3
+FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy kernel)
+from profiles/traffic.json when it was measured for this workload.
 """
 import argparse
 import json
@@ -45,6 +55,24 @@ def cpu_model():
     return "unknown"
 
 
+def measured_traffic(workload_name, side):
+    """HBM bytes per launch of the blocked FP or BP kernel from profiles/traffic.json (rocprofv3 FETCH_SIZE and
+    WRITE_SIZE in KiB per dispatch, each scaled by bytes/KiB read from the copy kernel), or None."""
+    path = os.path.join(ROOT, "profiles", "traffic.json")
+    try:
+        with open(path) as f:
+            t = json.load(f)
+        if t.get("workload") != workload_name or side not in t:
+            return None
+        cp = t["copy"]
+        k_fetch = cp["bytes_read"] / (cp["fetch_kib"] * 1024.0)
+        k_write = cp["bytes_written"] / (cp["write_kib"] * 1024.0)
+        b = t[side]["fetch_kib"] * 1024.0 * k_fetch + t[side]["write_kib"] * 1024.0 * k_write
+        return {"bytes": int(b), "source": f"profiles/traffic.json ({t.get('measured', '')})"}
+    except (OSError, KeyError, ValueError, ZeroDivisionError):
+        return None
+
+
 def workload(name, threads):
     import mlem_amd as M
     if name == "c2":
@@ -70,9 +98,11 @@ def main():
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=["c2", "mid", "c1"])
     ap.add_argument("--fp", default="merge", choices=["merge", "vector"])
-    ap.add_argument("--bp", default="vector", choices=["merge", "vector"])
+    ap.add_argument("--bp", default="merge", choices=["merge", "vector"])
     ap.add_argument("--secsize", type=int, default=8)
     ap.add_argument("--tile", type=int, default=0, help="merge-path tiling: 0 auto, 1 off, else segment width")
+    ap.add_argument("--tile-rows", type=int, default=0, help="rows per tile block (0: default)")
+    ap.add_argument("--tile-groups", type=int, default=0, help="workgroups per blocked SpMV (0: default)")
     ap.add_argument("--graph", type=int, default=0)
     ap.add_argument("--threads", type=int, default=0, help="host threads (0: all)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
@@ -108,7 +138,8 @@ def main():
     # FP and BP kernels chosen independently: spmv selects FP, bp_* selects BP.
     fp_merge = args.fp == "merge"
     o = M.opts(spmv=0 if fp_merge else 1, using_trans=0, secsize_fp=args.secsize, secsize_bp=args.secsize,
-               profile=0 if args.graph else 1, use_graph=args.graph, tile_width=args.tile)
+               profile=0 if args.graph else 1, use_graph=args.graph, tile_width=args.tile,
+               tile_rows=args.tile_rows, tile_groups=args.tile_groups)
     o.bp_spmv = 1 if args.bp == "merge" else 2
     with M.Recon(m, g, f0, sens=s32, options=o) as r:
         r.iterate(args.warmup)
@@ -120,7 +151,7 @@ def main():
         r.sync()
         M.device_sync()
         t1 = time.perf_counter()
-        fp_ms, bp_ms, n_timed = r.kernel_times()
+        phase, n_timed = r.phase_times()
         img = r.image()
     elapsed = t1 - t0
     its = args.steps / elapsed
@@ -132,30 +163,50 @@ def main():
         f"sum f = {float(np.sum(img, dtype=np.float32)):.6g}")
 
     roofline = None
+    blocked = args.tile != 1
     if n_timed:
-        fp_avg, bp_avg = fp_ms / n_timed, bp_ms / n_timed
-        fp_gbs, bp_gbs = b_fp / (fp_avg * 1e-3) / 1e9, b_bp / (bp_avg * 1e-3) / 1e9
-        log(f"FP ({args.fp}+ratio) {fp_avg:.3f} ms {fp_gbs:.0f} GB/s; BP ({args.bp}+update) {bp_avg:.3f} ms "
-            f"{bp_gbs:.0f} GB/s")
-        dom_fp = fp_avg >= bp_avg
-        ach = fp_gbs if dom_fp else bp_gbs
+        ph = [v / n_timed for v in phase]  # FP kernel, FP epilogue, BP kernel, BP epilogue (ms)
+        k_fp = 8 * nnz + 8 * (rows + 1) + 4 * cols
+        k_bp = 8 * nnz + 8 * (cols + 1) + 4 * rows
+        gbs = lambda b, ms: b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
+        log(f"FP kernel {ph[0]:.3f} ms ({gbs(k_fp, ph[0]):.0f} GB/s) + epilogue {ph[1]:.3f} ms; "
+            f"BP kernel {ph[2]:.3f} ms ({gbs(k_bp, ph[2]):.0f} GB/s) + epilogue {ph[3]:.3f} ms")
+        dom_fp = ph[0] >= ph[2]
+        side = "fp" if dom_fp else "bp"
+        ms = ph[0] if dom_fp else ph[2]
+        kb = k_fp if dom_fp else k_bp
+        ach = gbs(kb, ms)
+        algo = args.fp if dom_fp else args.bp
+        if algo == "merge" and blocked:
+            kname = "blocked_spmv_kernel (LDS-staged x segments, 6-byte entries)"
+        elif algo == "merge":
+            kname = "merge_spmv_kernel"
+        else:
+            kname = "csr_vector_kernel"
         roofline = {"bound": "hbm", "achieved": round(ach, 1), "peak": HBM_PEAK_GBS, "unit": "GB/s",
                     "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": None,
-                    "kernel": (f"FP {args.fp}-path SpMV + ratio" if dom_fp else f"BP {args.bp} SpMV on A^T + update"),
-                    "algorithmic_bytes_per_launch": b_fp if dom_fp else b_bp,
-                    "avg_launch_ms": round(fp_avg if dom_fp else bp_avg, 4),
-                    "fp": {"ms": round(fp_avg, 4), "gbs": round(fp_gbs, 1), "bytes": b_fp},
-                    "bp": {"ms": round(bp_avg, 4), "gbs": round(bp_gbs, 1), "bytes": b_bp}}
+                    "kernel": f"{'FP' if dom_fp else 'BP'} {kname}",
+                    "algorithmic_bytes_per_launch": kb, "avg_launch_ms": round(ms, 4),
+                    "phases_ms": {"fp_kernel": round(ph[0], 4), "fp_epilogue": round(ph[1], 4),
+                                  "bp_kernel": round(ph[2], 4), "bp_epilogue": round(ph[3], 4)}}
+        tr = measured_traffic(args.workload, side) if (algo == "merge" and blocked) else None
+        if tr:
+            roofline["traffic"] = tr["bytes"]
+            roofline["traffic_source"] = tr["source"]
+            log(f"measured HBM traffic per {side.upper()} launch: {tr['bytes'] / 1e9:.2f} GB "
+                f"({tr['bytes'] / (ms * 1e-3) / 1e9:.0f} GB/s drawn)")
 
     if args.calibrate_copy:
         n = 1 << 28
         a = M.DeviceArray(n, np.float32)
         b = M.DeviceArray(n, np.float32)
         M.fill_f32(a, 1.0)
-        for _ in range(3):
-            M.hip_lib().mlem_memcpy_d2d(b.ptr, a.ptr, a.nbytes, None)
         M.device_sync()
-        log("copy calibration: 3 x 1 GiB device-to-device copies")
+        t = time.perf_counter()
+        for _ in range(3):
+            M.copy_f32(b, a)
+        M.device_sync()
+        log(f"copy calibration: 3 x 1 GiB copy_kernel, {3 * 2 * a.nbytes / (time.perf_counter() - t) / 1e9:.0f} GB/s")
 
     cpu = None
     if not args.no_cpu:
@@ -178,9 +229,10 @@ def main():
         "hbm_gbs": round(b_iter * its / 1e9, 1),
         "config": {"workload": label, "rows": rows, "cols": cols, "nnz": nnz,
                    "matrix_gb": round(8 * nnz / 1e9, 2), "bytes_per_iteration": b_iter,
-                   "fp": f"{args.fp}" + (f" (S={args.secsize})" if args.fp == "merge" else ""),
-                   "bp": f"{args.bp} on A^T" + (f" (S={args.secsize})" if args.bp == "merge" else ""),
-                   "tile_width": args.tile, "graph": bool(args.graph),
+                   "fp": args.fp + (" blocked" if args.fp == "merge" and blocked else ""),
+                   "bp": args.bp + (" blocked" if args.bp == "merge" and blocked else "") + " on A^T",
+                   "tile_width": args.tile, "tile_rows": args.tile_rows, "tile_groups": args.tile_groups,
+                   "graph": bool(args.graph),
                    "parallelism": "single GPU, matrix and transpose resident"},
         "roofline": roofline, "cpu_baseline": cpu,
     }

@@ -72,6 +72,10 @@ int mlem_event_record(void *event, void *stream);
 int mlem_event_elapsed_ms(void *start, void *stop, float *ms);
 /* Fill n floats with a value (a kernel, not hipMemset). */
 int mlem_fill_f32(float *dst, float value, uint64_t n, void *stream);
+/* dst = src over n floats with a streaming copy kernel (16-byte aligned
+ * pointers): the HBM reference for bandwidth and traffic counters.  No
+ * counterpart in the reference. */
+int mlem_copy_f32(float *dst, const float *src, uint64_t n, void *stream);
 
 /* ------------------------------------------------------------------------ */
 /* 1. kernel-level entry points                                             */
@@ -185,14 +189,14 @@ typedef struct mlem_opts {
                              1024 columns) with x segments staged in LDS
                              when the matrix allows it (rows sorted by
                              column), else the plain merge path;
-                             1 = plain merge path; 1024, 2048, 4096, 8192 =
-                             blocked with that segment width (an error if
-                             the matrix does not allow it)                   */
-    uint32_t tile_rows;   /* rows per block of the blocked kernels: 0 = 4096,
+                             1 = plain merge path; 1024 or 2048 = blocked
+                             with that segment width (an error if the
+                             matrix does not allow it)                       */
+    uint32_t tile_rows;   /* rows per block of the blocked kernels: 0 = 2048,
                              or 512, 1024, 2048, 4096, 8192, 16384           */
     uint32_t tile_groups; /* workgroups per blocked SpMV (runs of equal
                              length over each row block's tiles):
-                             0 = 32 per CU                                   */
+                             0 = 20 per CU                                   */
 } mlem_opts;
 
 /* Fill opts with the defaults: merge-based FP and BP (argv[8] = 0),
@@ -244,6 +248,12 @@ int mlem_recon_set_image(mlem_recon *r, const float *f);
  * BP+update (+ fix-up, or scatter+update). */
 int mlem_recon_kernel_times(mlem_recon *r, double *fp_ms, double *bp_ms,
                             uint32_t *iters_timed);
+/* The same split in four: ms[0] the FP SpMV kernel (the blocked kernel's
+ * partial sums; merge-based or CSR-vector FP with its fused ratio), ms[1]
+ * what follows it in FP (the blocked path's reduction + ratio, else ~0),
+ * ms[2] the BP SpMV (or scatter) kernel, ms[3] the rest of BP (reduction +
+ * update, or the scatter path's update). */
+int mlem_recon_phase_times(mlem_recon *r, double ms[4], uint32_t *iters_timed);
 int mlem_recon_reset_times(mlem_recon *r);
 /* Poisson log-likelihood sum_i g_i log(Ax)_i - (Ax)_i of the current image,
  * computed on the GPU in FP64 (rows with (Ax)_i <= 0 contribute -(Ax)_i). */

@@ -102,13 +102,27 @@ def test_recon_loglik_fp64(c1, resident, ref50):
     assert all(b >= a - 1e-9 * abs(a) for a, b in zip(lls, lls[1:]))  # EM ascent
 
 
-def test_profile_times(c1, resident, ref50):
-    o = M.opts(spmv=0, using_trans=0, profile=1)
+@pytest.mark.parametrize("tile", [0, 1])
+def test_profile_times(c1, resident, ref50, tile):
+    o = M.opts(spmv=0, using_trans=0, profile=1, tile_width=tile)
     with M.Recon(resident, c1["g"], ref50["f0"], sens=ref50["s32"], options=o) as r:
         r.iterate(10)
         r.sync()
         fp, bp, n = r.kernel_times()
-    assert n == 10 and fp > 0 and bp > 0
+        ph, n2 = r.phase_times()
+    assert n == 10 and n2 == 10 and fp > 0 and bp > 0
+    assert all(v >= 0 for v in ph) and ph[0] > 0 and ph[2] > 0
+    assert abs(ph[0] + ph[1] - fp) <= 1e-9 * fp and abs(ph[2] + ph[3] - bp) <= 1e-9 * bp
+
+
+def test_copy_kernel():
+    n = (1 << 20) + 3
+    a = M.DeviceArray(n, np.float32)
+    b = M.DeviceArray(n, np.float32)
+    x = np.random.default_rng(0).random(n).astype(np.float32)
+    a.upload(x)
+    M.copy_f32(b, a)
+    assert np.array_equal(b.numpy(), x)
 
 
 def test_edge_matrices_full_mlem():
