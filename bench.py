@@ -55,14 +55,17 @@ def cpu_model():
     return "unknown"
 
 
-def measured_traffic(workload_name, side):
+def measured_traffic(workload_name, side, tile_width, tile_rows):
     """HBM bytes per launch of the blocked FP or BP kernel from profiles/traffic.json (rocprofv3 FETCH_SIZE and
-    WRITE_SIZE in KiB per dispatch, each scaled by bytes/KiB read from the copy kernel), or None."""
+    WRITE_SIZE in KiB per dispatch, each scaled by bytes/KiB read from the copy kernel), or None when it was not
+    measured for this workload and tiling."""
     path = os.path.join(ROOT, "profiles", "traffic.json")
     try:
         with open(path) as f:
             t = json.load(f)
         if t.get("workload") != workload_name or side not in t:
+            return None
+        if t.get("tile_width") != (tile_width or 1024) or t.get("tile_rows") != (tile_rows or 2048):
             return None
         cp = t["copy"]
         k_fetch = cp["bytes_read"] / (cp["fetch_kib"] * 1024.0)
@@ -189,7 +192,8 @@ def main():
                     "algorithmic_bytes_per_launch": kb, "avg_launch_ms": round(ms, 4),
                     "phases_ms": {"fp_kernel": round(ph[0], 4), "fp_epilogue": round(ph[1], 4),
                                   "bp_kernel": round(ph[2], 4), "bp_epilogue": round(ph[3], 4)}}
-        tr = measured_traffic(args.workload, side) if (algo == "merge" and blocked) else None
+        tr = (measured_traffic(args.workload, side, args.tile, args.tile_rows)
+              if (algo == "merge" and blocked and not args.tile_groups) else None)
         if tr:
             roofline["traffic"] = tr["bytes"]
             roofline["traffic_source"] = tr["source"]
