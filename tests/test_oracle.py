@@ -340,3 +340,12 @@ def test_planted_mistakes_are_caught():
     for slip in ({"transpose_slip": True}, {"drop_ratio": True}, {"sens_slip": True}):
         bad = step(x, **slip)
         assert abs(float(np.dot(sens, bad)) - rhs) > 1e-6 * rhs, slip
+
+
+def test_loglik_hand_worked():
+    # A = [[1, 2], [0, 0], [0, 3]], f = [1, 2], g = [4, 5, 6]:
+    # y = [5, 0, 6]: 4 log 5 - 5 + (-0) + 6 log 6 - 6.
+    A = M.Csr(3, 2, np.array([0, 2, 2, 3], np.uint64), np.array([0, 1, 1], np.uint32),
+              np.array([1, 2, 3], np.float32))
+    ll = O.loglik_f64(A.rowptr, A.col, A.val, 3, np.array([4, 5, 6], np.int32), np.array([1.0, 2.0]))
+    assert abs(ll - (4 * np.log(5) - 5 + 6 * np.log(6) - 6)) < 1e-12
