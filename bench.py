@@ -144,7 +144,9 @@ def main():
                profile=0 if args.graph else 1, use_graph=args.graph, tile_width=args.tile,
                tile_rows=args.tile_rows, tile_groups=args.tile_groups)
     o.bp_spmv = 1 if args.bp == "merge" else 2
+    t = time.time()
     with M.Recon(m, g, f0, sens=s32, options=o) as r:
+        log(f"reconstruction set-up (tiles of A and A^T, work lists) {time.time() - t:.2f} s")
         r.iterate(args.warmup)
         r.sync()
         r.reset_times()
