@@ -300,7 +300,7 @@ int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t target, const TileWorkDev *
 
 // Chooses the blocked (LDS-tiled) or the plain merge path for one side of
 // the iteration.  The blocked path (default tiles of 2048 rows x 1024
-// columns) runs about tile_groups workgroups (0: 20 per CU); secsize applies
+// columns) runs about tile_groups workgroups (0: 24 per CU); secsize applies
 // to the plain merge path only.
 int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t secsize, const Tiled **tiled,
                      const TileWorkDev **work, DevBuf *part, const Partition **partn, DevBuf *scratch) {
@@ -317,7 +317,7 @@ int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t sec
             (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
         }
         if (st == MLEM_OK) {
-            const uint32_t target = o.tile_groups ? o.tile_groups : 20u * (uint32_t)std::max(1, cus);
+            const uint32_t target = o.tile_groups ? o.tile_groups : 24u * (uint32_t)std::max(1, cus);
             st = get_tile_work(m, const_cast<Tiled *>(*tiled), target, work);
         }
         if (st == MLEM_OK && tiled_lds_bytes((*tiled)->plan, (*work)->view.span) + 4096 > (size_t)lds_max) {

@@ -19,7 +19,9 @@
 //   and the rest of the old one is padding (value 0, column W, whose LDS
 //   slot holds 0).  A tile is cut into units of 2048 entries (the last one
 //   shorter, a multiple of 8); unit u of n entries is stored contiguously as
-//     n f32 values, then n/8 group words of 16 bytes:
+//     n f32 values: entries 0-3 of every group of 8, then entries 4-7 (so
+//     each 16-byte load of a wave reads 1 KiB of consecutive bytes),
+//     then n/8 group words of 16 bytes:
 //       bits  0-15  row - b*R of the group's first entry
 //       bits 16-29  2-bit row steps of entries 1..7
 //       bits 32-..  8 columns - s*W, 11 bits (W = 1024) or 12 bits (W = 2048)
@@ -95,20 +97,28 @@ struct Unit {
     bool valid;
 };
 
+// Staged tile i: the same for every work-item, so kept in scalar registers
+// (unit bookkeeping and load addresses then run on the scalar unit).
+__device__ __forceinline__ uint4 uniform_tile(const uint4 *tl, uint32_t i) {
+    const uint4 t = tl[i];
+    return make_uint4(__builtin_amdgcn_readfirstlane(t.x), __builtin_amdgcn_readfirstlane(t.y),
+                      __builtin_amdgcn_readfirstlane(t.z), __builtin_amdgcn_readfirstlane(t.w));
+}
+
 __device__ __forceinline__ Unit make_unit(const uint4 *tl, uint32_t ti, uint32_t c, bool valid) {
     Unit u;
     u.ti = ti;
     u.c = c;
     u.valid = valid;
     const uint32_t tj = valid ? ti : 0u;  // ti < span whenever valid
-    u.n = valid ? min(kUE, tl[tj].y - c * kUE) : 0u;
+    u.n = valid ? min(kUE, uniform_tile(tl, tj).y - c * kUE) : 0u;
     return u;
 }
 
 // wl[w] = {row block, first tile (index into tiles), first unit in it, units}
 // tiles[i] = {segment, entries, entry offset lo, hi} of the non-empty tiles.
 template <int W>
-__global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t R, uint32_t span, uint32_t ntl,
+__global__ __launch_bounds__(kWG, W == 1024 ? 8 : 7) void blocked_spmv_kernel(uint32_t R, uint32_t span, uint32_t ntl,
                                                            const uint4 *__restrict__ wl,
                                                            const uint4 *__restrict__ tiles,
                                                            const uint8_t *__restrict__ tdata,
@@ -120,8 +130,8 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t R, uint32_t 
     __shared__ float s_cf[kWG];         // carry after the last row end
     __shared__ uint32_t s_hf[kWG];      // work-item saw a row end
     float *xs = smem;                                      // [W + 4]; xs[W] = 0 for padding entries
-    float *acc = xs + W + 4;                               // [R]
-    uint4 *tl = reinterpret_cast<uint4 *>(acc + R);        // [span]
+    float *acc = xs + W + 4;                               // [R + kWG]: the rows, then one dummy word per work-item
+    uint4 *tl = reinterpret_cast<uint4 *>(acc + R + kWG);  // [span]
 
     const uint32_t tid = threadIdx.x;
     const uint4 w = wl[blockIdx.x];
@@ -144,14 +154,14 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t R, uint32_t 
     // and lanes past the unit's end re-read lane 0's entries (every address
     // stays inside the unit, so the wait counts are exact).
     auto prefetch = [&](const Unit &u, uint32_t fallback_ti) {
-        const uint4 t = tl[u.valid ? u.ti : fallback_ti];
+        const uint4 t = uniform_tile(tl, u.valid ? u.ti : fallback_ti);
         const uint64_t off = ((uint64_t)t.w << 32) | t.z;
         const uint32_t c = u.valid ? u.c : 0u;
         const uint32_t n8 = (min(kUE, t.y - c * kUE) + 7u) & ~7u;
         const uint8_t *base = tdata + 6 * (off + (uint64_t)c * kUE);
         const uint32_t le = (tid * kE < (u.valid ? u.n : 1u)) ? tid * kE : 0u;
-        pv[0] = ldnt(reinterpret_cast<const f32x4 *>(base) + le / 4);
-        pv[1] = ldnt(reinterpret_cast<const f32x4 *>(base) + le / 4 + 1);
+        pv[0] = ldnt(reinterpret_cast<const f32x4 *>(base) + le / 8);
+        pv[1] = ldnt(reinterpret_cast<const f32x4 *>(base + 2 * n8) + le / 8);
         pg = ldnt(reinterpret_cast<const u32x4 *>(base + 4 * n8) + le / 8);
         // x is padded to whole segments (xlen >= nseg * W): no clamping.
         const float *xb = x + (uint64_t)t.x * W;
@@ -168,73 +178,82 @@ __global__ __launch_bounds__(kWG) void blocked_spmv_kernel(uint32_t R, uint32_t 
 #pragma unroll
         for (int k = 0; k < XR / 4; ++k) cx[k] = px[k];
         --left;
-        const bool more_in_tile = (cur.c + 1) * kUE < tl[cur.ti].y;
+        const bool more_in_tile = (cur.c + 1) * kUE < uniform_tile(tl, cur.ti).y;
         const Unit nxt = make_unit(tl, more_in_tile ? cur.ti : cur.ti + 1, more_in_tile ? cur.c + 1 : 0u, left > 0);
         prefetch(nxt, cur.ti);  // in flight while this unit is computed
 
-        // Keys of this work-item's entries; entries past n get kNoKey.  (The
-        // clamps to R-1 and W only keep LDS addresses inside the block's
-        // arrays; a well-formed group word never needs them.)
-        const uint32_t e0 = tid * kE;
-        uint32_t key[kE + 1];
-        {
-            uint32_t k = cg.x & 0xffffu;
-            key[0] = min(k, R - 1);
-#pragma unroll
-            for (int j = 1; j < 8; ++j) {
-                k += (cg.x >> (14 + 2 * j)) & 3u;
-                key[j] = min(k, R - 1);
-            }
-        }
-#pragma unroll
-        for (int m = 0; m < (int)kE; ++m) key[m] = (e0 + m < cur.n) ? key[m] : kNoKey;
-        s_k0[tid] = key[0];
+        // Keys of this work-item's entries (first key + 2-bit steps; the
+        // builder checks that every key stays below R and every column at
+        // most W).
+        const bool lane_valid = tid * kE < cur.n;  // entries [n, n8) are padding
+        // A work-item past the unit's end reads lane 0's entries: give it the
+        // key R + tid (its own dummy accumulator word) and no row steps, so
+        // whatever it adds lands there.
+        const uint32_t word = lane_valid ? cg.x : 0u;
+        const uint32_t key0 = lane_valid ? (cg.x & 0xffffu) : R + tid;
+        s_k0[tid] = lane_valid ? key0 : kNoKey;
         if (cur.c == 0 || fresh) {  // a new tile: its x segment
 #pragma unroll
             for (int k = 0; k < XR / 4; ++k) *(reinterpret_cast<f32x4 *>(xs) + tid + kWG * k) = cx[k];
         }
         fresh = false;
         __syncthreads();
-        key[kE] = s_k0[tid + 1];
-
-        // Products first (all LDS gathers in flight together; masked
-        // entries read a valid column and are zeroed).
-        const float v[8] = {cv0.x, cv0.y, cv0.z, cv0.w, cv1.x, cv1.y, cv1.z, cv1.w};
-        float prod[kE];
+        uint32_t step[kE];  // step[j]: key of entry j minus key of entry j-1
 #pragma unroll
-        for (int m = 0; m < (int)kE; ++m) {
-            const float p = v[m] * xs[min(group_col<CB>(cg, m), (uint32_t)W)];
-            prod[m] = key[m] != kNoKey ? p : 0.0f;
+        for (int j = 1; j < 8; ++j) step[j] = (word >> (14 + 2 * j)) & 3u;
+        uint32_t key7 = key0;
+#pragma unroll
+        for (int j = 1; j < 8; ++j) key7 += step[j];
+        // Row ends: entry j < 7 ends its row where step[j+1] != 0, entry 7
+        // where the next work-item starts another row.  Every row that ends
+        // here gets its partial added to the accumulator (this work-item is
+        // the only writer of those rows in this unit); the other entries
+        // point at the dummy word acc[R + tid].  All reads come before any
+        // write.
+        const bool end7 = key7 != s_k0[tid + 1];
+        uint32_t slot[kE];
+        {
+            uint32_t k = key0;
+#pragma unroll
+            for (int m = 0; m < (int)kE; ++m) {
+                const bool end = m < 7 ? step[m + 1] != 0u : end7;
+                slot[m] = end ? k : R + tid;
+                if (m < 7) k += step[m + 1];
+            }
         }
-        // Walk my entries: a row ends where the key changes.  A row that
-        // starts and ends here is complete: this work-item is its only
-        // writer in this unit.
-        float a = 0.0f, head = 0.0f;
-        uint32_t has_end = 0, first_key = 0;
+        float old[kE];
+#pragma unroll
+        for (int m = 0; m < (int)kE; ++m) old[m] = acc[slot[m]];
+        // Products (padding reads column W, whose slot holds 0), then the
+        // running row sums, restarted after each row end.
+        const float v[8] = {cv0.x, cv0.y, cv0.z, cv0.w, cv1.x, cv1.y, cv1.z, cv1.w};
+        float run[kE];
+        float a = 0.0f;
 #pragma unroll
         for (int m = 0; m < (int)kE; ++m) {
-            a += prod[m];
-            const bool end = (key[m] != key[m + 1]) && (key[m] != kNoKey);
-            if (end && has_end) acc[key[m]] += a;
-            const bool first = end && !has_end;
-            head = first ? a : head;
-            first_key = first ? key[m] : first_key;
-            has_end |= end ? 1u : 0u;
+            a += v[m] * xs[group_col<CB>(cg, m)];
+            run[m] = a;
+            const bool end = m < 7 ? step[m + 1] != 0u : end7;
             a = end ? 0.0f : a;
         }
-        s_cf[tid] = a;
-        s_hf[tid] = has_end;
+#pragma unroll
+        for (int m = 0; m < (int)kE; ++m) acc[slot[m]] = old[m] + run[m];
+        uint32_t ends = end7 ? 1u : 0u;
+#pragma unroll
+        for (int j = 1; j < 8; ++j) ends |= step[j];
+        s_cf[tid] = a;  // the part of the last row that continues in the next work-item
+        s_hf[tid] = ends;
         __syncthreads();
-        // The first row ending here may have started in earlier work-items:
-        // add their carries, nearest first, back to the last one with a row
-        // end (whose carry belongs to this row too).
-        if (has_end) {
+        // The row of entry 0 may have started in earlier work-items: when it
+        // ends here, add their carries (nearest first, back to the last one
+        // with a row end, whose carry belongs to this row too).
+        if (ends) {
             float cin = 0.0f;
             for (int jj = (int)tid - 1; jj >= 0; --jj) {
                 cin += s_cf[jj];
                 if (s_hf[jj]) break;
             }
-            acc[first_key] += cin + head;
+            acc[key0] += cin;
         }
         cur = nxt;
     }
@@ -354,7 +373,10 @@ __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__res
         const uint64_t off = tile_off[t];
         const uint64_t p = rowpos[rsn] + (k - rs - segoff[rsn]);  // position in the tile
         const uint64_t u = p / kUE, q = p % kUE;
-        reinterpret_cast<float *>(tdata + 6 * (off + u * kUE))[q] = val[k];
+        const uint64_t n8u = std::min<uint64_t>(kUE, tile_off[t + 1] - off - u * kUE);
+        // entry q of the unit = entry j of work-item q/8: half j/4 of the values
+        const uint64_t vq = (q % 8 / 4) * (n8u / 2) + q / 8 * 4 + q % 4;
+        reinterpret_cast<float *>(tdata + 6 * (off + u * kUE))[vq] = val[k];
         ctmp[off + p] = (uint16_t)(c & ((1u << wshift) - 1u));
         rtmp[off + p] = (uint16_t)i;
     }
@@ -363,12 +385,14 @@ __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__res
 // One work-item per group of 8 entries (entry offset 8*q): finds its tile,
 // packs the group word (layout at the top of this file) into its unit.
 // Padding (rtmp = kPadRow) repeats the previous key with column W.  bit 4
-// of *bad: a step above kMaxRowStep (the layout guarantees none).
+// of *bad: a step above kMaxRowStep or a key not below R (the layout
+// guarantees neither; the SpMV kernel relies on both).
 template <int W>
 __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restrict__ ctmp,
                                                         const uint16_t *__restrict__ rtmp,
                                                         const uint64_t *__restrict__ tile_off, uint64_t ntiles,
-                                                        uint8_t *__restrict__ tdata, uint32_t *__restrict__ bad) {
+                                                        uint32_t R, uint8_t *__restrict__ tdata,
+                                                        uint32_t *__restrict__ bad) {
     constexpr int CB = col_bits<W>();
     const uint64_t q = blockIdx.x * (uint64_t)kWG + threadIdx.x;
     const uint64_t L = 8 * q;
@@ -390,7 +414,7 @@ __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restri
     const u16x8 r = *reinterpret_cast<const u16x8 *>(rtmp + L);
     const u16x8 c = *reinterpret_cast<const u16x8 *>(ctmp + L);
     uint32_t prev = r[0] == kPadRow ? 0u : r[0];
-    uint32_t w0 = prev, big = 0;
+    uint32_t w0 = prev, big = prev >= R ? 1u : 0u;
     uint32_t cols[8];
     cols[0] = r[0] == kPadRow ? (uint32_t)W : c[0];
 #pragma unroll
@@ -398,7 +422,7 @@ __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restri
         const uint32_t v = r[j];
         const uint32_t d = v == kPadRow ? 0u : v - prev;
         prev = v == kPadRow ? prev : v;
-        big |= d > kMaxRowStep ? 1u : 0u;
+        big |= (d > kMaxRowStep || prev >= R) ? 1u : 0u;
         w0 |= (d & 3u) << (14 + 2 * j);
         cols[j] = v == kPadRow ? (uint32_t)W : c[j];
     }
@@ -428,7 +452,9 @@ __global__ void fill_u16_kernel(uint16_t *__restrict__ p, uint16_t v, uint64_t n
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-size_t lds_bytes(uint32_t W, uint32_t R, uint32_t span) { return ((size_t)W + 4 + R) * 4 + (size_t)span * 16; }
+size_t lds_bytes(uint32_t W, uint32_t R, uint32_t span) {
+    return ((size_t)W + 4 + R + kWG) * 4 + (size_t)span * 16;
+}
 
 template <int W>
 hipError_t launch_w(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
@@ -639,10 +665,10 @@ hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uin
     if (p.ntiles) {
         const dim3 grid(blocks_for(p.cap / 8, kWG));
         if (p.W == 1024)
-            hipLaunchKernelGGL(tile_pack_kernel<1024>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles,
+            hipLaunchKernelGGL(tile_pack_kernel<1024>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles, p.R,
                                tm.tdata, c.bad);
         else
-            hipLaunchKernelGGL(tile_pack_kernel<2048>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles,
+            hipLaunchKernelGGL(tile_pack_kernel<2048>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles, p.R,
                                tm.tdata, c.bad);
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;

@@ -196,7 +196,7 @@ typedef struct mlem_opts {
                              or 512, 1024, 2048, 4096, 8192, 16384           */
     uint32_t tile_groups; /* workgroups per blocked SpMV (runs of equal
                              length over each row block's tiles):
-                             0 = 20 per CU                                   */
+                             0 = 24 per CU                                   */
 } mlem_opts;
 
 /* Fill opts with the defaults: merge-based FP and BP (argv[8] = 0),
