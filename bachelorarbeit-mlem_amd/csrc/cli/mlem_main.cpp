@@ -16,6 +16,7 @@
 // 4 GPU / engine error, 5 output file.
 #include <omp.h>
 
+#include <algorithm>
 #include <chrono>
 #include <cstdint>
 #include <cstdio>
@@ -111,24 +112,48 @@ int main(int argc, char **argv) {
     }
 
     Timer T;
-    // Matrix (Csr4Matrix + mapRows + csr_format_for_cuda, main.cu:71-79).
-    mlem_csr4 *file = nullptr;
-    if (mlem_csr4_open(matrixPath.c_str(), &file) != MLEM_OK) {
-        fprintf(stderr, "error: %s\n", mlem_host_last_error());
-        return 2;
-    }
+    // With the transpose and the sensitivity on the GPU no host CSR is needed:
+    // the csr4 data section streams straight to the device.
+    const bool stream_load = trans_mode == "gpu" && sens_mode == "gpu64";
     uint32_t rows = 0, cols = 0;
     uint64_t nnz = 0;
-    mlem_csr4_info(file, &rows, &cols, &nnz, nullptr, nullptr);
-    std::vector<uint64_t> rowptr((size_t)rows + 1);
-    std::vector<uint32_t> col(nnz);
-    std::vector<float> val(nnz);
-    if (mlem_csr4_to_csr(file, rowptr.data(), col.data(), val.data(), threads) != MLEM_OK) {
-        fprintf(stderr, "error: %s\n", mlem_host_last_error());
-        return 2;
+    std::vector<uint64_t> rowptr;
+    std::vector<uint32_t> col;
+    std::vector<float> val;
+    mlem_matrix *M = nullptr;
+    int st = MLEM_OK;
+    double t_load = 0, t_upload = 0, t_trans = 0;
+    if (stream_load) {
+        mlem_load_stats ls{};
+        if (mlem_matrix_load_csr4(0, matrixPath.c_str(), threads, &M, &ls) != MLEM_OK) {
+            fprintf(stderr, "error: %s\n", mlem_last_error());
+            return 2;
+        }
+        int has_t = 0;
+        uint64_t dev_bytes = 0;
+        mlem_matrix_info(M, &rows, &cols, &nnz, &has_t, &dev_bytes);
+        t_load = T.lap();
+        if (verbose)
+            fprintf(stderr, "streamed %.2f GB of csr4 data to the GPU in %.3f s (%.1f GB/s)\n", ls.bytes / 1e9,
+                    ls.stream_s, ls.bytes / 1e9 / std::max(ls.stream_s, 1e-9));
+    } else {
+        // Matrix (Csr4Matrix + mapRows + csr_format_for_cuda, main.cu:71-79).
+        mlem_csr4 *file = nullptr;
+        if (mlem_csr4_open(matrixPath.c_str(), &file) != MLEM_OK) {
+            fprintf(stderr, "error: %s\n", mlem_host_last_error());
+            return 2;
+        }
+        mlem_csr4_info(file, &rows, &cols, &nnz, nullptr, nullptr);
+        rowptr.resize((size_t)rows + 1);
+        col.resize(nnz);
+        val.resize(nnz);
+        if (mlem_csr4_to_csr(file, rowptr.data(), col.data(), val.data(), threads) != MLEM_OK) {
+            fprintf(stderr, "error: %s\n", mlem_host_last_error());
+            return 2;
+        }
+        mlem_csr4_close(file);
+        t_load = T.lap();
     }
-    mlem_csr4_close(file);
-    double t_load = T.lap();
 
     // Sinogram (Vector<int>, main.cu:87-90).
     void *graw = nullptr;
@@ -146,34 +171,31 @@ int main(int argc, char **argv) {
     }
     const int32_t *g = static_cast<const int32_t *>(graw);
 
-    // Resident matrix on GPU 0; the transpose on the host (sptrans) or GPU.
-    std::vector<uint64_t> rowptrT;
-    std::vector<uint32_t> colT;
-    std::vector<float> valT;
-    double t_trans = 0;
-    if (using_trans == 0 && trans_mode == "host") {
-        rowptrT.resize((size_t)cols + 1);
-        colT.resize(nnz);
-        valT.resize(nnz);
-        if (mlem_transpose(rows, cols, nnz, rowptr.data(), col.data(), val.data(), rowptrT.data(), colT.data(),
-                           valT.data(), threads) != MLEM_OK) {
-            fprintf(stderr, "error: %s\n", mlem_host_last_error());
-            return 4;
+    if (!stream_load) {
+        // Resident matrix on GPU 0; the transpose on the host (sptrans) or GPU.
+        std::vector<uint64_t> rowptrT;
+        std::vector<uint32_t> colT;
+        std::vector<float> valT;
+        if (using_trans == 0 && trans_mode == "host") {
+            rowptrT.resize((size_t)cols + 1);
+            colT.resize(nnz);
+            valT.resize(nnz);
+            if (mlem_transpose(rows, cols, nnz, rowptr.data(), col.data(), val.data(), rowptrT.data(), colT.data(),
+                               valT.data(), threads) != MLEM_OK) {
+                fprintf(stderr, "error: %s\n", mlem_host_last_error());
+                return 4;
+            }
+            t_trans = T.lap();
         }
-        t_trans = T.lap();
-    }
-    mlem_matrix *M = nullptr;
-    int st = mlem_matrix_create(0, rows, cols, nnz, rowptr.data(), col.data(), val.data(),
+        st = mlem_matrix_create(0, rows, cols, nnz, rowptr.data(), col.data(), val.data(),
                                 rowptrT.empty() ? nullptr : rowptrT.data(), colT.empty() ? nullptr : colT.data(),
                                 valT.empty() ? nullptr : valT.data(), &M);
-    if (st != MLEM_OK) {
-        fprintf(stderr, "error: %s\n", mlem_last_error());
-        return 4;
+        if (st != MLEM_OK) {
+            fprintf(stderr, "error: %s\n", mlem_last_error());
+            return 4;
+        }
+        t_upload = T.lap();
     }
-    std::vector<uint64_t>().swap(rowptrT);
-    std::vector<uint32_t>().swap(colT);
-    std::vector<float>().swap(valT);
-    double t_upload = T.lap();
     if (using_trans == 0 && trans_mode == "gpu") {
         if (mlem_matrix_build_transpose(M) != MLEM_OK) {
             fprintf(stderr, "error: %s\n", mlem_last_error());
@@ -200,7 +222,7 @@ int main(int argc, char **argv) {
     float init = 1.0f;
     if (x0_mode == "ref") mlem_initial_image(g, rows, norm.data(), cols, f0.data(), &init);
 
-    if (verbose) {
+    if (verbose && !rowptr.empty()) {
         std::vector<uint32_t> seg(device_numbers + 1), srows(device_numbers), snnz(device_numbers);
         std::vector<uint64_t> off(device_numbers);
         mlem_partition(rowptr.data(), nnz, rows, device_numbers, seg.data(), srows.data(), snnz.data(), nullptr,

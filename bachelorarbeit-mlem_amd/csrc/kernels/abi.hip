@@ -12,6 +12,7 @@
 
 #include <algorithm>
 #include <array>
+#include <chrono>
 #include <cstdarg>
 #include <cstdio>
 #include <cstring>
@@ -25,6 +26,7 @@
 
 #include "launch.hpp"
 #include "mlem_hip.h"
+#include "mlem_host.h"
 
 using namespace mlem;
 
@@ -797,6 +799,107 @@ int mlem_matrix_create(int device, uint32_t rows, uint32_t cols, uint64_t nnz, c
             HIPCHK(hipMemcpy(m->valT.p, valT, m->valT.bytes, hipMemcpyHostToDevice));
         }
         m->has_T = true;
+    }
+    *out = m.release();
+    return MLEM_OK;
+}
+
+int mlem_matrix_load_csr4(int device, const char *path, int threads, mlem_matrix **out, mlem_load_stats *stats) {
+    ARGCHK(path && out, "mlem_matrix_load_csr4: null argument");
+    *out = nullptr;
+    using clk = std::chrono::steady_clock;
+    const auto t0 = clk::now();
+    auto secs = [](clk::time_point a, clk::time_point b) { return std::chrono::duration<double>(b - a).count(); };
+    mlem_csr4 *f = nullptr;
+    if (int st = mlem_csr4_open(path, &f)) return set_err(st, "%s", mlem_host_last_error());
+    struct Closer {
+        mlem_csr4 *f;
+        ~Closer() { mlem_csr4_close(f); }
+    } closer{f};
+    uint32_t rows = 0, cols = 0;
+    uint64_t nnz = 0;
+    mlem_csr4_info(f, &rows, &cols, &nnz, nullptr, nullptr);
+    ARGCHK(cols > 0, "mlem_matrix_load_csr4: %s has no columns", path);
+    const uint64_t *ends = mlem_csr4_row_ends(f);
+    const uint8_t *data = static_cast<const uint8_t *>(mlem_csr4_data(f));
+    const auto t1 = clk::now();
+
+    HIPCHK(hipSetDevice(device));
+    auto m = std::make_unique<mlem_matrix>();
+    m->device = device;
+    m->rows = rows;
+    m->cols = cols;
+    m->nnz = nnz;
+    HIPCHK(m->rowptr.alloc(sizeof(uint64_t) * ((size_t)rows + 1)));
+    HIPCHK(m->col.alloc(sizeof(uint32_t) * nnz));
+    HIPCHK(m->val.alloc(sizeof(float) * nnz));
+    {
+        std::vector<uint64_t> rp((size_t)rows + 1, 0);
+        if (rows) std::memcpy(rp.data() + 1, ends, sizeof(uint64_t) * rows);
+        HIPCHK(hipMemcpy(m->rowptr.p, rp.data(), m->rowptr.bytes, hipMemcpyHostToDevice));
+    }
+    // Three pinned buffers of 64 MiB in rotation: while buffer i is copied
+    // to the device and split, the host threads fill buffer i+1.
+    constexpr int kBufs = 3;
+    constexpr size_t kChunk = size_t(64) << 20;  // a multiple of the 8-byte entry
+    const uint64_t total = 8 * nnz;
+    const int nthr = threads > 0 ? threads : (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
+    struct Staging {
+        void *host[kBufs] = {};
+        hipEvent_t ev[kBufs] = {};
+        hipStream_t s = nullptr;
+        ~Staging() {
+            if (s) (void)hipStreamSynchronize(s);
+            for (int i = 0; i < kBufs; ++i) {
+                if (host[i]) (void)hipHostFree(host[i]);
+                if (ev[i]) (void)hipEventDestroy(ev[i]);
+            }
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } st;
+    DevBuf dev[kBufs], bad;
+    HIPCHK(hipStreamCreate(&st.s));
+    HIPCHK(bad.alloc(sizeof(uint32_t)));
+    HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), st.s));
+    if (total) {
+        for (int i = 0; i < kBufs; ++i) {
+            HIPCHK(hipHostMalloc(&st.host[i], kChunk, hipHostMallocDefault));
+            HIPCHK(hipEventCreateWithFlags(&st.ev[i], hipEventDisableTiming));
+            HIPCHK(dev[i].alloc(kChunk));
+        }
+    }
+    bool used[kBufs] = {};
+    int i = 0;
+    for (uint64_t off = 0; off < total; off += kChunk, i = (i + 1) % kBufs) {
+        const size_t len = (size_t)std::min<uint64_t>(kChunk, total - off);
+        if (used[i]) HIPCHK(hipEventSynchronize(st.ev[i]));
+        {
+            // host threads copy slices of the mapped file into the pinned buffer
+            const size_t per = ((len / nthr) + 4095) & ~size_t(4095);
+            std::vector<std::thread> pool;
+            for (int t = 0; t < nthr; ++t) {
+                const size_t b = std::min(len, per * t), e = std::min(len, per * (t + 1));
+                if (b >= e) break;
+                pool.emplace_back([&, b, e] { std::memcpy(static_cast<uint8_t *>(st.host[i]) + b, data + off + b, e - b); });
+            }
+            for (auto &th : pool) th.join();
+        }
+        HIPCHK(hipMemcpyAsync(dev[i].p, st.host[i], len, hipMemcpyHostToDevice, st.s));
+        HIPCHK(launch_aos_split(dev[i].p, len / 8, off / 8, cols, m->col.as<uint32_t>(), m->val.as<float>(),
+                                bad.as<uint32_t>(), st.s));
+        HIPCHK(hipEventRecord(st.ev[i], st.s));
+        used[i] = true;
+    }
+    uint32_t flag = 0;
+    HIPCHK(hipMemcpyAsync(&flag, bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st.s));
+    HIPCHK(hipStreamSynchronize(st.s));
+    const auto t2 = clk::now();
+    if (flag) return set_err(MLEM_ERR_FORMAT, "mlem_matrix_load_csr4: %s has a column index >= %u", path, cols);
+    if (stats) {
+        stats->bytes = total;
+        stats->open_s = secs(t0, t1);
+        stats->stream_s = secs(t1, t2);
+        stats->total_s = secs(t0, t2);
     }
     *out = m.release();
     return MLEM_OK;

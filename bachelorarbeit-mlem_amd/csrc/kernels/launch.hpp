@@ -61,6 +61,10 @@ hipError_t launch_update(const float *f, const float *norm, float *bwproj, uint3
 hipError_t launch_update_image(float *f, const float *norm, float *bp, uint32_t cols, hipStream_t s);
 hipError_t launch_fill(float *dst, float v, uint64_t n, hipStream_t s);
 hipError_t launch_fill_f64(double *dst, double v, uint64_t n, hipStream_t s);
+// csr4 AoS entries [first, first+n) -> col/val; bit 1 of *bad when a
+// column is >= cols.
+hipError_t launch_aos_split(const void *aos, uint64_t n, uint64_t first, uint32_t cols, uint32_t *col, float *val,
+                            uint32_t *bad, hipStream_t s);
 // dst = src (16-byte aligned), non-temporal streaming copy.
 hipError_t launch_copy(float *dst, const float *src, uint64_t n, hipStream_t s);
 hipError_t launch_colsum_rows_f64(const uint64_t *rowptrT, const float *valT, uint32_t cols, double *s64, float *s32,

@@ -352,6 +352,23 @@ __global__ __launch_bounds__(kWG) void copy_kernel(float *__restrict__ dst, cons
     for (uint64_t i = 4 * n4 + blockIdx.x * (uint64_t)kWG + threadIdx.x; i < n; i += stride) dst[i] = src[i];
 }
 
+// csr4 AoS {u32 col, f32 val} entries [first, first+n) of the matrix -> SoA;
+// bit 1 of *bad: a column index >= cols.
+__global__ __launch_bounds__(kWG) void aos_split_kernel(const uint32_t *__restrict__ aos, uint64_t n, uint64_t first,
+                                                        uint32_t cols, uint32_t *__restrict__ col,
+                                                        float *__restrict__ val, uint32_t *__restrict__ bad) {
+    const uint64_t stride = (uint64_t)gridDim.x * kWG;
+    uint32_t flag = 0;
+    for (uint64_t i = blockIdx.x * (uint64_t)kWG + threadIdx.x; i < n; i += stride) {
+        const uint32_t c = __builtin_nontemporal_load(aos + 2 * i);
+        const uint32_t v = __builtin_nontemporal_load(aos + 2 * i + 1);
+        flag |= c >= cols ? 1u : 0u;
+        col[first + i] = c;
+        val[first + i] = __uint_as_float(v);
+    }
+    if (flag) atomicOr(bad, 1u);
+}
+
 __global__ void f64_to_f32_kernel(const double *__restrict__ a, float *__restrict__ b, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
@@ -567,6 +584,15 @@ hipError_t launch_copy(float *dst, const float *src, uint64_t n, hipStream_t s) 
     uint64_t blocks = (n / 4 + kWG - 1) / kWG;
     blocks = std::max<uint64_t>(1, std::min<uint64_t>(blocks, 8192));
     hipLaunchKernelGGL(copy_kernel, dim3((uint32_t)blocks), dim3(kWG), 0, s, dst, src, n);
+    return hipGetLastError();
+}
+
+hipError_t launch_aos_split(const void *aos, uint64_t n, uint64_t first, uint32_t cols, uint32_t *col, float *val,
+                            uint32_t *bad, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((n + kWG - 1) / kWG, 16384));
+    hipLaunchKernelGGL(aos_split_kernel, dim3((uint32_t)blocks), dim3(kWG), 0, s, static_cast<const uint32_t *>(aos),
+                       n, first, cols, col, val, bad);
     return hipGetLastError();
 }
 

@@ -66,6 +66,7 @@ def hip_lib() -> C.CDLL:
             "mlem_opts_default": (None, [P(Opts)]),
             "mlem_matrix_create": (cint, [cint, u32, u32, u64, vp, vp, vp, vp, vp, vp, P(vp)]),
             "mlem_matrix_build_transpose": (cint, [vp]),
+            "mlem_matrix_load_csr4": (cint, [cint, C.c_char_p, cint, P(vp), vp]),
             "mlem_matrix_destroy": (cint, [vp]),
             "mlem_matrix_info": (cint, [vp, P(u32), P(u32), P(u64), P(cint), P(u64)]),
             "mlem_matrix_device_arrays": (cint, [vp, P(vp), P(vp), P(vp), P(vp), P(vp), P(vp)]),
@@ -319,6 +320,10 @@ def colsum_scatter_f64(rowptr, col, val, rows, cols, nnz, sens64=None, sens32=No
 
 
 # ---------------------------------------------------------------- handles
+class LoadStats(C.Structure):
+    _fields_ = [("bytes", u64), ("open_s", f64), ("stream_s", f64), ("total_s", f64)]
+
+
 class ResidentMatrix:
     """A system matrix (and optionally its transpose) resident in HBM."""
 
@@ -329,6 +334,31 @@ class ResidentMatrix:
             device, A.rows, A.cols, A.nnz, _ptr(A.rowptr), _ptr(A.col), _ptr(A.val),
             _ptr(AT.rowptr) if AT is not None else None, _ptr(AT.col) if AT is not None else None,
             _ptr(AT.val) if AT is not None else None, C.byref(self.h)))
+
+    @classmethod
+    def load_csr4(cls, path: str, device: int = 0, threads: int = 0):
+        """A straight from a csr4 file (streamed through pinned buffers, split on the GPU).
+        Returns (matrix, stats dict)."""
+        self = cls.__new__(cls)
+        self.h = vp()
+        st = LoadStats()
+        _check(hip_lib().mlem_matrix_load_csr4(device, path.encode(), threads, C.byref(self.h), C.byref(st)))
+        info = self.info()
+        self.rows, self.cols, self.nnz = info["rows"], info["cols"], info["nnz"]
+        return self, dict(bytes=st.bytes, open_s=st.open_s, stream_s=st.stream_s, total_s=st.total_s)
+
+    def download(self) -> Csr:
+        """A's resident arrays copied back to the host."""
+        rp, ci, va = self.device_arrays()[:3]
+        rowptr = np.empty(self.rows + 1, np.uint64)
+        col = np.empty(self.nnz, np.uint32)
+        val = np.empty(self.nnz, np.float32)
+        lib = hip_lib()
+        _check(lib.mlem_memcpy_d2h(_ptr(rowptr), rp, rowptr.nbytes, None))
+        if self.nnz:
+            _check(lib.mlem_memcpy_d2h(_ptr(col), ci, col.nbytes, None))
+            _check(lib.mlem_memcpy_d2h(_ptr(val), va, val.nbytes, None))
+        return Csr(self.rows, self.cols, rowptr, col, val)
 
     def build_transpose(self):
         _check(hip_lib().mlem_matrix_build_transpose(self.h))

@@ -110,6 +110,8 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="host threads (0: all)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--load-test", action="store_true",
+                    help="also write the matrix as a csr4 file and time the streamed loader on it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
@@ -214,6 +216,35 @@ def main():
         M.device_sync()
         log(f"copy calibration: 3 x 1 GiB copy_kernel, {3 * 2 * a.nbytes / (time.perf_counter() - t) / 1e9:.0f} GB/s")
 
+    load = None
+    if args.load_test:
+        import tempfile
+        d = tempfile.mkdtemp(prefix="mlem_load_", dir=os.environ.get("TMPDIR", "/tmp"))
+        path = os.path.join(d, "A.csr4")
+        t = time.time()
+        M.write_csr4(path, A)
+        log(f"wrote {path} ({os.path.getsize(path) / 1e9:.2f} GB) in {time.time() - t:.1f} s")
+        try:
+            for rep in range(2):
+                lm, st = M.ResidentMatrix.load_csr4(path, threads=threads)
+                lm.close()
+                log(f"streamed csr4 load: {st['bytes'] / 1e9:.2f} GB in {st['stream_s']:.3f} s "
+                    f"({st['bytes'] / 1e9 / st['stream_s']:.1f} GB/s), open {st['open_s'] * 1e3:.1f} ms")
+            t = time.time()
+            H = M.read_csr4(path, threads=threads)
+            t_read = time.time() - t
+            t = time.time()
+            hm = M.ResidentMatrix(H)
+            t_up = time.time() - t
+            hm.close()
+            log(f"host path: read_csr4 + CSR build {t_read:.2f} s, upload {t_up:.2f} s")
+            load = {"bytes": st["bytes"], "stream_s": round(st["stream_s"], 4),
+                    "gbs": round(st["bytes"] / 1e9 / st["stream_s"], 2), "threads": threads,
+                    "host_path_s": round(t_read + t_up, 3), "file": "warm page cache (just written)"}
+        finally:
+            os.remove(path)
+            os.rmdir(d)
+
     cpu = None
     if not args.no_cpu:
         import oracle as O
@@ -242,6 +273,8 @@ def main():
                    "parallelism": "single GPU, matrix and transpose resident"},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if load:
+        out["load_csr4"] = load
     print(json.dumps(out), flush=True)
 
 

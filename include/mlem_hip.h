@@ -215,6 +215,23 @@ int mlem_matrix_create(int device, uint32_t rows, uint32_t cols, uint64_t nnz,
 /* Build A^T on the device from the resident A (stable counting sort,
  * bit-exact with the host transpose) instead of uploading one. */
 int mlem_matrix_build_transpose(mlem_matrix *m);
+
+/* Load A straight from a csr4 file into a resident matrix (Csr4Matrix +
+ * mapRows + csr_format_for_cuda + the H2D copies of main.cu:71-79,263-272,
+ * without a host CSR): the mapped AoS data section is copied by `threads`
+ * host threads (<= 0: all cores) into pinned staging buffers, streamed to
+ * the device and split there into col u32[nnz] / val f32[nnz]; copies,
+ * transfers and the split overlap.  The row pointers are the file's
+ * inclusive row ends with a leading 0.  A column index >= cols is an
+ * MLEM_ERR_FORMAT error.  stats may be NULL. */
+typedef struct mlem_load_stats {
+    uint64_t bytes;  /* data-section bytes streamed to the device */
+    double open_s;   /* open, validate and map the file */
+    double stream_s; /* copy + H2D + split of the data section */
+    double total_s;
+} mlem_load_stats;
+int mlem_matrix_load_csr4(int device, const char *path, int threads,
+                          mlem_matrix **out, mlem_load_stats *stats);
 int mlem_matrix_destroy(mlem_matrix *m);
 int mlem_matrix_info(const mlem_matrix *m, uint32_t *rows, uint32_t *cols,
                      uint64_t *nnz, int *has_transpose, uint64_t *device_bytes);

@@ -52,6 +52,21 @@ def test_cli_reconstruction_matches_oracle(c1, files, trans, spmv, extra):
     assert r.stdout.endswith(f"\nSum f: {s:f}\n")
 
 
+def test_cli_streamed_load_equals_host_path(c1, files):
+    # --transpose=gpu with the GPU sensitivity streams the csr4 file to the
+    # device (no host CSR); the image is bitwise the one of the host path.
+    outs = []
+    for mode in ("host", "gpu"):
+        out = str(files["dir"] / f"img_{mode}.f32")
+        r = run(files["matrix"], files["sino"], "15", "1", "8", "8", "0", "0", f"--out={out}", f"--transpose={mode}",
+                "--verbose")
+        assert r.returncode == 0, r.stderr
+        outs.append(M.read_raw(out, np.float32))
+        if mode == "gpu":
+            assert "streamed" in r.stderr
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+
+
 def test_cli_file_errors(c1, files):
     r = run("/nonexistent.csr4", files["sino"], "2", "1", "2", "2", "0", "0")
     assert r.returncode == 2
