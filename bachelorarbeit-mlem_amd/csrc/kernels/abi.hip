@@ -160,8 +160,9 @@ struct mlem_matrix {
     std::mutex mu;
     std::map<std::pair<int, uint32_t>, std::unique_ptr<Partition>> parts;  // (0: A / 1: A^T, secsize)
     // Tiles keyed by (0: A / 1: A^T, width W, rows R).
-    std::map<std::tuple<int, uint32_t, uint32_t>, std::unique_ptr<Tiled>> tiles;
-    std::map<std::tuple<int, uint32_t, uint32_t>, uint32_t> tile_refused;  // build flags of refused tilings
+    // Tiles keyed by (0: A / 1: A^T, width W, rows R, value type).
+    std::map<std::tuple<int, uint32_t, uint32_t, uint32_t>, std::unique_ptr<Tiled>> tiles;
+    std::map<std::tuple<int, uint32_t, uint32_t, uint32_t>, uint32_t> tile_refused;  // build flags of refused tilings
 
     uint64_t device_bytes() const {
         uint64_t b = rowptr.bytes + col.bytes + val.bytes + rowptrT.bytes + colT.bytes + valT.bytes;
@@ -174,6 +175,24 @@ struct mlem_matrix {
 // kernel done, BP done.
 struct IterEvents {
     hipEvent_t e[5];
+};
+
+struct mlem_batch {
+    mlem_matrix *m = nullptr;
+    uint32_t slots = 0, BC = 4, passes = 0;
+    size_t rows_x = 0, cols_x = 0;  // padded vector lengths
+    hipStream_t stream = nullptr;
+    // [passes][rows_x][BC] counts and ratios, [passes][cols_x][BC] images
+    DevBuf g, ratio, f, sens, active, stage, part_fp, part_bp;
+    const Tiled *tfp = nullptr, *tbp = nullptr;
+    const TileWorkDev *wfp = nullptr, *wbp = nullptr;
+    std::vector<uint32_t> active_host;
+    ~mlem_batch() {
+        if (stream) {
+            (void)hipStreamSynchronize(stream);
+            (void)hipStreamDestroy(stream);
+        }
+    }
 };
 
 struct mlem_recon {
@@ -220,9 +239,9 @@ int get_partition(mlem_matrix *m, int which, uint32_t secsize, const Partition *
 // Tiles of A (which 0) or A^T (which 1) with segment width W and R rows per
 // block, built on first use.  MLEM_ERR_UNSUPPORTED (and *flags) when the
 // matrix does not allow them (a row not sorted by column).
-int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, const Tiled **out, uint32_t *flags) {
+int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, const Tiled **out, uint32_t *flags) {
     std::lock_guard<std::mutex> lk(m->mu);
-    auto key = std::make_tuple(which, W, R);
+    auto key = std::make_tuple(which, W, R, vt);
     *flags = 0;
     auto it = m->tiles.find(key);
     if (it != m->tiles.end()) {
@@ -236,7 +255,7 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, const Tiled **o
     }
     auto t = std::make_unique<Tiled>();
     const uint32_t nrow = which == 0 ? m->rows : m->cols, ncol = which == 0 ? m->cols : m->rows;
-    t->plan = tile_plan(nrow, ncol, m->nnz, W, R);
+    t->plan = tile_plan(nrow, ncol, m->nnz, W, R, vt);
     const uint64_t *rp = which == 0 ? m->rowptr.as<uint64_t>() : m->rowptrT.as<uint64_t>();
     const uint32_t *ci = which == 0 ? m->col.as<uint32_t>() : m->colT.as<uint32_t>();
     const float *va = which == 0 ? m->val.as<float>() : m->valT.as<float>();
@@ -300,10 +319,40 @@ int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t target, const TileWorkDev *
     return MLEM_OK;
 }
 
+// The blocked path for one side (0: A, 1: A^T): tiles (built on first use),
+// a work list of about `groups` runs (0: 24 per CU) and the partial-sum
+// buffer for BC right-hand sides.  MLEM_ERR_UNSUPPORTED when the matrix or
+// the LDS does not allow the shape.
+int setup_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, uint32_t BC, uint32_t groups,
+                const Tiled **tiled, const TileWorkDev **work, DevBuf *part) {
+    *tiled = nullptr;
+    *work = nullptr;
+    if (!valid_tile_config(W, BC))
+        return set_err(MLEM_ERR_UNSUPPORTED, "no blocked kernel for width %u with %u right-hand sides", W, BC);
+    uint32_t flags = 0;
+    const Tiled *t = nullptr;
+    if (int st = get_tiled(m, which, W, R, vt, &t, &flags)) return st;
+    int dev = 0, lds_max = 65536, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) {
+        (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
+        (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    }
+    const uint32_t target = groups ? groups : 24u * (uint32_t)std::max(1, cus);
+    const TileWorkDev *w = nullptr;
+    if (int st = get_tile_work(m, const_cast<Tiled *>(t), target, &w)) return st;
+    if (tiled_lds_bytes(t->plan, w->view.span, BC) + 4096 > (size_t)lds_max)
+        return set_err(MLEM_ERR_UNSUPPORTED, "tiling (width %u, rows %u, %u right-hand sides) needs too much LDS", W,
+                       R, BC);
+    HIPCHK(part->alloc(std::max<size_t>(4, tiled_part_bytes(t->plan, w->view.nwg, BC))));
+    *tiled = t;
+    *work = w;
+    return MLEM_OK;
+}
+
 // Chooses the blocked (LDS-tiled) or the plain merge path for one side of
 // the iteration.  The blocked path (default tiles of 2048 rows x 1024
 // columns) runs about tile_groups workgroups (0: 24 per CU); secsize applies
-// to the plain merge path only.
+// to the plain merge path only.  16-bit values need the blocked path.
 int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t secsize, const Tiled **tiled,
                      const TileWorkDev **work, DevBuf *part, const Partition **partn, DevBuf *scratch) {
     *tiled = nullptr;
@@ -311,30 +360,13 @@ int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t sec
     if (o.tile_width != 1) {
         const uint32_t W = o.tile_width == 0 ? 1024u : o.tile_width;
         const uint32_t R = o.tile_rows == 0 ? 2048u : o.tile_rows;
-        uint32_t flags = 0;
-        int st = get_tiled(m, which, W, R, tiled, &flags);
-        int dev = 0, lds_max = 65536, cus = 256;
-        if (hipGetDevice(&dev) == hipSuccess) {
-            (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);
-            (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-        }
-        if (st == MLEM_OK) {
-            const uint32_t target = o.tile_groups ? o.tile_groups : 24u * (uint32_t)std::max(1, cus);
-            st = get_tile_work(m, const_cast<Tiled *>(*tiled), target, work);
-        }
-        if (st == MLEM_OK && tiled_lds_bytes((*tiled)->plan, (*work)->view.span) + 4096 > (size_t)lds_max) {
-            *tiled = nullptr;
-            *work = nullptr;
-            st = set_err(MLEM_ERR_UNSUPPORTED, "tiling (width %u, rows %u) needs too much LDS", W, R);
-        }
-        if (st == MLEM_OK) {
-            HIPCHK(part->alloc(std::max<size_t>(4, tiled_part_bytes((*tiled)->plan, (*work)->view.nwg))));
-            return MLEM_OK;
-        }
-        if (st != MLEM_ERR_UNSUPPORTED || o.tile_width != 0) return st;  // explicit width: no fallback
-        *tiled = nullptr;
-        *work = nullptr;
+        const int st = setup_tiled(m, which, W, R, o.values, 1, o.tile_groups, tiled, work, part);
+        if (st == MLEM_OK) return MLEM_OK;
+        // explicit width or 16-bit values: no fallback
+        if (st != MLEM_ERR_UNSUPPORTED || o.tile_width != 0 || o.values != VAL_F32) return st;
     }
+    if (o.values != VAL_F32)
+        return set_err(MLEM_ERR_UNSUPPORTED, "16-bit matrix values need the blocked merge-based kernels");
     if (int st = get_partition(m, which, secsize, partn)) return st;
     HIPCHK(scratch->alloc(merge_scratch_bytes((*partn)->plan)));
     return MLEM_OK;
@@ -353,10 +385,10 @@ hipError_t enqueue_iteration(mlem_recon *r, IterEvents *ev) {
     fa.g = r->g.as<int32_t>();
     if (r->tfp) {
         if ((e = launch_tiled_partials(r->tfp->plan, r->tfp->view, r->wfp->view, r->f.as<float>(),
-                                       r->part_fp.as<float>(), s)) != hipSuccess)
+                                       r->part_fp.as<float>(), 1, s)) != hipSuccess)
             return e;
         if ((e = mark(1)) != hipSuccess) return e;
-        e = launch_tiled_reduce(r->tfp->plan, r->wfp->view, r->part_fp.as<float>(), EPI_RATIO, fa, s);
+        e = launch_tiled_reduce(r->tfp->plan, r->tfp->view, r->wfp->view, r->part_fp.as<float>(), 1, EPI_RATIO, fa, s);
     } else {
         if (r->opts.spmv == 0)
             e = launch_merge_spmv(r->pfp->plan, r->pfp->buf.p, r->scratch_fp.p, m->rowptr.as<uint64_t>(),
@@ -375,10 +407,11 @@ hipError_t enqueue_iteration(mlem_recon *r, IterEvents *ev) {
     if (r->opts.using_trans == 0) {
         if (r->tbp) {
             if ((e = launch_tiled_partials(r->tbp->plan, r->tbp->view, r->wbp->view, r->ratio.as<float>(),
-                                           r->part_bp.as<float>(), s)) != hipSuccess)
+                                           r->part_bp.as<float>(), 1, s)) != hipSuccess)
                 return e;
             if ((e = mark(3)) != hipSuccess) return e;
-            e = launch_tiled_reduce(r->tbp->plan, r->wbp->view, r->part_bp.as<float>(), EPI_UPDATE, ba, s);
+            e = launch_tiled_reduce(r->tbp->plan, r->tbp->view, r->wbp->view, r->part_bp.as<float>(), 1, EPI_UPDATE,
+                                    ba, s);
         } else {
             if (r->bp_merge)
                 e = launch_merge_spmv(r->pbp->plan, r->pbp->buf.p, r->scratch_bp.p, m->rowptrT.as<uint64_t>(),
@@ -1044,8 +1077,11 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
         HIPCHK(r->bp.alloc(sizeof(float) * m->cols));
         HIPCHK(launch_fill(r->bp.as<float>(), 0.0f, m->cols, r->stream));
     }
-    ARGCHK(o.tile_width <= 1 || valid_tile_width(o.tile_width),
+    ARGCHK(o.tile_width <= 1 || o.tile_width == 1024 || o.tile_width == 2048,
            "mlem_recon_create: tile_width must be 0 (auto), 1 (off), 1024 or 2048");
+    ARGCHK(o.values <= VAL_F16, "mlem_recon_create: values must be 0 (f32), 1 (bf16) or 2 (fp16)");
+    ARGCHK(o.values == VAL_F32 || (o.spmv == 0 && o.using_trans == 0 && r->bp_merge),
+           "mlem_recon_create: 16-bit values need merge-based FP and BP on A^T (the blocked kernels)");
     ARGCHK(o.tile_rows == 0 || valid_tile_rows(o.tile_rows),
            "mlem_recon_create: tile_rows must be 0 (auto), 512, 1024, 2048, 4096, 8192 or 16384");
     if (o.spmv == 0)
@@ -1174,6 +1210,163 @@ int mlem_recon_destroy(mlem_recon *r) {
         for (auto &e : x.e) (void)hipEventDestroy(e);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
+    return MLEM_OK;
+}
+
+// ---------------------------------------------------------------- batches
+void mlem_batch_opts_default(mlem_batch_opts *o) {
+    if (!o) return;
+    std::memset(o, 0, sizeof(*o));
+    o->per_pass = 4;
+}
+
+int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opts, mlem_batch **out) {
+    ARGCHK(out, "mlem_batch_create: null out");
+    *out = nullptr;
+    ARGCHK(m && slots > 0, "mlem_batch_create: null matrix or no slots");
+    mlem_batch_opts o;
+    if (opts)
+        o = *opts;
+    else
+        mlem_batch_opts_default(&o);
+    const uint32_t BC = o.per_pass ? o.per_pass : 4u;
+    ARGCHK(BC == 1 || BC == 4 || BC == 8, "mlem_batch_create: per_pass must be 1, 4 or 8");
+    ARGCHK(o.values <= VAL_F16, "mlem_batch_create: values must be 0 (f32), 1 (bf16) or 2 (fp16)");
+    ARGCHK(m->has_T, "mlem_batch_create: the transpose must be resident (upload it or build it)");
+    const uint32_t W = o.tile_width ? o.tile_width : (BC == 1 ? 1024u : 512u);
+    const uint32_t R = o.tile_rows ? o.tile_rows : (BC == 8 ? 1024u : 2048u);
+    ARGCHK(valid_tile_config(W, BC), "mlem_batch_create: no blocked kernel for width %u with %u per pass", W, BC);
+    ARGCHK(valid_tile_rows(R), "mlem_batch_create: tile_rows must be 512 ... 16384 (a power of two)");
+    HIPCHK(hipSetDevice(m->device));
+    auto b = std::make_unique<mlem_batch>();
+    b->m = m;
+    b->slots = slots;
+    b->BC = BC;
+    b->passes = (slots + BC - 1) / BC;
+    b->rows_x = (m->rows + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
+    b->cols_x = (m->cols + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
+    HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
+    int cus = 256, dev = 0;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    const uint32_t groups = o.tile_groups ? o.tile_groups : (BC == 1 ? 24u : 8u) * (uint32_t)std::max(1, cus);
+    if (int st = setup_tiled(m, 0, W, R, o.values, BC, groups, &b->tfp, &b->wfp, &b->part_fp)) return st;
+    if (int st = setup_tiled(m, 1, W, R, o.values, BC, groups, &b->tbp, &b->wbp, &b->part_bp)) return st;
+    const size_t per = (size_t)BC * b->passes;
+    HIPCHK(b->g.alloc(sizeof(int32_t) * b->rows_x * per));
+    HIPCHK(b->ratio.alloc(sizeof(float) * b->rows_x * per));
+    HIPCHK(b->f.alloc(sizeof(float) * b->cols_x * per));
+    HIPCHK(b->sens.alloc(sizeof(float) * m->cols));
+    HIPCHK(b->active.alloc(sizeof(uint32_t) * per));
+    HIPCHK(b->stage.alloc(sizeof(float) * std::max<size_t>(m->rows, m->cols)));
+    HIPCHK(launch_fill(b->g.as<float>(), 0.0f, b->rows_x * per, b->stream));  // 0.0f has the bits of int 0
+    HIPCHK(launch_fill(b->ratio.as<float>(), 0.0f, b->rows_x * per, b->stream));
+    HIPCHK(launch_fill(b->f.as<float>(), 0.0f, b->cols_x * per, b->stream));
+    HIPCHK(launch_fill(b->active.as<float>(), 0.0f, per, b->stream));
+    b->active_host.assign(per, 0);
+    if (int st = sens_on_device(m, b->sens.as<float>(), nullptr, b->stream)) return st;
+    HIPCHK(hipStreamSynchronize(b->stream));
+    *out = b.release();
+    return MLEM_OK;
+}
+
+int mlem_batch_set(mlem_batch *b, uint32_t slot, const int32_t *g, const float *f0) {
+    ARGCHK(b && g && f0, "mlem_batch_set: null argument");
+    ARGCHK(slot < b->slots, "mlem_batch_set: slot %u >= %u", slot, b->slots);
+    HIPCHK(hipSetDevice(b->m->device));
+    const uint32_t p = slot / b->BC, q = slot % b->BC;
+    const mlem_matrix *m = b->m;
+    hipStream_t s = b->stream;
+    HIPCHK(hipMemcpyAsync(b->stage.p, g, sizeof(int32_t) * m->rows, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_strided_put(b->g.as<int32_t>() + ((size_t)p * b->rows_x) * b->BC + q, b->BC, b->stage.p, m->rows, s));
+    HIPCHK(hipStreamSynchronize(s));  // the stage is reused below
+    HIPCHK(hipMemcpyAsync(b->stage.p, f0, sizeof(float) * m->cols, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_strided_put(b->f.as<float>() + ((size_t)p * b->cols_x) * b->BC + q, b->BC, b->stage.p, m->cols, s));
+    b->active_host[slot] = 1;
+    HIPCHK(hipMemcpyAsync(b->active.p, b->active_host.data(), sizeof(uint32_t) * b->active_host.size(),
+                          hipMemcpyHostToDevice, s));
+    HIPCHK(hipStreamSynchronize(s));
+    return MLEM_OK;
+}
+
+int mlem_batch_deactivate(mlem_batch *b, uint32_t slot) {
+    ARGCHK(b, "mlem_batch_deactivate: null handle");
+    ARGCHK(slot < b->slots, "mlem_batch_deactivate: slot %u >= %u", slot, b->slots);
+    HIPCHK(hipSetDevice(b->m->device));
+    b->active_host[slot] = 0;
+    HIPCHK(hipMemcpyAsync(b->active.p, b->active_host.data(), sizeof(uint32_t) * b->active_host.size(),
+                          hipMemcpyHostToDevice, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return MLEM_OK;
+}
+
+int mlem_batch_iterate(mlem_batch *b, uint32_t iters) {
+    ARGCHK(b, "mlem_batch_iterate: null handle");
+    HIPCHK(hipSetDevice(b->m->device));
+    hipStream_t s = b->stream;
+    const uint32_t BC = b->BC;
+    for (uint32_t it = 0; it < iters; ++it) {
+        for (uint32_t p = 0; p < b->passes; ++p) {
+            bool any = false;
+            for (uint32_t q = 0; q < BC; ++q) any = any || b->active_host[(size_t)p * BC + q];
+            if (!any) continue;
+            float *f = b->f.as<float>() + (size_t)p * b->cols_x * BC;
+            float *ratio = b->ratio.as<float>() + (size_t)p * b->rows_x * BC;
+            EpiArgs fa{};
+            fa.out = ratio;
+            fa.g = b->g.as<int32_t>() + (size_t)p * b->rows_x * BC;
+            HIPCHK(launch_tiled_partials(b->tfp->plan, b->tfp->view, b->wfp->view, f, b->part_fp.as<float>(), BC, s));
+            HIPCHK(launch_tiled_reduce(b->tfp->plan, b->tfp->view, b->wfp->view, b->part_fp.as<float>(), BC, EPI_RATIO,
+                                       fa, s));
+            EpiArgs ba{};
+            ba.norm = b->sens.as<float>();
+            ba.f = f;
+            ba.active = b->active.as<uint32_t>() + (size_t)p * BC;
+            HIPCHK(launch_tiled_partials(b->tbp->plan, b->tbp->view, b->wbp->view, ratio, b->part_bp.as<float>(), BC,
+                                         s));
+            HIPCHK(launch_tiled_reduce(b->tbp->plan, b->tbp->view, b->wbp->view, b->part_bp.as<float>(), BC,
+                                       EPI_UPDATE, ba, s));
+        }
+    }
+    return MLEM_OK;
+}
+
+int mlem_batch_sync(mlem_batch *b) {
+    ARGCHK(b, "mlem_batch_sync: null handle");
+    HIPCHK(hipSetDevice(b->m->device));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return MLEM_OK;
+}
+
+int mlem_batch_get_image(mlem_batch *b, uint32_t slot, float *f_out) {
+    ARGCHK(b && f_out, "mlem_batch_get_image: null argument");
+    ARGCHK(slot < b->slots, "mlem_batch_get_image: slot %u >= %u", slot, b->slots);
+    HIPCHK(hipSetDevice(b->m->device));
+    const uint32_t p = slot / b->BC, q = slot % b->BC;
+    HIPCHK(launch_strided_get(b->stage.p, b->f.as<float>() + ((size_t)p * b->cols_x) * b->BC + q, b->BC, b->m->cols,
+                              b->stream));
+    HIPCHK(hipMemcpyAsync(f_out, b->stage.p, sizeof(float) * b->m->cols, hipMemcpyDeviceToHost, b->stream));
+    HIPCHK(hipStreamSynchronize(b->stream));
+    return MLEM_OK;
+}
+
+int mlem_batch_info(const mlem_batch *b, uint32_t *slots, uint32_t *per_pass, uint32_t *passes,
+                    uint64_t *device_bytes) {
+    ARGCHK(b, "mlem_batch_info: null handle");
+    if (slots) *slots = b->slots;
+    if (per_pass) *per_pass = b->BC;
+    if (passes) *passes = b->passes;
+    if (device_bytes)
+        *device_bytes = b->g.bytes + b->ratio.bytes + b->f.bytes + b->sens.bytes + b->active.bytes + b->stage.bytes +
+                        b->part_fp.bytes + b->part_bp.bytes;
+    return MLEM_OK;
+}
+
+void *mlem_batch_stream(mlem_batch *b) { return b ? b->stream : nullptr; }
+
+int mlem_batch_destroy(mlem_batch *b) {
+    if (!b) return MLEM_OK;
+    (void)hipSetDevice(b->m->device);
+    delete b;
     return MLEM_OK;
 }
 

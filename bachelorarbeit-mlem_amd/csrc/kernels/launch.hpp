@@ -22,10 +22,11 @@ enum Epi : int {
 };
 
 struct EpiArgs {
-    float *out;         // PLAIN / RATIO output
-    const int32_t *g;   // RATIO counts
-    const float *norm;  // UPDATE sensitivity
-    float *f;           // UPDATE image, in place
+    float *out;                // PLAIN / RATIO output
+    const int32_t *g;          // RATIO counts
+    const float *norm;         // UPDATE sensitivity
+    float *f;                  // UPDATE image, in place
+    const uint32_t *active;    // blocked UPDATE over several right-hand sides: [BC] flags, NULL = all
 };
 
 // Tile partition of the merge path for one matrix and one secsize.
@@ -65,6 +66,9 @@ hipError_t launch_fill_f64(double *dst, double v, uint64_t n, hipStream_t s);
 // column is >= cols.
 hipError_t launch_aos_split(const void *aos, uint64_t n, uint64_t first, uint32_t cols, uint32_t *col, float *val,
                             uint32_t *bad, hipStream_t s);
+// 4-byte elements: dst[i*stride] = src[i] / dst[i] = src[i*stride], i < n.
+hipError_t launch_strided_put(void *dst, uint32_t stride, const void *src, uint64_t n, hipStream_t s);
+hipError_t launch_strided_get(void *dst, const void *src, uint32_t stride, uint64_t n, hipStream_t s);
 // dst = src (16-byte aligned), non-temporal streaming copy.
 hipError_t launch_copy(float *dst, const float *src, uint64_t n, hipStream_t s);
 hipError_t launch_colsum_rows_f64(const uint64_t *rowptrT, const float *valT, uint32_t cols, double *s64, float *s32,
@@ -88,17 +92,23 @@ constexpr uint32_t kTileXPad = 8192;     // pad gathered vectors to a multiple o
 constexpr uint32_t kTileSpanMax = 256;   // most tiles one run may touch
 constexpr uint32_t kTileE = 8;           // entries per work-item of a unit
 constexpr uint32_t kTileUnit = kWG * kTileE;
+// Stored value type of a tiled matrix: f32, or bf16 / fp16 holding
+// value / scale[row] with an f32 scale per row (the largest |value|).
+enum ValType : uint32_t { VAL_F32 = 0, VAL_BF16 = 1, VAL_F16 = 2 };
 struct TilePlan {
     uint32_t rows = 0, cols = 0;
     uint64_t nnz = 0;
     uint32_t W = 0, R = 0, nseg = 0, nblk = 0;
     uint64_t ntiles = 0;
     uint64_t cap = 0;  // entries including padding (set by launch_tile_count)
+    uint32_t vt = VAL_F32;
+    uint32_t vb() const { return vt == VAL_F32 ? 4u : 2u; }  // bytes per stored value
 };
 struct TiledMatrix {
     uint64_t *tile_off = nullptr;  // [ntiles+1] entry offsets (padded to 8)
     uint32_t *tile_nz = nullptr;   // [ntiles] entries including padding
-    uint8_t *tdata = nullptr;      // 6 bytes per entry: per unit, values then group words
+    uint8_t *tdata = nullptr;      // vb+2 bytes per entry: per unit, values then group words
+    float *scale = nullptr;        // [rows] value scales (16-bit values only)
 };
 // Work list on the host: 4 u32 per run {row block, first tile (index into
 // tiles), first unit in that tile, units}; 4 u32 per non-empty tile
@@ -115,12 +125,15 @@ struct TileWork {
     const uint4 *tiles = nullptr;
     const uint32_t *wl_first = nullptr;
 };
-TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R);
+TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t vt);
 bool valid_tile_width(uint32_t W);
+// Kernel shapes built: (W, right-hand sides per pass) in {1024, 2048} x 1,
+// {512, 1024} x 4, 512 x 8.
+bool valid_tile_config(uint32_t W, uint32_t BC);
 bool valid_tile_rows(uint32_t R);
 size_t tiled_matrix_bytes(const TilePlan &p);
-size_t tiled_part_bytes(const TilePlan &p, uint32_t nwg);
-size_t tiled_lds_bytes(const TilePlan &p, uint32_t span);
+size_t tiled_part_bytes(const TilePlan &p, uint32_t nwg, uint32_t BC);
+size_t tiled_lds_bytes(const TilePlan &p, uint32_t span, uint32_t BC);
 size_t tile_count_scratch_bytes(const TilePlan &p);
 size_t tile_fill_scratch_bytes(const TilePlan &p);  // after the count (uses p.cap)
 TiledMatrix tiled_matrix_view(const TilePlan &p, void *mem);
@@ -137,14 +150,15 @@ hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uin
 // tile_off.
 TileWorkList tile_work_list(const TilePlan &p, const uint32_t *tile_nz, const uint64_t *tile_off,
                             uint32_t target_wgs);
-// y = A x in two launches: the partial row sums of every run into part
-// (tiled_part_bytes(p, wk.nwg)), then their reduction with the epilogue.
-// x must be readable (and finite) up to nseg * W floats: callers pad their
-// vectors to whole segments (kTileXPad).
+// Y = A X for BC right-hand sides in two launches: the partial row sums of
+// every run into part (tiled_part_bytes(p, wk.nwg, BC)), then their
+// reduction with the row scales and the epilogue.  Vectors are [n][BC]
+// (right-hand side fastest); X must be readable (and finite) up to
+// nseg * W rows: callers pad their vectors to whole segments (kTileXPad).
 hipError_t launch_tiled_partials(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x,
-                                 float *part, hipStream_t s);
-hipError_t launch_tiled_reduce(const TilePlan &p, const TileWork &wk, const float *part, Epi epi, const EpiArgs &ea,
-                               hipStream_t s);
+                                 float *part, uint32_t BC, hipStream_t s);
+hipError_t launch_tiled_reduce(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *part,
+                               uint32_t BC, Epi epi, const EpiArgs &ea, hipStream_t s);
 
 // Device transpose (stable counting sort): see transpose.hip.
 size_t transpose_scratch_bytes(uint32_t rows, uint32_t cols, uint64_t nnz);

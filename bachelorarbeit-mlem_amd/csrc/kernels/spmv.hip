@@ -369,6 +369,18 @@ __global__ __launch_bounds__(kWG) void aos_split_kernel(const uint32_t *__restri
     if (flag) atomicOr(bad, 1u);
 }
 
+// One slot of an interleaved [n][stride] array <-> a contiguous vector.
+template <typename T>
+__global__ void strided_put_kernel(T *__restrict__ dst, uint32_t stride, const T *__restrict__ src, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i * stride] = src[i];
+}
+template <typename T>
+__global__ void strided_get_kernel(T *__restrict__ dst, const T *__restrict__ src, uint32_t stride, uint64_t n) {
+    for (uint64_t i = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; i < n; i += (uint64_t)gridDim.x * blockDim.x)
+        dst[i] = src[i * stride];
+}
+
 __global__ void f64_to_f32_kernel(const double *__restrict__ a, float *__restrict__ b, uint32_t n) {
     const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) b[i] = (float)a[i];
@@ -593,6 +605,22 @@ hipError_t launch_aos_split(const void *aos, uint64_t n, uint64_t first, uint32_
     const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((n + kWG - 1) / kWG, 16384));
     hipLaunchKernelGGL(aos_split_kernel, dim3((uint32_t)blocks), dim3(kWG), 0, s, static_cast<const uint32_t *>(aos),
                        n, first, cols, col, val, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_strided_put(void *dst, uint32_t stride, const void *src, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((n + kWG - 1) / kWG, 8192));
+    hipLaunchKernelGGL(strided_put_kernel<uint32_t>, dim3((uint32_t)blocks), dim3(kWG), 0, s,
+                       static_cast<uint32_t *>(dst), stride, static_cast<const uint32_t *>(src), n);
+    return hipGetLastError();
+}
+
+hipError_t launch_strided_get(void *dst, const void *src, uint32_t stride, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((n + kWG - 1) / kWG, 8192));
+    hipLaunchKernelGGL(strided_get_kernel<uint32_t>, dim3((uint32_t)blocks), dim3(kWG), 0, s,
+                       static_cast<uint32_t *>(dst), static_cast<const uint32_t *>(src), stride, n);
     return hipGetLastError();
 }
 

@@ -49,6 +49,8 @@
 
 #include <algorithm>
 
+#include <hip/hip_fp16.h>
+
 #include "launch.hpp"
 
 namespace mlem {
@@ -72,7 +74,7 @@ static_assert(kE == 8 && kUE == kWG * kE, "one group word per work-item");
 
 template <int W>
 constexpr int col_bits() {
-    return W == 1024 ? 11 : 12;  // columns 0..W (W = padding)
+    return W == 512 ? 10 : W == 1024 ? 11 : 12;  // columns 0..W (W = padding)
 }
 
 // Column j of a group word: bits [32 + j*CB, 32 + (j+1)*CB).
@@ -115,41 +117,61 @@ __device__ __forceinline__ Unit make_unit(const uint4 *tl, uint32_t ti, uint32_t
     return u;
 }
 
+// Value j (0..7) of a work-item's group from its loaded 16 bytes: f32
+// values come in two halves (entries 0-3, then 4-7), 16-bit values in one
+// load, 8 to a work-item.
+template <int VT>
+__device__ __forceinline__ float group_val(const f32x4 &h0, const f32x4 &h1, const u32x4 &q, int j) {
+    if constexpr (VT == VAL_F32) {
+        return j < 4 ? h0[j] : h1[j - 4];
+    } else if constexpr (VT == VAL_BF16) {
+        const uint32_t wv = q[j / 2];
+        return __uint_as_float((j & 1) ? (wv & 0xffff0000u) : (wv << 16));
+    } else {
+        const uint32_t wv = q[j / 2];
+        return __half2float(__ushort_as_half((unsigned short)((j & 1) ? (wv >> 16) : (wv & 0xffffu))));
+    }
+}
+
+template <int W, int VT, int BC>
+constexpr int min_waves() {
+    return BC == 1 && VT == VAL_F32 ? (W == 1024 ? 8 : 7) : 1;
+}
+
 // wl[w] = {row block, first tile (index into tiles), first unit in it, units}
 // tiles[i] = {segment, entries, entry offset lo, hi} of the non-empty tiles.
-template <int W>
-__global__ __launch_bounds__(kWG, W == 1024 ? 8 : 7) void blocked_spmv_kernel(uint32_t R, uint32_t span, uint32_t ntl,
-                                                           const uint4 *__restrict__ wl,
-                                                           const uint4 *__restrict__ tiles,
-                                                           const uint8_t *__restrict__ tdata,
-                                                           const float *__restrict__ x, float *__restrict__ part) {
-    constexpr int XR = W / kWG;
+// BC right-hand sides at once: x is [nseg*W][BC], part [runs][R][BC].
+template <int W, int VT, int BC>
+__global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_kernel(
+    uint32_t R, uint32_t span, uint32_t ntl, const uint4 *__restrict__ wl, const uint4 *__restrict__ tiles,
+    const uint8_t *__restrict__ tdata, const float *__restrict__ x, float *__restrict__ part) {
+    constexpr int VB = VT == VAL_F32 ? 4 : 2;  // bytes per stored value
+    constexpr int XF = W * BC / kWG;           // floats of an x segment per work-item
     constexpr int CB = col_bits<W>();
+    static_assert(XF % 4 == 0 && (BC == 1 || BC % 4 == 0), "x segments move in 16-byte pieces");
     extern __shared__ __attribute__((aligned(16))) float smem[];
     __shared__ uint32_t s_k0[kWG + 1];  // first key of every work-item (+ sentinel)
-    __shared__ float s_cf[kWG];         // carry after the last row end
     __shared__ uint32_t s_hf[kWG];      // work-item saw a row end
-    float *xs = smem;                                      // [W + 4]; xs[W] = 0 for padding entries
-    float *acc = xs + W + 4;                               // [R + kWG]: the rows, then one dummy word per work-item
-    uint4 *tl = reinterpret_cast<uint4 *>(acc + R + kWG);  // [span]
+    float *xs = smem;                                            // [(W + 4) * BC]; slot W = 0 for padding
+    float *acc = xs + (W + 4) * BC;                              // [(R + kWG) * BC]: rows, then dummies
+    float *s_cf = acc + (R + kWG) * BC;                          // [kWG * BC] carries
+    uint4 *tl = reinterpret_cast<uint4 *>(s_cf + kWG * BC);      // [span]
 
     const uint32_t tid = threadIdx.x;
     const uint4 w = wl[blockIdx.x];
-    for (uint32_t i = tid; i < R; i += kWG) acc[i] = 0.0f;
+    for (uint32_t i = tid; i < R * BC; i += kWG) acc[i] = 0.0f;
     for (uint32_t i = tid; i < span; i += kWG)
         if (w.y + i < ntl) tl[i] = tiles[w.y + i];
-    if (tid == 0) {
-        s_k0[kWG] = kNoKey;
-        xs[W] = 0.0f;
-    }
+    if (tid < BC) xs[W * BC + tid] = 0.0f;
+    if (tid == 0) s_k0[kWG] = kNoKey;
     __syncthreads();
 
     uint32_t left = w.w;  // units of the run not yet started
     Unit cur = make_unit(tl, 0, w.z, left > 0);
 
-    f32x4 pv[2];
-    u32x4 pg;
-    f32x4 px[XR / 4];
+    f32x4 pv[2] = {};
+    u32x4 pq = {}, pg = {};
+    f32x4 px[XF / 4];
     // Loads of unit u; an invalid u re-reads the start of the current tile
     // and lanes past the unit's end re-read lane 0's entries (every address
     // stays inside the unit, so the wait counts are exact).
@@ -158,25 +180,29 @@ __global__ __launch_bounds__(kWG, W == 1024 ? 8 : 7) void blocked_spmv_kernel(ui
         const uint64_t off = ((uint64_t)t.w << 32) | t.z;
         const uint32_t c = u.valid ? u.c : 0u;
         const uint32_t n8 = (min(kUE, t.y - c * kUE) + 7u) & ~7u;
-        const uint8_t *base = tdata + 6 * (off + (uint64_t)c * kUE);
+        const uint8_t *base = tdata + (VB + 2) * (off + (uint64_t)c * kUE);
         const uint32_t le = (tid * kE < (u.valid ? u.n : 1u)) ? tid * kE : 0u;
-        pv[0] = ldnt(reinterpret_cast<const f32x4 *>(base) + le / 8);
-        pv[1] = ldnt(reinterpret_cast<const f32x4 *>(base + 2 * n8) + le / 8);
-        pg = ldnt(reinterpret_cast<const u32x4 *>(base + 4 * n8) + le / 8);
+        if constexpr (VT == VAL_F32) {
+            pv[0] = ldnt(reinterpret_cast<const f32x4 *>(base) + le / 8);
+            pv[1] = ldnt(reinterpret_cast<const f32x4 *>(base + 2 * n8) + le / 8);
+        } else {
+            pq = ldnt(reinterpret_cast<const u32x4 *>(base) + le / 8);
+        }
+        pg = ldnt(reinterpret_cast<const u32x4 *>(base + VB * n8) + le / 8);
         // x is padded to whole segments (xlen >= nseg * W): no clamping.
-        const float *xb = x + (uint64_t)t.x * W;
+        const float *xb = x + (uint64_t)t.x * W * BC;
 #pragma unroll
-        for (int k = 0; k < XR / 4; ++k) px[k] = *(reinterpret_cast<const f32x4 *>(xb) + tid + kWG * k);
+        for (int k = 0; k < XF / 4; ++k) px[k] = *(reinterpret_cast<const f32x4 *>(xb) + tid + kWG * k);
     };
 
     if (cur.valid) prefetch(cur, 0);
     bool fresh = true;  // first unit of the run: its x segment is not staged yet
     while (cur.valid) {
         const f32x4 cv0 = pv[0], cv1 = pv[1];
-        const u32x4 cg = pg;
-        f32x4 cx[XR / 4];
+        const u32x4 cq = pq, cg = pg;
+        f32x4 cx[XF / 4];
 #pragma unroll
-        for (int k = 0; k < XR / 4; ++k) cx[k] = px[k];
+        for (int k = 0; k < XF / 4; ++k) cx[k] = px[k];
         --left;
         const bool more_in_tile = (cur.c + 1) * kUE < uniform_tile(tl, cur.ti).y;
         const Unit nxt = make_unit(tl, more_in_tile ? cur.ti : cur.ti + 1, more_in_tile ? cur.c + 1 : 0u, left > 0);
@@ -187,14 +213,14 @@ __global__ __launch_bounds__(kWG, W == 1024 ? 8 : 7) void blocked_spmv_kernel(ui
         // most W).
         const bool lane_valid = tid * kE < cur.n;  // entries [n, n8) are padding
         // A work-item past the unit's end reads lane 0's entries: give it the
-        // key R + tid (its own dummy accumulator word) and no row steps, so
+        // key R + tid (its own dummy accumulator row) and no row steps, so
         // whatever it adds lands there.
         const uint32_t word = lane_valid ? cg.x : 0u;
         const uint32_t key0 = lane_valid ? (cg.x & 0xffffu) : R + tid;
         s_k0[tid] = lane_valid ? key0 : kNoKey;
         if (cur.c == 0 || fresh) {  // a new tile: its x segment
 #pragma unroll
-            for (int k = 0; k < XR / 4; ++k) *(reinterpret_cast<f32x4 *>(xs) + tid + kWG * k) = cx[k];
+            for (int k = 0; k < XF / 4; ++k) *(reinterpret_cast<f32x4 *>(xs) + tid + kWG * k) = cx[k];
         }
         fresh = false;
         __syncthreads();
@@ -208,8 +234,7 @@ __global__ __launch_bounds__(kWG, W == 1024 ? 8 : 7) void blocked_spmv_kernel(ui
         // where the next work-item starts another row.  Every row that ends
         // here gets its partial added to the accumulator (this work-item is
         // the only writer of those rows in this unit); the other entries
-        // point at the dummy word acc[R + tid].  All reads come before any
-        // write.
+        // point at the dummy row R + tid.  All reads come before any write.
         const bool end7 = key7 != s_k0[tid + 1];
         uint32_t slot[kE];
         {
@@ -217,69 +242,128 @@ __global__ __launch_bounds__(kWG, W == 1024 ? 8 : 7) void blocked_spmv_kernel(ui
 #pragma unroll
             for (int m = 0; m < (int)kE; ++m) {
                 const bool end = m < 7 ? step[m + 1] != 0u : end7;
-                slot[m] = end ? k : R + tid;
+                slot[m] = (end ? k : R + tid) * BC;
                 if (m < 7) k += step[m + 1];
             }
         }
-        float old[kE];
-#pragma unroll
-        for (int m = 0; m < (int)kE; ++m) old[m] = acc[slot[m]];
-        // Products (padding reads column W, whose slot holds 0), then the
-        // running row sums, restarted after each row end.
-        const float v[8] = {cv0.x, cv0.y, cv0.z, cv0.w, cv1.x, cv1.y, cv1.z, cv1.w};
-        float run[kE];
-        float a = 0.0f;
+        float old[kE][BC];
 #pragma unroll
         for (int m = 0; m < (int)kE; ++m) {
-            a += v[m] * xs[group_col<CB>(cg, m)];
-            run[m] = a;
+            if constexpr (BC == 1) {
+                old[m][0] = acc[slot[m]];
+            } else {
+#pragma unroll
+                for (int q = 0; q < BC / 4; ++q) {
+                    const f32x4 o = *reinterpret_cast<const f32x4 *>(acc + slot[m] + 4 * q);
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) old[m][4 * q + b] = o[b];
+                }
+            }
+        }
+        // Products (padding reads column W, whose slot holds 0), the running
+        // row sums (restarted after each row end) added into the slots.
+        float a[BC];
+#pragma unroll
+        for (int b = 0; b < BC; ++b) a[b] = 0.0f;
+#pragma unroll
+        for (int m = 0; m < (int)kE; ++m) {
+            const float v = group_val<VT>(cv0, cv1, cq, m);
+            const float *xg = xs + group_col<CB>(cg, m) * BC;
+            float xv[BC];
+            if constexpr (BC == 1) {
+                xv[0] = xg[0];
+            } else {
+#pragma unroll
+                for (int q = 0; q < BC / 4; ++q) {
+                    const f32x4 t4 = *reinterpret_cast<const f32x4 *>(xg + 4 * q);
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) xv[4 * q + b] = t4[b];
+                }
+            }
             const bool end = m < 7 ? step[m + 1] != 0u : end7;
-            a = end ? 0.0f : a;
+#pragma unroll
+            for (int b = 0; b < BC; ++b) {
+                a[b] += v * xv[b];
+                old[m][b] += a[b];
+                a[b] = end ? 0.0f : a[b];
+            }
         }
 #pragma unroll
-        for (int m = 0; m < (int)kE; ++m) acc[slot[m]] = old[m] + run[m];
+        for (int m = 0; m < (int)kE; ++m) {
+            if constexpr (BC == 1) {
+                acc[slot[m]] = old[m][0];
+            } else {
+#pragma unroll
+                for (int q = 0; q < BC / 4; ++q) {
+                    f32x4 o;
+#pragma unroll
+                    for (int b = 0; b < 4; ++b) o[b] = old[m][4 * q + b];
+                    *reinterpret_cast<f32x4 *>(acc + slot[m] + 4 * q) = o;
+                }
+            }
+        }
         uint32_t ends = end7 ? 1u : 0u;
 #pragma unroll
         for (int j = 1; j < 8; ++j) ends |= step[j];
-        s_cf[tid] = a;  // the part of the last row that continues in the next work-item
+        // the part of the last row that continues in the next work-item
+#pragma unroll
+        for (int b = 0; b < BC; ++b) s_cf[tid * BC + b] = a[b];
         s_hf[tid] = ends;
         __syncthreads();
         // The row of entry 0 may have started in earlier work-items: when it
         // ends here, add their carries (nearest first, back to the last one
         // with a row end, whose carry belongs to this row too).
         if (ends) {
-            float cin = 0.0f;
+            float cin[BC];
+#pragma unroll
+            for (int b = 0; b < BC; ++b) cin[b] = 0.0f;
             for (int jj = (int)tid - 1; jj >= 0; --jj) {
-                cin += s_cf[jj];
+#pragma unroll
+                for (int b = 0; b < BC; ++b) cin[b] += s_cf[jj * BC + b];
                 if (s_hf[jj]) break;
             }
-            acc[key0] += cin;
+#pragma unroll
+            for (int b = 0; b < BC; ++b) acc[key0 * BC + b] += cin[b];
         }
         cur = nxt;
     }
     __syncthreads();
-    float *pw = part + (uint64_t)blockIdx.x * R;
-    for (uint32_t i = tid; i < R; i += kWG) pw[i] = acc[i];
+    float *pw = part + (uint64_t)blockIdx.x * R * BC;
+    for (uint32_t i = tid; i < R * BC; i += kWG) pw[i] = acc[i];
 }
 
 // Row r of block b = r / R sums the partials of the block's workgroups
-// wl_first[b] .. wl_first[b+1]-1 in order, then applies the epilogue.
-template <int EPI>
+// wl_first[b] .. wl_first[b+1]-1 in order (for each of the BC right-hand
+// sides), multiplies by the row's value scale (16-bit values) and applies
+// the epilogue.  Vectors are [rows][BC].
+template <int EPI, int BC>
 __global__ void blocked_reduce_kernel(const float *__restrict__ part, const uint32_t *__restrict__ wl_first,
-                                      uint32_t rshift, uint32_t rows, EpiArgs ea) {
+                                      uint32_t rshift, uint32_t rows, const float *__restrict__ scale, EpiArgs ea) {
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     const uint32_t b = r >> rshift, i = r & ((1u << rshift) - 1u);
-    float y = 0.0f;
-    for (uint32_t w = wl_first[b]; w < wl_first[b + 1]; ++w) y += part[((uint64_t)w << rshift) + i];
-    if constexpr (EPI == EPI_PLAIN) {
-        ea.out[r] = y;
-    } else if constexpr (EPI == EPI_RATIO) {
-        ea.out[r] = (y != 0.0f) ? (float)ea.g[r] / y : y;
-    } else {
-        const float v = ea.f[r] * y;
-        const float n = ea.norm[r];
-        ea.f[r] = (n == 0.0f) ? v : v / n;
+    float y[BC];
+#pragma unroll
+    for (int q = 0; q < BC; ++q) y[q] = 0.0f;
+    for (uint32_t w = wl_first[b]; w < wl_first[b + 1]; ++w) {
+        const float *pr = part + (((uint64_t)w << rshift) + i) * BC;
+#pragma unroll
+        for (int q = 0; q < BC; ++q) y[q] += pr[q];
+    }
+    const float sc = scale ? scale[r] : 1.0f;
+#pragma unroll
+    for (int q = 0; q < BC; ++q) {
+        const uint64_t k = (uint64_t)r * BC + q;
+        const float yq = scale ? y[q] * sc : y[q];
+        if constexpr (EPI == EPI_PLAIN) {
+            ea.out[k] = yq;
+        } else if constexpr (EPI == EPI_RATIO) {
+            ea.out[k] = (yq != 0.0f) ? (float)ea.g[k] / yq : yq;
+        } else {
+            const float v = ea.f[k] * yq;
+            const float n = ea.norm[r];
+            if (!ea.active || ea.active[q]) ea.f[k] = (n == 0.0f) ? v : v / n;
+        }
     }
 }
 
@@ -349,6 +433,56 @@ __global__ void row_segscan_kernel(uint32_t *__restrict__ cnt, uint32_t rows, ui
     }
 }
 
+// scale[r] = largest |value| of row r (1 for a row without nonzero values):
+// 16-bit tiles store value / scale.
+__global__ __launch_bounds__(kWG) void row_absmax_kernel(const uint64_t *__restrict__ rowptr,
+                                                         const float *__restrict__ val, uint32_t rows,
+                                                         float *__restrict__ scale) {
+    const uint32_t row = blockIdx.x * (kWG / 64) + threadIdx.x / 64;
+    const uint32_t lane = threadIdx.x % 64;
+    if (row >= rows) return;
+    float m = 0.0f;
+    for (uint64_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += 64) m = fmaxf(m, fabsf(val[k]));
+    for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
+    if (lane == 0) scale[row] = m > 0.0f ? m : 1.0f;
+}
+
+// f32 -> bf16 bits, round to nearest even (finite inputs).
+__device__ __forceinline__ uint16_t to_bf16(float f) {
+    const uint32_t u = __float_as_uint(f);
+    return (uint16_t)((u + 0x7fffu + ((u >> 16) & 1u)) >> 16);
+}
+
+__device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
+
+// 16-bit values of every row, one work-item per row in column order, with
+// error feedback: each value is rounded together with the rounding error
+// carried from the previous entry of its row, so the running sums of a row's
+// stored values stay within one rounding of the true ones (a smooth x then
+// sees almost no quantization error in the row sum).
+__global__ void quantize_rows_kernel(const uint64_t *__restrict__ rowptr, const float *__restrict__ val,
+                                     uint32_t rows, const float *__restrict__ scale, uint32_t vt,
+                                     uint16_t *__restrict__ q) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    const float inv = 1.0f / scale[row];
+    float e = 0.0f;
+    for (uint64_t k = rowptr[row]; k < rowptr[row + 1]; ++k) {
+        const float t = val[k] * inv + e;
+        uint16_t b;
+        float back;
+        if (vt == VAL_BF16) {
+            b = to_bf16(t);
+            back = from_bf16(b);
+        } else {
+            b = __half_as_ushort(__float2half_rn(t));
+            back = __half2float(__ushort_as_half(b));
+        }
+        q[k] = b;
+        e = t - back;
+    }
+}
+
 // Places every entry: its value straight into the unit it falls in, its
 // local column and row key into scratch (by entry offset) for the packing.
 __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__restrict__ rowptr,
@@ -357,7 +491,8 @@ __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__res
                                                            uint32_t nseg, uint32_t wshift,
                                                            const uint32_t *__restrict__ segoff,
                                                            const uint32_t *__restrict__ rowpos,
-                                                           const uint64_t *__restrict__ tile_off,
+                                                           const uint64_t *__restrict__ tile_off, uint32_t vt,
+                                                           const uint16_t *__restrict__ qval,
                                                            uint8_t *__restrict__ tdata, uint16_t *__restrict__ ctmp,
                                                            uint16_t *__restrict__ rtmp) {
     const uint32_t row = blockIdx.x * (kWG / 64) + threadIdx.x / 64;
@@ -373,10 +508,14 @@ __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__res
         const uint64_t off = tile_off[t];
         const uint64_t p = rowpos[rsn] + (k - rs - segoff[rsn]);  // position in the tile
         const uint64_t u = p / kUE, q = p % kUE;
-        const uint64_t n8u = std::min<uint64_t>(kUE, tile_off[t + 1] - off - u * kUE);
-        // entry q of the unit = entry j of work-item q/8: half j/4 of the values
-        const uint64_t vq = (q % 8 / 4) * (n8u / 2) + q / 8 * 4 + q % 4;
-        reinterpret_cast<float *>(tdata + 6 * (off + u * kUE))[vq] = val[k];
+        if (vt == VAL_F32) {
+            const uint64_t n8u = std::min<uint64_t>(kUE, tile_off[t + 1] - off - u * kUE);
+            // entry q of the unit = entry j of work-item q/8: half j/4 of the values
+            const uint64_t vq = (q % 8 / 4) * (n8u / 2) + q / 8 * 4 + q % 4;
+            reinterpret_cast<float *>(tdata + 6 * (off + u * kUE))[vq] = val[k];
+        } else {
+            reinterpret_cast<uint16_t *>(tdata + 4 * (off + u * kUE))[q] = qval[k];
+        }
         ctmp[off + p] = (uint16_t)(c & ((1u << wshift) - 1u));
         rtmp[off + p] = (uint16_t)i;
     }
@@ -391,7 +530,7 @@ template <int W>
 __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restrict__ ctmp,
                                                         const uint16_t *__restrict__ rtmp,
                                                         const uint64_t *__restrict__ tile_off, uint64_t ntiles,
-                                                        uint32_t R, uint8_t *__restrict__ tdata,
+                                                        uint32_t R, uint32_t vb, uint8_t *__restrict__ tdata,
                                                         uint32_t *__restrict__ bad) {
     constexpr int CB = col_bits<W>();
     const uint64_t q = blockIdx.x * (uint64_t)kWG + threadIdx.x;
@@ -440,7 +579,7 @@ __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restri
     word.y = (uint32_t)lo64;
     word.z = (uint32_t)(lo64 >> 32);
     word.w = w3;
-    *reinterpret_cast<u32x4 *>(tdata + 6 * (off + u * kUE) + 4 * n8u + 2 * e) = word;
+    *reinterpret_cast<u32x4 *>(tdata + (vb + 2) * (off + u * kUE) + vb * n8u + 2 * e) = word;
     if (big) atomicOr(bad, 4u);
 }
 
@@ -452,16 +591,27 @@ __global__ void fill_u16_kernel(uint16_t *__restrict__ p, uint16_t v, uint64_t n
 inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + per - 1) / per); }
 constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
-size_t lds_bytes(uint32_t W, uint32_t R, uint32_t span) {
-    return ((size_t)W + 4 + R + kWG) * 4 + (size_t)span * 16;
+size_t lds_bytes(uint32_t W, uint32_t R, uint32_t span, uint32_t BC) {
+    return ((size_t)W + 4 + R + kWG + kWG) * BC * 4 + (size_t)span * 16;
 }
 
-template <int W>
-hipError_t launch_w(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
+template <int W, int VT, int BC>
+hipError_t launch_k(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
                     hipStream_t s) {
-    hipLaunchKernelGGL((blocked_spmv_kernel<W>), dim3(wk.nwg), dim3(kWG), lds_bytes(W, p.R, wk.span), s, p.R,
-                       wk.span, wk.ntl, wk.wl, wk.tiles, tm.tdata, x, part);
+    hipLaunchKernelGGL((blocked_spmv_kernel<W, VT, BC>), dim3(wk.nwg), dim3(kWG), lds_bytes(W, p.R, wk.span, BC), s,
+                       p.R, wk.span, wk.ntl, wk.wl, wk.tiles, tm.tdata, x, part);
     return hipGetLastError();
+}
+
+template <int VT>
+hipError_t launch_vt(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
+                     uint32_t BC, hipStream_t s) {
+    if (BC == 1 && p.W == 1024) return launch_k<1024, VT, 1>(p, tm, wk, x, part, s);
+    if (BC == 1 && p.W == 2048) return launch_k<2048, VT, 1>(p, tm, wk, x, part, s);
+    if (BC == 4 && p.W == 512) return launch_k<512, VT, 4>(p, tm, wk, x, part, s);
+    if (BC == 4 && p.W == 1024) return launch_k<1024, VT, 4>(p, tm, wk, x, part, s);
+    if (BC == 8 && p.W == 512) return launch_k<512, VT, 8>(p, tm, wk, x, part, s);
+    return hipErrorInvalidValue;
 }
 
 size_t scan_bytes(const TilePlan &p) {
@@ -504,8 +654,9 @@ CountScratch count_scratch(const TilePlan &p, void *scratch) {
 
 }  // namespace
 
-TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R) {
+TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t vt) {
     TilePlan p;
+    p.vt = vt;
     p.rows = rows;
     p.cols = cols;
     p.nnz = nnz;
@@ -517,16 +668,22 @@ TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint3
     return p;
 }
 
-bool valid_tile_width(uint32_t W) { return W == 1024 || W == 2048; }
+bool valid_tile_width(uint32_t W) { return W == 512 || W == 1024 || W == 2048; }
+bool valid_tile_config(uint32_t W, uint32_t BC) {
+    return (BC == 1 && (W == 1024 || W == 2048)) || (BC == 4 && (W == 512 || W == 1024)) || (BC == 8 && W == 512);
+}
 bool valid_tile_rows(uint32_t R) { return R == 512 || R == 1024 || R == 2048 || R == 4096 || R == 8192 || R == 16384; }
 
-size_t tiled_lds_bytes(const TilePlan &p, uint32_t span) { return lds_bytes(p.W, p.R, span); }
+size_t tiled_lds_bytes(const TilePlan &p, uint32_t span, uint32_t BC) { return lds_bytes(p.W, p.R, span, BC); }
 
 size_t tiled_matrix_bytes(const TilePlan &p) {
-    return align256(8 * (p.ntiles + 1)) + align256(4 * p.ntiles) + align256(6 * p.cap);
+    return align256(8 * (p.ntiles + 1)) + align256(4 * p.ntiles) + align256((p.vb() + 2) * p.cap) +
+           align256(4 * (size_t)p.rows);
 }
 
-size_t tiled_part_bytes(const TilePlan &p, uint32_t nwg) { return sizeof(float) * (size_t)nwg * p.R; }
+size_t tiled_part_bytes(const TilePlan &p, uint32_t nwg, uint32_t BC) {
+    return sizeof(float) * (size_t)nwg * p.R * BC;
+}
 
 TileWorkList tile_work_list(const TilePlan &p, const uint32_t *nz, const uint64_t *off, uint32_t target_wgs) {
     const uint64_t UE = kUE;
@@ -586,7 +743,9 @@ size_t tile_count_scratch_bytes(const TilePlan &p) {
            align256(scan_bytes(p)) + 256;
 }
 
-size_t tile_fill_scratch_bytes(const TilePlan &p) { return 2 * align256(2 * p.cap); }
+size_t tile_fill_scratch_bytes(const TilePlan &p) {
+    return 2 * align256(2 * p.cap) + (p.vt == VAL_F32 ? 0 : align256(2 * p.nnz));
+}
 
 TiledMatrix tiled_matrix_view(const TilePlan &p, void *mem) {
     TiledMatrix t;
@@ -596,6 +755,8 @@ TiledMatrix tiled_matrix_view(const TilePlan &p, void *mem) {
     t.tile_nz = reinterpret_cast<uint32_t *>(c);
     c += align256(4 * p.ntiles);
     t.tdata = reinterpret_cast<uint8_t *>(c);
+    c += align256((p.vb() + 2) * p.cap);
+    t.scale = p.vt == VAL_F32 ? nullptr : reinterpret_cast<float *>(c);
     return t;
 }
 
@@ -654,22 +815,34 @@ hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uin
         (e = hipMemcpyAsync(tm.tile_nz, c.tile_nz, 4 * p.ntiles, hipMemcpyDeviceToDevice, s)) != hipSuccess)
         return e;
     // Padding entries: value 0 (and, in the packing, column W).
-    if ((e = hipMemsetAsync(tm.tdata, 0, 6 * p.cap, s)) != hipSuccess) return e;
+    if ((e = hipMemsetAsync(tm.tdata, 0, (p.vb() + 2) * p.cap, s)) != hipSuccess) return e;
+    uint16_t *qval = nullptr;
+    if (tm.scale && p.rows) {
+        qval = reinterpret_cast<uint16_t *>(static_cast<char *>(fill_scratch) + 2 * align256(2 * p.cap));
+        hipLaunchKernelGGL(row_absmax_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, val,
+                           p.rows, tm.scale);
+        hipLaunchKernelGGL(quantize_rows_kernel, dim3(blocks_for(p.rows, 64)), dim3(64), 0, s, rowptr, val, p.rows,
+                           tm.scale, p.vt, qval);
+    }
     if ((e = hipMemsetAsync(rtmp, 0xff, 2 * p.cap, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(fill_u16_kernel, dim3(1024), dim3(kWG), 0, s, ctmp, (uint16_t)p.W, p.cap);
     if (p.rows) {
         hipLaunchKernelGGL(row_segscan_kernel, dim3(blocks_for(p.rows, kWG)), dim3(kWG), 0, s, c.cnt, p.rows, p.nseg);
         hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, col, val,
-                           p.rows, p.R, p.nseg, wshift, c.cnt, c.rowpos, tm.tile_off, tm.tdata, ctmp, rtmp);
+                           p.rows, p.R, p.nseg, wshift, c.cnt, c.rowpos, tm.tile_off, p.vt, qval, tm.tdata, ctmp,
+                           rtmp);
     }
     if (p.ntiles) {
         const dim3 grid(blocks_for(p.cap / 8, kWG));
-        if (p.W == 1024)
+        if (p.W == 512)
+            hipLaunchKernelGGL(tile_pack_kernel<512>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles, p.R,
+                               p.vb(), tm.tdata, c.bad);
+        else if (p.W == 1024)
             hipLaunchKernelGGL(tile_pack_kernel<1024>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles, p.R,
-                               tm.tdata, c.bad);
+                               p.vb(), tm.tdata, c.bad);
         else
             hipLaunchKernelGGL(tile_pack_kernel<2048>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles, p.R,
-                               tm.tdata, c.bad);
+                               p.vb(), tm.tdata, c.bad);
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(flags_host, c.bad, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
@@ -677,32 +850,49 @@ hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uin
 }
 
 hipError_t launch_tiled_partials(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x,
-                                 float *part, hipStream_t s) {
+                                 float *part, uint32_t BC, hipStream_t s) {
     if (p.rows == 0 || wk.nwg == 0) return hipSuccess;
-    return p.W == 1024 ? launch_w<1024>(p, tm, wk, x, part, s) : launch_w<2048>(p, tm, wk, x, part, s);
+    switch (p.vt) {
+        case VAL_F32: return launch_vt<VAL_F32>(p, tm, wk, x, part, BC, s);
+        case VAL_BF16: return launch_vt<VAL_BF16>(p, tm, wk, x, part, BC, s);
+        default: return launch_vt<VAL_F16>(p, tm, wk, x, part, BC, s);
+    }
 }
 
-hipError_t launch_tiled_reduce(const TilePlan &p, const TileWork &wk, const float *part, Epi epi, const EpiArgs &ea,
-                               hipStream_t s) {
-    if (p.rows == 0) return hipSuccess;
+namespace {
+template <int BC>
+hipError_t launch_reduce_bc(const TilePlan &p, const TileWork &wk, const float *part, const float *scale, Epi epi,
+                            const EpiArgs &ea, hipStream_t s) {
     uint32_t rshift = 0;
     while ((1u << rshift) < p.R) ++rshift;
     const dim3 grid(blocks_for(p.rows, kWG)), block(kWG);
     switch (epi) {
         case EPI_PLAIN:
-            hipLaunchKernelGGL(blocked_reduce_kernel<EPI_PLAIN>, grid, block, 0, s, part, wk.wl_first, rshift, p.rows,
-                               ea);
+            hipLaunchKernelGGL((blocked_reduce_kernel<EPI_PLAIN, BC>), grid, block, 0, s, part, wk.wl_first, rshift,
+                               p.rows, scale, ea);
             break;
         case EPI_RATIO:
-            hipLaunchKernelGGL(blocked_reduce_kernel<EPI_RATIO>, grid, block, 0, s, part, wk.wl_first, rshift, p.rows,
-                               ea);
+            hipLaunchKernelGGL((blocked_reduce_kernel<EPI_RATIO, BC>), grid, block, 0, s, part, wk.wl_first, rshift,
+                               p.rows, scale, ea);
             break;
         default:
-            hipLaunchKernelGGL(blocked_reduce_kernel<EPI_UPDATE>, grid, block, 0, s, part, wk.wl_first, rshift, p.rows,
-                               ea);
+            hipLaunchKernelGGL((blocked_reduce_kernel<EPI_UPDATE, BC>), grid, block, 0, s, part, wk.wl_first, rshift,
+                               p.rows, scale, ea);
             break;
     }
     return hipGetLastError();
+}
+}  // namespace
+
+hipError_t launch_tiled_reduce(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *part,
+                               uint32_t BC, Epi epi, const EpiArgs &ea, hipStream_t s) {
+    if (p.rows == 0) return hipSuccess;
+    switch (BC) {
+        case 1: return launch_reduce_bc<1>(p, wk, part, tm.scale, epi, ea, s);
+        case 4: return launch_reduce_bc<4>(p, wk, part, tm.scale, epi, ea, s);
+        case 8: return launch_reduce_bc<8>(p, wk, part, tm.scale, epi, ea, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace mlem

@@ -72,7 +72,7 @@ class CsrHost(C.Structure):
 class Opts(C.Structure):
     _fields_ = [("spmv", u32), ("using_trans", u32), ("secsize_fp", u32), ("secsize_bp", u32),
                 ("fp_lanes", u32), ("bp_lanes", u32), ("profile", u32), ("use_graph", u32),
-                ("bp_spmv", u32), ("tile_width", u32), ("tile_rows", u32), ("tile_groups", u32)]
+                ("bp_spmv", u32), ("tile_width", u32), ("tile_rows", u32), ("tile_groups", u32), ("values", u32)]
 
 
 def geometry(n_rings, n_per_ring, radius, crystal, ring_pitch, grid, voxel, sub_rays) -> Geometry:

@@ -13,6 +13,7 @@ import numpy as np
 from . import (HIP_LIB_PATH, MLEM_OK, Csr, MlemError, Opts, _load, _ptr, cint, f32, f64, i32, sz, u32, u64, vp)
 
 __all__ = ["hip_lib", "device_count", "device_info", "DeviceArray", "Stream", "Event", "ResidentMatrix", "Recon",
+           "Batch", "BatchOpts",
            "mlem_run", "default_opts", "opts", "merge_workspace_bytes", "fp_merge", "bp_merge", "fp_csr_vector",
            "bp_csr_vector", "bp_scatter", "fp_ratio_merge", "fp_ratio_csr_vector", "bp_update_merge",
            "bp_update_csr_vector", "ratio", "update", "colsum_f64", "colsum_scatter_f64", "fill_f32", "copy_f32",
@@ -67,6 +68,16 @@ def hip_lib() -> C.CDLL:
             "mlem_matrix_create": (cint, [cint, u32, u32, u64, vp, vp, vp, vp, vp, vp, P(vp)]),
             "mlem_matrix_build_transpose": (cint, [vp]),
             "mlem_matrix_load_csr4": (cint, [cint, C.c_char_p, cint, P(vp), vp]),
+            "mlem_batch_opts_default": (None, [vp]),
+            "mlem_batch_create": (cint, [vp, u32, vp, P(vp)]),
+            "mlem_batch_set": (cint, [vp, u32, vp, vp]),
+            "mlem_batch_deactivate": (cint, [vp, u32]),
+            "mlem_batch_iterate": (cint, [vp, u32]),
+            "mlem_batch_sync": (cint, [vp]),
+            "mlem_batch_get_image": (cint, [vp, u32, vp]),
+            "mlem_batch_info": (cint, [vp, P(u32), P(u32), P(u32), P(u64)]),
+            "mlem_batch_stream": (vp, [vp]),
+            "mlem_batch_destroy": (cint, [vp]),
             "mlem_matrix_destroy": (cint, [vp]),
             "mlem_matrix_info": (cint, [vp, P(u32), P(u32), P(u64), P(cint), P(u64)]),
             "mlem_matrix_device_arrays": (cint, [vp, P(vp), P(vp), P(vp), P(vp), P(vp), P(vp)]),
@@ -227,15 +238,17 @@ def default_opts() -> Opts:
 
 
 def opts(spmv=0, using_trans=0, secsize_fp=8, secsize_bp=8, fp_lanes=0, bp_lanes=0, profile=0, use_graph=0,
-         bp_spmv=0, tile_width=0, tile_rows=0, tile_groups=0) -> Opts:
+         bp_spmv=0, tile_width=0, tile_rows=0, tile_groups=0, values=0) -> Opts:
     """bp_spmv: 0 = BP uses the spmv algorithm too, 1 = merge-based BP, 2 = CSR-vector BP.
     tile_width: 0 = automatic LDS tiling of the merge-based kernels, 1 = plain merge path, else the segment width.
-    tile_rows: rows per tile block (0 = default); tile_groups: workgroups per blocked SpMV (0 = default)."""
+    tile_rows: rows per tile block (0 = default); tile_groups: workgroups per blocked SpMV (0 = default).
+    values: stored matrix values of the blocked kernels, 0 = f32, 1 = bf16, 2 = fp16."""
     o = Opts()
     o.bp_spmv = bp_spmv
     o.tile_width = tile_width
     o.tile_rows = tile_rows
     o.tile_groups = tile_groups
+    o.values = values
     o.spmv, o.using_trans, o.secsize_fp, o.secsize_bp = spmv, using_trans, secsize_fp, secsize_bp
     o.fp_lanes, o.bp_lanes, o.profile, o.use_graph = fp_lanes, bp_lanes, profile, use_graph
     return o
@@ -320,6 +333,65 @@ def colsum_scatter_f64(rowptr, col, val, rows, cols, nnz, sens64=None, sens32=No
 
 
 # ---------------------------------------------------------------- handles
+class BatchOpts(C.Structure):
+    _fields_ = [("values", u32), ("per_pass", u32), ("tile_width", u32), ("tile_rows", u32), ("tile_groups", u32)]
+
+
+class Batch:
+    """Many reconstructions over one resident matrix (A and A^T), iterated together (mlem_batch_*)."""
+
+    def __init__(self, matrix: "ResidentMatrix", slots: int, values: int = 0, per_pass: int = 4, tile_width: int = 0,
+                 tile_rows: int = 0, tile_groups: int = 0):
+        self.h = vp()
+        self.matrix = matrix
+        o = BatchOpts(values, per_pass, tile_width, tile_rows, tile_groups)
+        _check(hip_lib().mlem_batch_create(matrix.h, slots, C.byref(o), C.byref(self.h)))
+        self.slots = slots
+
+    def set(self, slot: int, g: np.ndarray, f0: np.ndarray):
+        g = np.ascontiguousarray(g, dtype=np.int32)
+        f0 = np.ascontiguousarray(f0, dtype=np.float32)
+        if g.size < self.matrix.rows or f0.size != self.matrix.cols:
+            raise MlemError(1, "Batch.set: sinogram or image of the wrong size")
+        _check(hip_lib().mlem_batch_set(self.h, slot, _ptr(g), _ptr(f0)))
+
+    def deactivate(self, slot: int):
+        _check(hip_lib().mlem_batch_deactivate(self.h, slot))
+
+    def iterate(self, iters: int = 1):
+        _check(hip_lib().mlem_batch_iterate(self.h, iters))
+
+    def sync(self):
+        _check(hip_lib().mlem_batch_sync(self.h))
+
+    def image(self, slot: int) -> np.ndarray:
+        out = np.empty(self.matrix.cols, np.float32)
+        _check(hip_lib().mlem_batch_get_image(self.h, slot, _ptr(out)))
+        return out
+
+    def info(self):
+        a, b, c, d = u32(), u32(), u32(), u64()
+        _check(hip_lib().mlem_batch_info(self.h, C.byref(a), C.byref(b), C.byref(c), C.byref(d)))
+        return dict(slots=a.value, per_pass=b.value, passes=c.value, device_bytes=d.value)
+
+    def close(self):
+        if self.h:
+            hip_lib().mlem_batch_destroy(self.h)
+            self.h = vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
 class LoadStats(C.Structure):
     _fields_ = [("bytes", u64), ("open_s", f64), ("stream_s", f64), ("total_s", f64)]
 

@@ -197,6 +197,10 @@ typedef struct mlem_opts {
     uint32_t tile_groups; /* workgroups per blocked SpMV (runs of equal
                              length over each row block's tiles):
                              0 = 24 per CU                                   */
+    uint32_t values;      /* stored matrix values of the blocked kernels:
+                             0 = f32, 1 = bf16, 2 = fp16 (16-bit values are
+                             value / rowmax with an f32 scale per row; the
+                             products, sums, ratio and update stay f32)     */
 } mlem_opts;
 
 /* Fill opts with the defaults: merge-based FP and BP (argv[8] = 0),
@@ -278,6 +282,45 @@ int mlem_recon_loglik(mlem_recon *r, double *ll);
 /* The handle's stream (a hipStream_t). */
 void *mlem_recon_stream(mlem_recon *r);
 int mlem_recon_destroy(mlem_recon *r);
+
+/* ------------------------------------------------------------------------ */
+/* 2b. batched reconstructions (many sinograms, one resident matrix)        */
+/* ------------------------------------------------------------------------ */
+
+/* `slots` reconstructions over one matrix (A and A^T resident) iterated
+ * together: every pass of the blocked FP / BP kernels over the matrix serves
+ * `per_pass` slots at once, so the matrix is read once per per_pass
+ * reconstructions.  Each slot's arithmetic does not depend on the other
+ * slots' data: its image is the same whatever else is in the batch (with the
+ * same options).  The update skips inactive slots, whose images stay as
+ * they are.  No counterpart in the reference (one sinogram per process). */
+typedef struct mlem_batch mlem_batch;
+typedef struct mlem_batch_opts {
+    uint32_t values;      /* stored matrix values: 0 f32, 1 bf16, 2 fp16   */
+    uint32_t per_pass;    /* slots per pass over the matrix: 1, 4, 8
+                             (0 = 4)                                       */
+    uint32_t tile_width;  /* 0 = automatic (1024 for 1 per pass, else 512) */
+    uint32_t tile_rows;   /* 0 = automatic                                 */
+    uint32_t tile_groups; /* workgroups per blocked SpMV, 0 = automatic    */
+} mlem_batch_opts;
+void mlem_batch_opts_default(mlem_batch_opts *opts);
+/* The sensitivity (shared by all slots) is the matrix's FP64 column sums. */
+int mlem_batch_create(mlem_matrix *m, uint32_t slots,
+                      const mlem_batch_opts *opts, mlem_batch **out);
+/* Load a reconstruction into a slot (g int32[rows], f0 f32[cols]) and make
+ * it active. */
+int mlem_batch_set(mlem_batch *b, uint32_t slot, const int32_t *g,
+                   const float *f0);
+/* Stop updating a slot (its image stays readable). */
+int mlem_batch_deactivate(mlem_batch *b, uint32_t slot);
+/* Enqueue `iters` iterations of every pass that holds an active slot. */
+int mlem_batch_iterate(mlem_batch *b, uint32_t iters);
+int mlem_batch_sync(mlem_batch *b);
+int mlem_batch_get_image(mlem_batch *b, uint32_t slot, float *f_out);
+int mlem_batch_info(const mlem_batch *b, uint32_t *slots, uint32_t *per_pass,
+                    uint32_t *passes, uint64_t *device_bytes);
+void *mlem_batch_stream(mlem_batch *b);
+int mlem_batch_destroy(mlem_batch *b);
 
 /* ------------------------------------------------------------------------ */
 /* 3. whole reconstruction                                                  */

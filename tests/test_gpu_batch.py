@@ -1,0 +1,120 @@
+"""16-bit matrix values and batched reconstructions (many sinograms over one
+resident matrix) against the FP64 oracle, on the 2D ring matrix.
+
+Bars: f32 values 1e-5 rel-L2; fp16 values 1e-3; bf16 values 2e-3 (8-bit
+significand: the stored matrix differs from the FP32 one by up to 2^-9 per
+value, rounded with per-row error feedback; the products, sums, ratio and
+update stay FP32); a slot's image is bitwise the same alone and in a full
+batch."""
+import numpy as np
+import pytest
+
+import mlem_amd as M
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+ITERS = 30
+BAR = {0: 1e-5, 1: 2e-3, 2: 1e-3}
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+@pytest.fixture(scope="module")
+def setup(c1):
+    A = c1["A"]
+    m = M.ResidentMatrix(A, c1["AT"])
+    s32, s64 = m.sensitivity()
+    rng = np.random.default_rng(12)
+    # several sinograms: seeded phantoms, counts from 1e4 to 1e7
+    sinos = []
+    for k in range(10):
+        x = M.gen_phantom(c1["geo"], seed=100 + k, n_shapes=3 + k % 4)
+        sinos.append(M.gen_sinogram(A, x, float(10 ** rng.uniform(4, 7)), seed=200 + k))
+    f0 = np.ones(A.cols, np.float32)
+    refs = [O.mlem_f64(A.rowptr, A.col, A.val, A.rows, A.cols, g, s32.astype(np.float64), f0, ITERS) for g in sinos]
+    yield dict(m=m, s32=s32, sinos=sinos, f0=f0, refs=refs)
+    m.close()
+
+
+@pytest.mark.parametrize("values", [1, 2])
+def test_single_reconstruction_with_16bit_values(c1, setup, values):
+    bar = BAR[values]
+    print(f"single values={values}", flush=True)
+    o = M.opts(spmv=0, values=values)
+    img = M.mlem_run(setup["m"], setup["sinos"][0], setup["f0"], ITERS, sens=setup["s32"], options=o)
+    err = rel_l2(img, setup["refs"][0])
+    print(f"  rel-L2 {err:.2e}", flush=True)
+    assert err <= bar
+
+
+def test_16bit_values_need_the_blocked_path(setup):
+    with pytest.raises(M.MlemError):
+        M.Recon(setup["m"], setup["sinos"][0], setup["f0"], sens=setup["s32"], options=M.opts(spmv=1, values=1))
+
+
+@pytest.mark.parametrize("per_pass,width,values", [(1, 1024, 0), (4, 512, 0), (4, 1024, 0), (8, 512, 0), (4, 512, 1),
+                                                   (4, 512, 2), (8, 512, 2), (4, 1024, 1), (1, 1024, 2)])
+def test_batch_matches_oracle(setup, per_pass, width, values):
+    print(f"batch per_pass={per_pass} W={width} values={values}", flush=True)
+    n = len(setup["sinos"])
+    with M.Batch(setup["m"], n, values=values, per_pass=per_pass, tile_width=width) as b:
+        for k, g in enumerate(setup["sinos"]):
+            b.set(k, g, setup["f0"])
+        b.iterate(ITERS)
+        b.sync()
+        errs = [rel_l2(b.image(k), setup["refs"][k]) for k in range(n)]
+    print("  worst rel-L2", max(errs), flush=True)
+    assert max(errs) <= BAR[values]
+
+
+@pytest.mark.parametrize("values", [0, 1])
+def test_slot_alone_equals_slot_in_full_batch(setup, values):
+    k = 6
+    with M.Batch(setup["m"], 1, values=values, per_pass=4) as alone:
+        alone.set(0, setup["sinos"][k], setup["f0"])
+        alone.iterate(ITERS)
+        a = alone.image(0)
+    with M.Batch(setup["m"], len(setup["sinos"]), values=values, per_pass=4) as mix:
+        for j, g in enumerate(setup["sinos"]):
+            mix.set(j, g, setup["f0"])
+        mix.iterate(ITERS)
+        c = mix.image(k)
+    assert np.array_equal(a.view(np.uint32), c.view(np.uint32))
+
+
+def test_deactivated_slot_is_frozen_and_slots_are_reusable(setup):
+    with M.Batch(setup["m"], 5, per_pass=4) as b:
+        for k in range(5):
+            b.set(k, setup["sinos"][k], setup["f0"])
+        b.iterate(5)
+        frozen = b.image(2)
+        b.deactivate(2)
+        b.iterate(7)
+        assert np.array_equal(b.image(2).view(np.uint32), frozen.view(np.uint32))
+        # slot 4 is alone in the second pass: deactivating it skips that pass
+        b.deactivate(4)
+        last = b.image(4)
+        b.iterate(3)
+        assert np.array_equal(b.image(4).view(np.uint32), last.view(np.uint32))
+        # reuse slot 2 for another sinogram: it restarts from f0
+        b.set(2, setup["sinos"][7], setup["f0"])
+        b.iterate(ITERS)
+        assert rel_l2(b.image(2), setup["refs"][7]) <= 1e-5
+        info = b.info()
+        assert info["per_pass"] == 4 and info["passes"] == 2 and info["device_bytes"] > 0
+
+
+def test_batch_argument_errors(setup, c1):
+    with pytest.raises(M.MlemError):
+        M.Batch(setup["m"], 4, per_pass=3)
+    with pytest.raises(M.MlemError):
+        M.Batch(setup["m"], 4, per_pass=8, tile_width=1024)
+    with M.ResidentMatrix(c1["A"]) as no_t:
+        with pytest.raises(M.MlemError):
+            M.Batch(no_t, 4)
+    with M.Batch(setup["m"], 2) as b:
+        with pytest.raises(M.MlemError):
+            b.set(2, setup["sinos"][0], setup["f0"])
