@@ -152,9 +152,10 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
     extern __shared__ __attribute__((aligned(16))) float smem[];
     __shared__ uint32_t s_k0[kWG + 1];  // first key of every work-item (+ sentinel)
     __shared__ uint32_t s_hf[kWG];      // work-item saw a row end
+    constexpr uint32_t kDummy = BC == 1 ? kWG : 0;               // dummy rows (one right-hand side only)
     float *xs = smem;                                            // [(W + 4) * BC]; slot W = 0 for padding
-    float *acc = xs + (W + 4) * BC;                              // [(R + kWG) * BC]: rows, then dummies
-    float *s_cf = acc + (R + kWG) * BC;                          // [kWG * BC] carries
+    float *acc = xs + (W + 4) * BC;                              // [(R + kDummy) * BC]: rows, then dummies
+    float *s_cf = acc + (R + kDummy) * BC;                       // [kWG * BC] carries
     uint4 *tl = reinterpret_cast<uint4 *>(s_cf + kWG * BC);      // [span]
 
     const uint32_t tid = threadIdx.x;
@@ -213,8 +214,9 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
         // most W).
         const bool lane_valid = tid * kE < cur.n;  // entries [n, n8) are padding
         // A work-item past the unit's end reads lane 0's entries: give it the
-        // key R + tid (its own dummy accumulator row) and no row steps, so
-        // whatever it adds lands there.
+        // key R + tid (its own dummy accumulator row, one right-hand side) and
+        // no row steps, so whatever it adds lands there; with several
+        // right-hand sides it writes nothing.
         const uint32_t word = lane_valid ? cg.x : 0u;
         const uint32_t key0 = lane_valid ? (cg.x & 0xffffu) : R + tid;
         s_k0[tid] = lane_valid ? key0 : kNoKey;
@@ -235,14 +237,21 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
         // here gets its partial added to the accumulator (this work-item is
         // the only writer of those rows in this unit); the other entries
         // point at the dummy row R + tid.  All reads come before any write.
+        // One right-hand side: every entry reads and writes a slot (non-ends
+        // their work-item's dummy row), no branches.  Several: only the row
+        // ends touch the accumulator (a masked read and write each), which
+        // saves most of the LDS traffic of BC-wide slots.
         const bool end7 = key7 != s_k0[tid + 1];
+        constexpr bool kMasked = BC > 1;
+        bool endm[kE];
         uint32_t slot[kE];
         {
             uint32_t k = key0;
 #pragma unroll
             for (int m = 0; m < (int)kE; ++m) {
-                const bool end = m < 7 ? step[m + 1] != 0u : end7;
-                slot[m] = (end ? k : R + tid) * BC;
+                endm[m] = m < 7 ? step[m + 1] != 0u : end7;
+                if (kMasked) endm[m] = endm[m] && lane_valid;
+                slot[m] = (kMasked || endm[m] ? k : R + tid) * BC;
                 if (m < 7) k += step[m + 1];
             }
         }
@@ -253,10 +262,14 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
                 old[m][0] = acc[slot[m]];
             } else {
 #pragma unroll
-                for (int q = 0; q < BC / 4; ++q) {
-                    const f32x4 o = *reinterpret_cast<const f32x4 *>(acc + slot[m] + 4 * q);
+                for (int b = 0; b < BC; ++b) old[m][b] = 0.0f;
+                if (endm[m]) {
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) old[m][4 * q + b] = o[b];
+                    for (int q = 0; q < BC / 4; ++q) {
+                        const f32x4 o = *reinterpret_cast<const f32x4 *>(acc + slot[m] + 4 * q);
+#pragma unroll
+                        for (int b = 0; b < 4; ++b) old[m][4 * q + b] = o[b];
+                    }
                 }
             }
         }
@@ -293,18 +306,21 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
             if constexpr (BC == 1) {
                 acc[slot[m]] = old[m][0];
             } else {
+                if (endm[m]) {
 #pragma unroll
-                for (int q = 0; q < BC / 4; ++q) {
-                    f32x4 o;
+                    for (int q = 0; q < BC / 4; ++q) {
+                        f32x4 o;
 #pragma unroll
-                    for (int b = 0; b < 4; ++b) o[b] = old[m][4 * q + b];
-                    *reinterpret_cast<f32x4 *>(acc + slot[m] + 4 * q) = o;
+                        for (int b = 0; b < 4; ++b) o[b] = old[m][4 * q + b];
+                        *reinterpret_cast<f32x4 *>(acc + slot[m] + 4 * q) = o;
+                    }
                 }
             }
         }
         uint32_t ends = end7 ? 1u : 0u;
 #pragma unroll
         for (int j = 1; j < 8; ++j) ends |= step[j];
+        if (kMasked && !lane_valid) ends = 0;
         // the part of the last row that continues in the next work-item
 #pragma unroll
         for (int b = 0; b < BC; ++b) s_cf[tid * BC + b] = a[b];
@@ -592,7 +608,7 @@ inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + pe
 constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 size_t lds_bytes(uint32_t W, uint32_t R, uint32_t span, uint32_t BC) {
-    return ((size_t)W + 4 + R + kWG + kWG) * BC * 4 + (size_t)span * 16;
+    return ((size_t)W + 4 + R + (BC == 1 ? kWG : 0) + kWG) * BC * 4 + (size_t)span * 16;
 }
 
 template <int W, int VT, int BC>

@@ -94,6 +94,66 @@ def workload(name, threads):
     return geo, A, g, label
 
 
+VALUES = {"f32": 0, "bf16": 1, "fp16": 2}
+
+
+def run_batch(args, M, geo, A, g0, m, s32, threads, label):
+    """Batched reconstructions: B seeded sinograms iterated together over one matrix; compared with the
+    one-at-a-time FP32 path (the single Recon) timed in the same process."""
+    rng = np.random.default_rng(2026)
+    B = args.batch
+    t = time.time()
+    sinos = []
+    for k in range(B):
+        x = M.gen_phantom(geo, seed=1000 + k, n_shapes=4 + k % 5, a_min=0.5, a_max=4.0)
+        sinos.append(M.gen_sinogram(A, x, float(10 ** rng.uniform(6, 9)), seed=2000 + k, threads=threads))
+    log(f"{B} sinograms (counts log-uniform 1e6-1e9) in {time.time() - t:.1f} s")
+    f0 = np.ones(A.cols, np.float32)
+    # one-at-a-time FP32 reference path
+    o = M.opts(spmv=0, using_trans=0, profile=1)
+    with M.Recon(m, g0, f0, sens=s32, options=o) as r:
+        r.iterate(3)
+        r.sync()
+        t0 = time.perf_counter()
+        r.iterate(20)
+        r.sync()
+        t_single = (time.perf_counter() - t0) / 20
+    t = time.time()
+    with M.Batch(m, B, values=VALUES[args.values], per_pass=args.per_pass,
+                 tile_width=args.tile if args.tile > 1 else 0, tile_rows=args.tile_rows,
+                 tile_groups=args.tile_groups) as b:
+        for k in range(B):
+            b.set(k, sinos[k], f0)
+        info = b.info()
+        log(f"batch set-up {time.time() - t:.2f} s: {info}")
+        b.iterate(args.warmup)
+        b.sync()
+        M.device_sync()
+        t0 = time.perf_counter()
+        b.iterate(args.steps)
+        b.sync()
+        M.device_sync()
+        elapsed = time.perf_counter() - t0
+    t_iter = elapsed / args.steps
+    recon_s = B / (args.recon_iters * t_iter)
+    single_recon_s = 1.0 / (args.recon_iters * t_single)
+    log(f"batched: {args.steps} iterations of {B} sinograms in {elapsed * 1e3:.1f} ms = {t_iter * 1e3:.2f} ms per "
+        f"iteration ({t_iter / B * 1e3:.3f} ms per sinogram-iteration); one-at-a-time FP32 {t_single * 1e3:.2f} ms; "
+        f"{recon_s:.2f} recon/s of {args.recon_iters} iterations = {recon_s / single_recon_s:.2f}x")
+    out = {
+        "metric": METRIC, "value": round(recon_s, 3), "unit": "recon/s", "n_gpus": 1, "steps": args.steps,
+        "warmup": args.warmup, "ms_per_step": round(t_iter * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": f"{args.values} values, f32 arithmetic", "data": "synthetic",
+        "config": {"workload": label + f", batch of {B} sinograms", "batch": B, "per_pass": args.per_pass,
+                   "values": args.values, "iterations_per_recon": args.recon_iters, "passes": info["passes"],
+                   "rows": A.rows, "cols": A.cols, "nnz": A.nnz},
+        "one_at_a_time_f32": {"ms_per_iteration": round(t_single * 1e3, 4), "recon_s": round(single_recon_s, 3)},
+        "speedup_vs_one_at_a_time_f32": round(recon_s / single_recon_s, 3),
+    }
+    print(json.dumps(out), flush=True)
+    m.close()
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -110,6 +170,11 @@ def main():
     ap.add_argument("--threads", type=int, default=0, help="host threads (0: all)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
+    ap.add_argument("--batch", type=int, default=0,
+                    help="batched mode: B sinograms (seeded phantoms, counts log-uniform 1e6-1e9) over the matrix")
+    ap.add_argument("--per-pass", type=int, default=4, help="batched mode: sinograms per pass over the matrix")
+    ap.add_argument("--values", default="f32", choices=["f32", "bf16", "fp16"], help="stored matrix values")
+    ap.add_argument("--recon-iters", type=int, default=100, help="iterations per reconstruction (recon/s)")
     ap.add_argument("--load-test", action="store_true",
                     help="also write the matrix as a csr4 file and time the streamed loader on it")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -139,6 +204,9 @@ def main():
     s32, _ = m.sensitivity()
     log(f"FP64 sensitivity {time.time() - t:.2f} s")
     f0, init = M.initial_image(g, A.rows, s32)
+
+    if args.batch:
+        return run_batch(args, M, geo, A, g, m, s32, threads, label)
 
     # FP and BP kernels chosen independently: spmv selects FP, bp_* selects BP.
     fp_merge = args.fp == "merge"
