@@ -65,6 +65,14 @@ int hip_err(hipError_t e, const char *what, int line) {
 
 inline hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
 
+}  // namespace
+
+// The message mlem_last_error() returns, for the other translation units of
+// the library (server.hip).
+void mlem::set_last_error(const char *msg) { g_err = msg; }
+
+namespace {
+
 // Device memory owned by a handle.
 struct DevBuf {
     void *p = nullptr;

@@ -160,6 +160,9 @@ hipError_t launch_tiled_partials(const TilePlan &p, const TiledMatrix &tm, const
 hipError_t launch_tiled_reduce(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *part,
                                uint32_t BC, Epi epi, const EpiArgs &ea, hipStream_t s);
 
+// Sets the message of mlem_last_error() (this thread).
+void set_last_error(const char *msg);
+
 // Device transpose (stable counting sort): see transpose.hip.
 size_t transpose_scratch_bytes(uint32_t rows, uint32_t cols, uint64_t nnz);
 hipError_t launch_transpose(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr, const uint32_t *col,

@@ -13,7 +13,7 @@ import numpy as np
 from . import (HIP_LIB_PATH, MLEM_OK, Csr, MlemError, Opts, _load, _ptr, cint, f32, f64, i32, sz, u32, u64, vp)
 
 __all__ = ["hip_lib", "device_count", "device_info", "DeviceArray", "Stream", "Event", "ResidentMatrix", "Recon",
-           "Batch", "BatchOpts",
+           "Batch", "BatchOpts", "Server",
            "mlem_run", "default_opts", "opts", "merge_workspace_bytes", "fp_merge", "bp_merge", "fp_csr_vector",
            "bp_csr_vector", "bp_scatter", "fp_ratio_merge", "fp_ratio_csr_vector", "bp_update_merge",
            "bp_update_csr_vector", "ratio", "update", "colsum_f64", "colsum_scatter_f64", "fill_f32", "copy_f32",
@@ -78,6 +78,13 @@ def hip_lib() -> C.CDLL:
             "mlem_batch_info": (cint, [vp, P(u32), P(u32), P(u32), P(u64)]),
             "mlem_batch_stream": (vp, [vp]),
             "mlem_batch_destroy": (cint, [vp]),
+            "mlem_server_create": (cint, [u64, vp, P(vp)]),
+            "mlem_server_add_matrix": (cint, [vp, vp, u32, P(u32)]),
+            "mlem_server_submit": (cint, [vp, u32, vp, vp, u32, P(u64)]),
+            "mlem_server_step": (cint, [vp, u32, P(u32), P(u32)]),
+            "mlem_server_poll": (cint, [vp, u64, vp, P(f64), P(cint)]),
+            "mlem_server_info": (cint, [vp, P(u64), P(u64), P(u32), P(u64)]),
+            "mlem_server_destroy": (cint, [vp]),
             "mlem_matrix_destroy": (cint, [vp]),
             "mlem_matrix_info": (cint, [vp, P(u32), P(u32), P(u64), P(cint), P(u64)]),
             "mlem_matrix_device_arrays": (cint, [vp, P(vp), P(vp), P(vp), P(vp), P(vp), P(vp)]),
@@ -377,6 +384,75 @@ class Batch:
     def close(self):
         if self.h:
             hip_lib().mlem_batch_destroy(self.h)
+            self.h = vp()
+
+    def __enter__(self):
+        return self
+
+    def __exit__(self, *a):
+        self.close()
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+
+class Server:
+    """Continuous batching of reconstruction requests over resident matrices (mlem_server_*).
+
+    Each matrix added gets a batch of ``slots``; requests queue per matrix, join
+    between two iterations and leave when their iteration count is reached."""
+
+    def __init__(self, hbm_budget_bytes: int = 0, values: int = 0, per_pass: int = 4, tile_width: int = 0,
+                 tile_rows: int = 0, tile_groups: int = 0):
+        self.h = vp()
+        o = BatchOpts(values, per_pass, tile_width, tile_rows, tile_groups)
+        _check(hip_lib().mlem_server_create(hbm_budget_bytes, C.byref(o), C.byref(self.h)))
+        self.cols = []
+        self.rows = []
+        self.matrices = []  # keeps the matrices alive while served
+
+    def add_matrix(self, matrix: "ResidentMatrix", slots: int) -> int:
+        mid = u32()
+        _check(hip_lib().mlem_server_add_matrix(self.h, matrix.h, slots, C.byref(mid)))
+        self.matrices.append(matrix)
+        self.rows.append(matrix.rows)
+        self.cols.append(matrix.cols)
+        return mid.value
+
+    def submit(self, matrix_id: int, g: np.ndarray, f0: np.ndarray, iters: int) -> int:
+        g = np.ascontiguousarray(g, dtype=np.int32)
+        f0 = np.ascontiguousarray(f0, dtype=np.float32)
+        if matrix_id >= len(self.cols) or g.size < self.rows[matrix_id] or f0.size != self.cols[matrix_id]:
+            raise MlemError(1, "Server.submit: unknown matrix, or sinogram or image of the wrong size")
+        rid = u64()
+        _check(hip_lib().mlem_server_submit(self.h, matrix_id, _ptr(g), _ptr(f0), iters, C.byref(rid)))
+        return rid.value
+
+    def step(self, max_wait_us: int = 0xFFFFFFFF):
+        """One scheduling step (start an iteration on every idle busy batch, wait up to max_wait_us for
+        one to end); returns (active, queued) requests after it."""
+        a, q = u32(), u32()
+        _check(hip_lib().mlem_server_step(self.h, max_wait_us, C.byref(a), C.byref(q)))
+        return a.value, q.value
+
+    def poll(self, request_id: int, matrix_id: int):
+        """(image, latency_s) once the request is done (it is then released), else None."""
+        out = np.empty(self.cols[matrix_id], np.float32)
+        lat, done = f64(), cint()
+        _check(hip_lib().mlem_server_poll(self.h, request_id, _ptr(out), C.byref(lat), C.byref(done)))
+        return (out, lat.value) if done.value else None
+
+    def info(self):
+        b, u, m, s = u64(), u64(), u32(), u64()
+        _check(hip_lib().mlem_server_info(self.h, C.byref(b), C.byref(u), C.byref(m), C.byref(s)))
+        return dict(budget=b.value, used=u.value, matrices=m.value, steps=s.value)
+
+    def close(self):
+        if self.h:
+            hip_lib().mlem_server_destroy(self.h)
             self.h = vp()
 
     def __enter__(self):

@@ -154,6 +154,128 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label):
     m.close()
 
 
+def serve_drive(srv, reqs, arrivals, mats):
+    """Submits reqs[k] = (matrix index, sinogram index, iterations) at arrivals[k] seconds from the start and steps
+    the server until every request is done.  Returns (latencies in s, makespan in s)."""
+    order = np.argsort(arrivals, kind="stable")
+    lat = np.zeros(len(reqs))
+    live = {}
+    nxt, done = 0, 0
+    t0 = time.perf_counter()
+    last = t0
+    while done < len(reqs):
+        now = time.perf_counter() - t0
+        while nxt < len(reqs) and arrivals[order[nxt]] <= now:
+            k = int(order[nxt])
+            mi, j, it = reqs[k]
+            d = mats[mi]
+            live[srv.submit(d["id"], d["sinos"][j], d["f0"], it)] = k
+            nxt += 1
+        wait = 0xFFFFFFFF
+        if nxt < len(reqs):
+            wait = max(0, int((arrivals[order[nxt]] - (time.perf_counter() - t0)) * 1e6))
+            if not live:  # idle until the next arrival
+                time.sleep(wait * 1e-6)
+                continue
+        srv.step(wait)
+        for rid in list(live):
+            k = live[rid]
+            r = srv.poll(rid, mats[reqs[k][0]]["id"])
+            if r is not None:
+                lat[k] = r[1]
+                del live[rid]
+                done += 1
+                last = time.perf_counter()
+    return lat, last - t0
+
+
+def serial_fifo(reqs, arrivals, mats):
+    """The same requests served one at a time, first come first served, by the single-reconstruction FP32 path:
+    each takes iterations x its matrix's measured time per iteration (set-up not counted).  Returns latencies."""
+    t = 0.0
+    lat = np.zeros(len(reqs))
+    for k in np.argsort(arrivals, kind="stable"):
+        mi, _, it = reqs[k]
+        t = max(t, arrivals[k]) + it * mats[mi]["t_single"]
+        lat[k] = t - arrivals[k]
+    return lat
+
+
+def run_serve(args, M, threads):
+    """Serving mode: reconstruction requests over several resident matrices, continuous batching
+    (mlem_server_*).  Phase 1 submits every request at once (throughput, recon/s); phase 2 replays the mix with
+    Poisson arrivals at a fraction of that rate (latency p50/p99).  Both are compared with the one-at-a-time
+    FP32 path serving the same requests first come first served."""
+    names = args.serve.split(",")
+    mats = []
+    for name in names:
+        geo, A, g, label = workload(name, threads)
+        m = M.ResidentMatrix(A)
+        m.build_transpose()
+        s32, _ = m.sensitivity()
+        rng = np.random.default_rng(len(mats) + 5)
+        counts = float(np.sum(g, dtype=np.float64))
+        sinos = [g] + [M.gen_sinogram(A, M.gen_phantom(geo, seed=70 + k, n_shapes=4 + k, a_min=0.5, a_max=4.0),
+                                      counts * float(rng.uniform(0.3, 3.0)), seed=80 + k, threads=threads)
+                       for k in range(3)]
+        f0 = np.ones(A.cols, np.float32)
+        with M.Recon(m, g, f0, sens=s32, options=M.opts(spmv=0, using_trans=0)) as r:
+            r.iterate(3)
+            r.sync()
+            t0 = time.perf_counter()
+            r.iterate(20)
+            r.sync()
+            t_single = (time.perf_counter() - t0) / 20
+        log(f"{name}: {label}; one-at-a-time FP32 {t_single * 1e3:.3f} ms per iteration")
+        mats.append(dict(name=name, label=label, m=m, sinos=sinos, f0=f0, t_single=t_single, rows=A.rows,
+                         cols=A.cols, nnz=A.nnz))
+        del A
+    srv = M.Server(values=VALUES[args.values], per_pass=args.per_pass)
+    t = time.time()
+    for d in mats:
+        d["id"] = srv.add_matrix(d["m"], slots=args.serve_slots)
+    log(f"server set-up {time.time() - t:.2f} s: {srv.info()}")
+    rng = np.random.default_rng(2027)
+    n = args.serve_requests
+    choices = [int(c) for c in args.serve_iters.split(",")]
+    reqs = [(int(rng.integers(len(mats))), int(rng.integers(4)), int(rng.choice(choices))) for _ in range(n)]
+    # warm-up: one request per matrix
+    serve_drive(srv, [(i, 0, 2) for i in range(len(mats))], np.zeros(len(mats)), mats)
+    lat1, span1 = serve_drive(srv, reqs, np.zeros(n), mats)
+    recon_s = n / span1
+    serial_s = n / sum(it * mats[mi]["t_single"] for mi, _, it in reqs)
+    log(f"backlog of {n} requests served in {span1:.3f} s: {recon_s:.2f} recon/s (one at a time: {serial_s:.2f}, "
+        f"{recon_s / serial_s:.2f}x)")
+    rate = args.serve_load * recon_s
+    arr = np.cumsum(rng.exponential(1.0 / rate, n))
+    lat2, span2 = serve_drive(srv, reqs, arr, mats)
+    ser2 = serial_fifo(reqs, arr, mats)
+    pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 3)
+    log(f"Poisson arrivals at {rate:.2f} req/s: p50 {pct(lat2, 50)} ms, p99 {pct(lat2, 99)} ms "
+        f"(one at a time: p50 {pct(ser2, 50)} ms, p99 {pct(ser2, 99)} ms)")
+    info = srv.info()
+    srv.close()
+    for d in mats:
+        d["m"].close()
+    out = {
+        "metric": METRIC, "value": round(recon_s, 3), "unit": "recon/s", "n_gpus": 1, "steps": n,
+        "warmup": len(mats), "ms_per_step": round(span1 / n * 1e3, 4), "higher_is_better": True, "scaling": "weak",
+        "vs_baseline": None, "dtype": f"{args.values} values, f32 arithmetic", "data": "synthetic",
+        "config": {"workload": "served mix: " + " + ".join(d["label"] for d in mats), "matrices": names,
+                   "requests": n, "iterations_per_request": choices, "slots_per_matrix": args.serve_slots,
+                   "per_pass": args.per_pass, "values": args.values, "hbm_used_bytes": info["used"]},
+        "latency_ms": {"arrivals": f"Poisson, {rate:.3f} req/s ({args.serve_load} x backlog throughput)",
+                       "p50": pct(lat2, 50), "p99": pct(lat2, 99), "max": pct(lat2, 100),
+                       "backlog_p50": pct(lat1, 50), "backlog_p99": pct(lat1, 99)},
+        "one_at_a_time_f32": {"recon_s": round(serial_s, 3), "p50_ms": pct(ser2, 50), "p99_ms": pct(ser2, 99),
+                              "ms_per_iteration": {d["name"]: round(d["t_single"] * 1e3, 4) for d in mats},
+                              "note": "same requests first come first served by the single FP32 path, from its "
+                                      "measured time per iteration (set-up not counted)"},
+        "speedup_vs_one_at_a_time_f32": round(recon_s / serial_s, 3),
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +297,13 @@ def main():
     ap.add_argument("--per-pass", type=int, default=4, help="batched mode: sinograms per pass over the matrix")
     ap.add_argument("--values", default="f32", choices=["f32", "bf16", "fp16"], help="stored matrix values")
     ap.add_argument("--recon-iters", type=int, default=100, help="iterations per reconstruction (recon/s)")
+    ap.add_argument("--serve", default="",
+                    help="serving mode over these workloads (comma list of c1, mid, c2): continuous batching")
+    ap.add_argument("--serve-requests", type=int, default=48, help="serving mode: number of requests")
+    ap.add_argument("--serve-slots", type=int, default=16, help="serving mode: slots per matrix")
+    ap.add_argument("--serve-iters", default="20,50,100", help="serving mode: iteration counts drawn per request")
+    ap.add_argument("--serve-load", type=float, default=0.7,
+                    help="serving mode: Poisson arrival rate as a fraction of the backlog throughput")
     ap.add_argument("--load-test", action="store_true",
                     help="also write the matrix as a csr4 file and time the streamed loader on it")
     ap.add_argument("--calibrate-copy", action="store_true",
@@ -192,6 +321,8 @@ def main():
     threads = args.threads or min(16, os.cpu_count() or 1)
     name, mem, cus = M.device_info(0)
     log(f"device {name}, {mem / 2**30:.0f} GiB, {cus} CUs; host {cpu_model()}, {threads} threads")
+    if args.serve:
+        return run_serve(args, M, threads)
     geo, A, g, label = workload(args.workload, threads)
 
     t = time.time()

@@ -323,6 +323,42 @@ void *mlem_batch_stream(mlem_batch *b);
 int mlem_batch_destroy(mlem_batch *b);
 
 /* ------------------------------------------------------------------------ */
+/* 2c. serving: continuous batching over resident matrices                  */
+/* ------------------------------------------------------------------------ */
+
+/* A request loop over several resident matrices, each with a batch of slots
+ * (mlem_batch_*).  Admission: a matrix and its batch are registered only if
+ * they fit in what is left of the HBM budget (0 = no limit); requests queue
+ * per matrix and take a free slot of their batch between two of its
+ * iterations.  mlem_server_step starts one iteration on every batch that has
+ * active slots and none in flight (each batch on its own stream: batches of
+ * different matrices overlap and go at their own pace), waits up to
+ * max_wait_us for one to end and retires the requests that reached their
+ * iteration count.  A request's image is
+ * the same as when it runs alone (same batch options).  Driven by the
+ * caller's thread; not thread-safe.  No counterpart in the reference (one
+ * reconstruction per process). */
+typedef struct mlem_server mlem_server;
+int mlem_server_create(uint64_t hbm_budget_bytes,
+                       const mlem_batch_opts *batch_opts, mlem_server **out);
+/* The matrix (A and A^T resident) stays owned by the caller. */
+int mlem_server_add_matrix(mlem_server *s, mlem_matrix *m, uint32_t max_slots,
+                           uint32_t *matrix_id);
+/* g int32[rows] and f0 f32[cols] are copied. */
+int mlem_server_submit(mlem_server *s, uint32_t matrix_id, const int32_t *g,
+                       const float *f0, uint32_t iters, uint64_t *request_id);
+/* *active = requests in slots after the step, *queued = waiting (both may be
+ * NULL).  max_wait_us = 0 only collects iterations that already ended. */
+int mlem_server_step(mlem_server *s, uint32_t max_wait_us, uint32_t *active, uint32_t *queued);
+/* *done = 1 with the image (f32[cols], may be NULL) and the latency from
+ * submission when finished (the request is then released), else 0. */
+int mlem_server_poll(mlem_server *s, uint64_t request_id, float *f_out,
+                     double *latency_s, int *done);
+int mlem_server_info(const mlem_server *s, uint64_t *budget, uint64_t *used,
+                     uint32_t *matrices, uint64_t *steps);
+int mlem_server_destroy(mlem_server *s);
+
+/* ------------------------------------------------------------------------ */
 /* 3. whole reconstruction                                                  */
 /* ------------------------------------------------------------------------ */
 

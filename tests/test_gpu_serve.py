@@ -1,0 +1,149 @@
+"""The serving loop (continuous batching over resident matrices): requests of
+different iteration counts on two matrices join and leave their batches
+between iterations; each result is bitwise the image the same request gets
+alone in a batch of its own, and within the FP64-oracle bar (1e-5 rel-L2 for
+f32 values).  Also: the HBM budget refuses a matrix that does not fit, more
+requests than slots queue, and bad arguments are reported."""
+import numpy as np
+import pytest
+
+import mlem_amd as M
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def rel_l2(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def ring(n_per_ring, radius, grid):
+    return M.geometry(1, n_per_ring, radius, (0.0, 0.0, 0.0), 0.0, (grid, grid, 1), (1.0, 1.0, 1.0), 0)
+
+
+@pytest.fixture(scope="module")
+def served(c1):
+    """Two resident matrices (288- and 192-crystal rings), seeded sinograms for each."""
+    mats = []
+    for k, geo in enumerate((c1["geo"], ring(192, 30.0, 48))):
+        A = c1["A"] if k == 0 else M.gen_system_matrix(geo, seed=3)
+        AT = c1["AT"] if k == 0 else M.transpose(A)
+        m = M.ResidentMatrix(A, AT)
+        s32, _ = m.sensitivity()
+        rng = np.random.default_rng(40 + k)
+        sinos = [M.gen_sinogram(A, M.gen_phantom(geo, seed=300 + 10 * k + j, n_shapes=3 + j % 3),
+                                float(10 ** rng.uniform(4, 6.5)), seed=400 + 10 * k + j) for j in range(6)]
+        mats.append(dict(A=A, m=m, s32=s32, sinos=sinos, f0=np.ones(A.cols, np.float32)))
+    yield mats
+    for d in mats:
+        d["m"].close()
+
+
+def alone(d, g, iters, per_pass):
+    with M.Batch(d["m"], 1, per_pass=per_pass) as b:
+        b.set(0, g, d["f0"])
+        b.iterate(iters)
+        b.sync()
+        return b.image(0)
+
+
+@pytest.mark.parametrize("per_pass", [1, 4])
+def test_mixed_requests_match_alone_and_oracle(served, per_pass):
+    print(f"serve per_pass={per_pass}", flush=True)
+    srv = M.Server(per_pass=per_pass)
+    ids = [srv.add_matrix(d["m"], slots=5) for d in served]
+    # 12 requests, iteration counts 3..20, submitted in two waves so that the
+    # second wave joins batches that are already iterating
+    plan = [(k % 2, k // 2, [7, 20, 3, 12, 9, 15][k // 2] + (k % 2)) for k in range(12)]
+    rid = {}
+    for k, (mi, j, it) in enumerate(plan[:7]):
+        rid[k] = srv.submit(ids[mi], served[mi]["sinos"][j], served[mi]["f0"], it)
+    results, steps = {}, 0
+    while len(results) < len(plan):
+        if steps == 4:
+            for k, (mi, j, it) in enumerate(plan[7:], start=7):
+                rid[k] = srv.submit(ids[mi], served[mi]["sinos"][j], served[mi]["f0"], it)
+        active, queued = srv.step()
+        steps += 1
+        assert active <= 10
+        for k in list(rid):
+            r = srv.poll(rid[k], ids[plan[k][0]])
+            if r is not None:
+                results[k] = r
+                del rid[k]
+        assert steps < 200
+    for k, (mi, j, it) in enumerate(plan):
+        d = served[mi]
+        img, lat = results[k]
+        assert lat > 0
+        ref_alone = alone(d, d["sinos"][j], it, per_pass)
+        assert np.array_equal(img.view(np.uint32), ref_alone.view(np.uint32)), f"request {k} differs from alone"
+        A = d["A"]
+        ref = O.mlem_f64(A.rowptr, A.col, A.val, A.rows, A.cols, d["sinos"][j], d["s32"].astype(np.float64),
+                         d["f0"], it)
+        err = rel_l2(img, ref)
+        print(f"  request {k}: matrix {mi} iters {it} rel-L2 {err:.2e} latency {lat * 1e3:.1f} ms", flush=True)
+        assert err <= 1e-5
+    info = srv.info()
+    assert info["matrices"] == 2 and info["steps"] == steps
+    srv.close()
+
+
+def test_more_requests_than_slots_queue(served):
+    d = served[1]
+    srv = M.Server(per_pass=4)
+    mid = srv.add_matrix(d["m"], slots=2)
+    rids = [srv.submit(mid, d["sinos"][j], d["f0"], 4) for j in range(5)]
+    seen = []
+    done = {}
+    while len(done) < 5:
+        active, queued = srv.step()
+        seen.append((active, queued))
+        for r in rids:
+            if r not in done:
+                out = srv.poll(r, mid)
+                if out is not None:
+                    done[r] = out[0]
+        assert len(seen) < 50
+    assert seen[0] == (2, 3)
+    assert max(a for a, _ in seen) <= 2
+    for j, r in enumerate(rids):
+        assert np.array_equal(done[r].view(np.uint32), alone(d, d["sinos"][j], 4, 4).view(np.uint32))
+    srv.close()
+
+
+def test_budget_refuses_a_matrix_that_does_not_fit(served):
+    d0, d1 = served
+    srv = M.Server(hbm_budget_bytes=1)
+    with pytest.raises(M.MlemError) as e:
+        srv.add_matrix(d0["m"], slots=4)
+    assert "budget" in str(e.value)
+    assert srv.info()["used"] == 0 and srv.info()["matrices"] == 0
+    srv.close()
+    # a budget that holds the small matrix and its batch but not both matrices
+    probe = M.Server()
+    probe.add_matrix(d1["m"], slots=4)
+    need1 = probe.info()["used"]
+    probe.close()
+    srv = M.Server(hbm_budget_bytes=need1 + 1024)
+    srv.add_matrix(d1["m"], slots=4)
+    assert srv.info()["used"] == need1
+    with pytest.raises(M.MlemError):
+        srv.add_matrix(d0["m"], slots=4)
+    assert srv.info()["matrices"] == 1
+    srv.close()
+
+
+def test_bad_requests_are_reported(served):
+    d = served[1]
+    srv = M.Server()
+    mid = srv.add_matrix(d["m"], slots=1)
+    with pytest.raises(M.MlemError):
+        srv.submit(mid + 1, d["sinos"][0], d["f0"], 3)
+    with pytest.raises(M.MlemError):
+        srv.submit(mid, d["sinos"][0], d["f0"], 0)
+    with pytest.raises(M.MlemError):
+        srv.poll(12345, mid)
+    # an idle server steps without work
+    assert srv.step() == (0, 0)
+    srv.close()
