@@ -167,10 +167,10 @@ struct mlem_matrix {
     bool has_T = false;
     std::mutex mu;
     std::map<std::pair<int, uint32_t>, std::unique_ptr<Partition>> parts;  // (0: A / 1: A^T, secsize)
-    // Tiles keyed by (0: A / 1: A^T, width W, rows R).
-    // Tiles keyed by (0: A / 1: A^T, width W, rows R, value type).
-    std::map<std::tuple<int, uint32_t, uint32_t, uint32_t>, std::unique_ptr<Tiled>> tiles;
-    std::map<std::tuple<int, uint32_t, uint32_t, uint32_t>, uint32_t> tile_refused;  // build flags of refused tilings
+    // Tiles keyed by (0: A / 1: A^T, width W, rows R, value type, quad groups).
+    using TileKey = std::tuple<int, uint32_t, uint32_t, uint32_t, bool>;
+    std::map<TileKey, std::unique_ptr<Tiled>> tiles;
+    std::map<TileKey, uint32_t> tile_refused;  // build flags of refused tilings
 
     uint64_t device_bytes() const {
         uint64_t b = rowptr.bytes + col.bytes + val.bytes + rowptrT.bytes + colT.bytes + valT.bytes;
@@ -247,9 +247,10 @@ int get_partition(mlem_matrix *m, int which, uint32_t secsize, const Partition *
 // Tiles of A (which 0) or A^T (which 1) with segment width W and R rows per
 // block, built on first use.  MLEM_ERR_UNSUPPORTED (and *flags) when the
 // matrix does not allow them (a row not sorted by column).
-int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, const Tiled **out, uint32_t *flags) {
+int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bool quad, const Tiled **out,
+              uint32_t *flags) {
     std::lock_guard<std::mutex> lk(m->mu);
-    auto key = std::make_tuple(which, W, R, vt);
+    auto key = std::make_tuple(which, W, R, vt, quad);
     *flags = 0;
     auto it = m->tiles.find(key);
     if (it != m->tiles.end()) {
@@ -263,7 +264,7 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, co
     }
     auto t = std::make_unique<Tiled>();
     const uint32_t nrow = which == 0 ? m->rows : m->cols, ncol = which == 0 ? m->cols : m->rows;
-    t->plan = tile_plan(nrow, ncol, m->nnz, W, R, vt);
+    t->plan = tile_plan(nrow, ncol, m->nnz, W, R, vt, quad);
     const uint64_t *rp = which == 0 ? m->rowptr.as<uint64_t>() : m->rowptrT.as<uint64_t>();
     const uint32_t *ci = which == 0 ? m->col.as<uint32_t>() : m->colT.as<uint32_t>();
     const float *va = which == 0 ? m->val.as<float>() : m->valT.as<float>();
@@ -339,7 +340,7 @@ int setup_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, 
         return set_err(MLEM_ERR_UNSUPPORTED, "no blocked kernel for width %u with %u right-hand sides", W, BC);
     uint32_t flags = 0;
     const Tiled *t = nullptr;
-    if (int st = get_tiled(m, which, W, R, vt, &t, &flags)) return st;
+    if (int st = get_tiled(m, which, W, R, vt, BC > 1, &t, &flags)) return st;
     int dev = 0, lds_max = 65536, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) {
         (void)hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev);

@@ -102,6 +102,10 @@ struct TilePlan {
     uint64_t ntiles = 0;
     uint64_t cap = 0;  // entries including padding (set by launch_tile_count)
     uint32_t vt = VAL_F32;
+    // Group layout: false = row steps (a group of 8 may hold several rows,
+    // the single right-hand-side kernels), true = quads (every 4 entries
+    // belong to one row, the several right-hand-side kernels).
+    bool quad = false;
     uint32_t vb() const { return vt == VAL_F32 ? 4u : 2u; }  // bytes per stored value
 };
 struct TiledMatrix {
@@ -125,7 +129,7 @@ struct TileWork {
     const uint4 *tiles = nullptr;
     const uint32_t *wl_first = nullptr;
 };
-TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t vt);
+TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t vt, bool quad);
 bool valid_tile_width(uint32_t W);
 // Kernel shapes built: (W, right-hand sides per pass) in {1024, 2048} x 1,
 // {512, 1024} x 4, 512 x 8.

@@ -28,6 +28,13 @@
 //   so an entry costs 6 bytes (CSR: 8) and a unit is one 12 KiB stream.
 //   tile_off u64[ntiles+1] (entry offsets, padded to 8), tile_nz u32[ntiles]
 //   (entries including padding).
+//   Quad groups (the kernels for several right-hand sides): every row of a
+//   tile starts at a multiple of 4 entries, so each half of a group (a quad)
+//   holds one row; the group word's bits 0-15 hold the row of entries 0-3,
+//   bits 16-31 the row of entries 4-7 (the columns as above).  A row then
+//   ends only after entry 3 or 7 of a work-item, which is where the
+//   accumulator is touched; the price is the padding to whole quads (about
+//   1.5 entries per row segment).
 //
 // Work: work-item k owns entries [8k, 8k+8) of a unit (two 16-byte value
 // loads, one group word).  The units of a row block (its non-empty tiles in
@@ -66,6 +73,8 @@ __device__ __forceinline__ T ldnt(const T *p) {
 }
 
 constexpr uint32_t kE = kTileE;          // entries per work-item of a unit
+constexpr uint32_t kSNT = 512;           // work-items of a stream_spmv_kernel workgroup (groups per window)
+constexpr uint32_t kRunWindows = 64;     // windows of a stream run at most
 constexpr uint32_t kUE = kTileUnit;      // entries per unit
 constexpr uint32_t kNoKey = 0x10000u;    // past every u16 row key
 constexpr uint32_t kMaxRowStep = 3;      // largest key step inside a group of 8 entries
@@ -138,20 +147,42 @@ constexpr int min_waves() {
     return BC == 1 && VT == VAL_F32 ? (W == 1024 ? 8 : 7) : 1;
 }
 
+// Units whose loads are in flight per workgroup.  One right-hand side: 8
+// workgroups per CU fit in LDS, one unit each is enough.  Several: the
+// accumulator of BC values per row limits a CU to 2-3 workgroups, so each
+// keeps more units in flight.
+template <int W, int VT, int BC>
+constexpr int pipeline_depth() {
+    return BC == 1 ? 1 : 3;
+}
+
+// Adds the BC sums sv into the accumulator at slot (16-byte pieces).
+template <int BC>
+__device__ __forceinline__ void acc_add(float *acc, uint32_t slot, const float *sv) {
+#pragma unroll
+    for (int q = 0; q < BC / 4; ++q) {
+        f32x4 o = *reinterpret_cast<const f32x4 *>(acc + slot + 4 * q);
+#pragma unroll
+        for (int b = 0; b < 4; ++b) o[b] += sv[4 * q + b];
+        *reinterpret_cast<f32x4 *>(acc + slot + 4 * q) = o;
+    }
+}
+
 // wl[w] = {row block, first tile (index into tiles), first unit in it, units}
 // tiles[i] = {segment, entries, entry offset lo, hi} of the non-empty tiles.
 // BC right-hand sides at once: x is [nseg*W][BC], part [runs][R][BC].
-template <int W, int VT, int BC>
+template <int W, int VT, int BC, int D>
 __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_kernel(
     uint32_t R, uint32_t span, uint32_t ntl, const uint4 *__restrict__ wl, const uint4 *__restrict__ tiles,
     const uint8_t *__restrict__ tdata, const float *__restrict__ x, float *__restrict__ part) {
     constexpr int VB = VT == VAL_F32 ? 4 : 2;  // bytes per stored value
     constexpr int XF = W * BC / kWG;           // floats of an x segment per work-item
     constexpr int CB = col_bits<W>();
-    static_assert(XF % 4 == 0 && (BC == 1 || BC % 4 == 0), "x segments move in 16-byte pieces");
+    static_assert(BC == 1, "several right-hand sides: stream_spmv_kernel");
+    static_assert(XF % 4 == 0, "x segments move in 16-byte pieces");
     extern __shared__ __attribute__((aligned(16))) float smem[];
     __shared__ uint32_t s_k0[kWG + 1];  // first key of every work-item (+ sentinel)
-    __shared__ uint32_t s_hf[kWG];      // work-item saw a row end
+    __shared__ uint32_t s_hf[kWG];      // work-item saw a row end (one right-hand side)
     constexpr uint32_t kDummy = BC == 1 ? kWG : 0;               // dummy rows (one right-hand side only)
     float *xs = smem;                                            // [(W + 4) * BC]; slot W = 0 for padding
     float *acc = xs + (W + 4) * BC;                              // [(R + kDummy) * BC]: rows, then dummies
@@ -167,185 +198,412 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
     if (tid == 0) s_k0[kWG] = kNoKey;
     __syncthreads();
 
-    uint32_t left = w.w;  // units of the run not yet started
-    Unit cur = make_unit(tl, 0, w.z, left > 0);
+    // Software pipeline: the loads of the next D units are in flight while a
+    // unit is computed (D register sets, used in turn).  `issued` counts the
+    // units of the run whose loads were issued.
+    uint32_t issued = w.w > 0 ? 1u : 0u;
+    auto advance = [&](const Unit &u) {  // the unit after u in the run
+        if (!u.valid || issued >= w.w) return make_unit(tl, 0, 0, false);
+        const bool more_in_tile = (u.c + 1) * kUE < uniform_tile(tl, u.ti).y;
+        ++issued;
+        return make_unit(tl, more_in_tile ? u.ti : u.ti + 1, more_in_tile ? u.c + 1 : 0u, true);
+    };
 
-    f32x4 pv[2] = {};
-    u32x4 pq = {}, pg = {};
-    f32x4 px[XF / 4];
-    // Loads of unit u; an invalid u re-reads the start of the current tile
-    // and lanes past the unit's end re-read lane 0's entries (every address
-    // stays inside the unit, so the wait counts are exact).
-    auto prefetch = [&](const Unit &u, uint32_t fallback_ti) {
-        const uint4 t = uniform_tile(tl, u.valid ? u.ti : fallback_ti);
+    f32x4 pv[D][2] = {};
+    u32x4 pq[D] = {}, pg[D] = {};
+    f32x4 px[D][XF / 4];
+    // Loads of unit u into register set k; an invalid u re-reads the start of
+    // the run's first tile and lanes past the unit's end re-read lane 0's
+    // entries (every address stays inside a unit, so the wait counts are
+    // exact).
+    auto prefetch = [&](int k, const Unit &u) {
+        const uint4 t = uniform_tile(tl, u.valid ? u.ti : 0u);
         const uint64_t off = ((uint64_t)t.w << 32) | t.z;
         const uint32_t c = u.valid ? u.c : 0u;
         const uint32_t n8 = (min(kUE, t.y - c * kUE) + 7u) & ~7u;
         const uint8_t *base = tdata + (VB + 2) * (off + (uint64_t)c * kUE);
         const uint32_t le = (tid * kE < (u.valid ? u.n : 1u)) ? tid * kE : 0u;
         if constexpr (VT == VAL_F32) {
-            pv[0] = ldnt(reinterpret_cast<const f32x4 *>(base) + le / 8);
-            pv[1] = ldnt(reinterpret_cast<const f32x4 *>(base + 2 * n8) + le / 8);
+            pv[k][0] = ldnt(reinterpret_cast<const f32x4 *>(base) + le / 8);
+            pv[k][1] = ldnt(reinterpret_cast<const f32x4 *>(base + 2 * n8) + le / 8);
         } else {
-            pq = ldnt(reinterpret_cast<const u32x4 *>(base) + le / 8);
+            pq[k] = ldnt(reinterpret_cast<const u32x4 *>(base) + le / 8);
         }
-        pg = ldnt(reinterpret_cast<const u32x4 *>(base + VB * n8) + le / 8);
+        pg[k] = ldnt(reinterpret_cast<const u32x4 *>(base + VB * n8) + le / 8);
         // x is padded to whole segments (xlen >= nseg * W): no clamping.
         const float *xb = x + (uint64_t)t.x * W * BC;
 #pragma unroll
-        for (int k = 0; k < XF / 4; ++k) px[k] = *(reinterpret_cast<const f32x4 *>(xb) + tid + kWG * k);
+        for (int q = 0; q < XF / 4; ++q) px[k][q] = *(reinterpret_cast<const f32x4 *>(xb) + tid + kWG * q);
     };
 
-    if (cur.valid) prefetch(cur, 0);
+    Unit su[D];  // the unit whose loads are in register set k
+    Unit iss = make_unit(tl, 0, w.z, w.w > 0);  // the last unit issued
+#pragma unroll
+    for (int k = 0; k < D; ++k) {
+        if (k) iss = advance(iss);
+        su[k] = iss;
+        prefetch(k, iss);
+    }
     bool fresh = true;  // first unit of the run: its x segment is not staged yet
-    while (cur.valid) {
-        const f32x4 cv0 = pv[0], cv1 = pv[1];
-        const u32x4 cq = pq, cg = pg;
-        f32x4 cx[XF / 4];
+    // Every round runs all D stages, so the loop has one exit and the loads'
+    // wait counts stay exact; the stages past the run's end (at most D-1, in
+    // the last round) see units without entries and change nothing.
+    const uint32_t rounds = (w.w + D - 1) / D;
+    for (uint32_t rd = 0; rd < rounds; ++rd) {
 #pragma unroll
-        for (int k = 0; k < XF / 4; ++k) cx[k] = px[k];
-        --left;
-        const bool more_in_tile = (cur.c + 1) * kUE < uniform_tile(tl, cur.ti).y;
-        const Unit nxt = make_unit(tl, more_in_tile ? cur.ti : cur.ti + 1, more_in_tile ? cur.c + 1 : 0u, left > 0);
-        prefetch(nxt, cur.ti);  // in flight while this unit is computed
+        for (int k = 0; k < D; ++k) {
+            const Unit cur = su[k];
+            const f32x4 cv0 = pv[k][0], cv1 = pv[k][1];
+            const u32x4 cq = pq[k], cg = pg[k];
+            f32x4 cx[XF / 4];
+#pragma unroll
+            for (int q = 0; q < XF / 4; ++q) cx[q] = px[k][q];
+            // The next unit's loads go into set k.  One set: issued now, in
+            // flight while this unit is computed.  Several: issued once this
+            // unit's registers are used up (after the walk), in flight while
+            // the next D-1 units are computed; loading into registers that
+            // still hold live data would make the compiler copy them, and a
+            // copy waits for the loads.
+            auto issue_next = [&]() {
+                iss = advance(iss);
+                su[k] = iss;
+                prefetch(k, iss);
+            };
+            if constexpr (D == 1) issue_next();
 
-        // Keys of this work-item's entries (first key + 2-bit steps; the
-        // builder checks that every key stays below R and every column at
-        // most W).
-        const bool lane_valid = tid * kE < cur.n;  // entries [n, n8) are padding
-        // A work-item past the unit's end reads lane 0's entries: give it the
-        // key R + tid (its own dummy accumulator row, one right-hand side) and
-        // no row steps, so whatever it adds lands there; with several
-        // right-hand sides it writes nothing.
-        const uint32_t word = lane_valid ? cg.x : 0u;
-        const uint32_t key0 = lane_valid ? (cg.x & 0xffffu) : R + tid;
-        s_k0[tid] = lane_valid ? key0 : kNoKey;
-        if (cur.c == 0 || fresh) {  // a new tile: its x segment
+            const bool lane_valid = tid * kE < cur.n;  // entries [n, n8) are padding
+            // First row key of every work-item, for the row ends at its last entry.
+            const uint32_t key0 = lane_valid ? (cg.x & 0xffffu) : R + tid;
+            s_k0[tid] = lane_valid ? key0 : kNoKey;
+            if (cur.c == 0 || fresh) {  // a new tile: its x segment
 #pragma unroll
-            for (int k = 0; k < XF / 4; ++k) *(reinterpret_cast<f32x4 *>(xs) + tid + kWG * k) = cx[k];
-        }
-        fresh = false;
-        __syncthreads();
-        uint32_t step[kE];  // step[j]: key of entry j minus key of entry j-1
-#pragma unroll
-        for (int j = 1; j < 8; ++j) step[j] = (word >> (14 + 2 * j)) & 3u;
-        uint32_t key7 = key0;
-#pragma unroll
-        for (int j = 1; j < 8; ++j) key7 += step[j];
-        // Row ends: entry j < 7 ends its row where step[j+1] != 0, entry 7
-        // where the next work-item starts another row.  Every row that ends
-        // here gets its partial added to the accumulator (this work-item is
-        // the only writer of those rows in this unit); the other entries
-        // point at the dummy row R + tid.  All reads come before any write.
-        // One right-hand side: every entry reads and writes a slot (non-ends
-        // their work-item's dummy row), no branches.  Several: only the row
-        // ends touch the accumulator (a masked read and write each), which
-        // saves most of the LDS traffic of BC-wide slots.
-        const bool end7 = key7 != s_k0[tid + 1];
-        constexpr bool kMasked = BC > 1;
-        bool endm[kE];
-        uint32_t slot[kE];
-        {
-            uint32_t k = key0;
-#pragma unroll
-            for (int m = 0; m < (int)kE; ++m) {
-                endm[m] = m < 7 ? step[m + 1] != 0u : end7;
-                if (kMasked) endm[m] = endm[m] && lane_valid;
-                slot[m] = (kMasked || endm[m] ? k : R + tid) * BC;
-                if (m < 7) k += step[m + 1];
+                for (int q = 0; q < XF / 4; ++q) *(reinterpret_cast<f32x4 *>(xs) + tid + kWG * q) = cx[q];
             }
-        }
-        float old[kE][BC];
-#pragma unroll
-        for (int m = 0; m < (int)kE; ++m) {
+            fresh = false;
+            __syncthreads();
             if constexpr (BC == 1) {
-                old[m][0] = acc[slot[m]];
-            } else {
+                float a[1];  // the part of the last row that continues in the next work-item
+                // Row steps (the builder checks that every key stays below R and
+                // every column at most W).  A work-item past the unit's end reads
+                // lane 0's entries: it has the key R + tid (its own dummy
+                // accumulator row) and no row steps, so whatever it adds lands
+                // there.
+                const uint32_t word = lane_valid ? cg.x : 0u;
+                uint32_t step[kE];  // step[j]: key of entry j minus key of entry j-1
 #pragma unroll
-                for (int b = 0; b < BC; ++b) old[m][b] = 0.0f;
-                if (endm[m]) {
+                for (int j = 1; j < 8; ++j) step[j] = (word >> (14 + 2 * j)) & 3u;
+                uint32_t key7 = key0;
 #pragma unroll
-                    for (int q = 0; q < BC / 4; ++q) {
-                        const f32x4 o = *reinterpret_cast<const f32x4 *>(acc + slot[m] + 4 * q);
+                for (int j = 1; j < 8; ++j) key7 += step[j];
+                // Row ends: entry j < 7 ends its row where step[j+1] != 0, entry 7
+                // where the next work-item starts another row.  Every row that
+                // ends here gets its partial added to the accumulator (this
+                // work-item is the only writer of those rows in this unit); the
+                // other entries read and write the dummy row R + tid, so there
+                // are no branches.  All reads come before any write.
+                const bool end7 = key7 != s_k0[tid + 1];
+                uint32_t slot[kE];
+                {
+                    uint32_t k = key0;
 #pragma unroll
-                        for (int b = 0; b < 4; ++b) old[m][4 * q + b] = o[b];
+                    for (int m = 0; m < (int)kE; ++m) {
+                        const bool e = m < 7 ? step[m + 1] != 0u : end7;
+                        slot[m] = e ? k : R + tid;
+                        if (m < 7) k += step[m + 1];
                     }
                 }
-            }
-        }
-        // Products (padding reads column W, whose slot holds 0), the running
-        // row sums (restarted after each row end) added into the slots.
-        float a[BC];
+                float old[kE];
 #pragma unroll
-        for (int b = 0; b < BC; ++b) a[b] = 0.0f;
+                for (int m = 0; m < (int)kE; ++m) old[m] = acc[slot[m]];
+                // Products (padding reads column W, whose slot holds 0), the
+                // running row sum (restarted after each row end) added into the
+                // slots.
+                a[0] = 0.0f;
 #pragma unroll
-        for (int m = 0; m < (int)kE; ++m) {
-            const float v = group_val<VT>(cv0, cv1, cq, m);
-            const float *xg = xs + group_col<CB>(cg, m) * BC;
-            float xv[BC];
-            if constexpr (BC == 1) {
-                xv[0] = xg[0];
-            } else {
-#pragma unroll
-                for (int q = 0; q < BC / 4; ++q) {
-                    const f32x4 t4 = *reinterpret_cast<const f32x4 *>(xg + 4 * q);
-#pragma unroll
-                    for (int b = 0; b < 4; ++b) xv[4 * q + b] = t4[b];
+                for (int m = 0; m < (int)kE; ++m) {
+                    const float v = group_val<VT>(cv0, cv1, cq, m);
+                    const bool end = m < 7 ? step[m + 1] != 0u : end7;
+                    a[0] += v * xs[group_col<CB>(cg, m)];
+                    old[m] += a[0];
+                    a[0] = end ? 0.0f : a[0];
                 }
-            }
-            const bool end = m < 7 ? step[m + 1] != 0u : end7;
 #pragma unroll
-            for (int b = 0; b < BC; ++b) {
-                a[b] += v * xv[b];
-                old[m][b] += a[b];
-                a[b] = end ? 0.0f : a[b];
-            }
-        }
+                for (int m = 0; m < (int)kE; ++m) acc[slot[m]] = old[m];
+                uint32_t ends = end7 ? 1u : 0u;
 #pragma unroll
-        for (int m = 0; m < (int)kE; ++m) {
-            if constexpr (BC == 1) {
-                acc[slot[m]] = old[m][0];
-            } else {
-                if (endm[m]) {
-#pragma unroll
-                    for (int q = 0; q < BC / 4; ++q) {
-                        f32x4 o;
-#pragma unroll
-                        for (int b = 0; b < 4; ++b) o[b] = old[m][4 * q + b];
-                        *reinterpret_cast<f32x4 *>(acc + slot[m] + 4 * q) = o;
+                for (int j = 1; j < 8; ++j) ends |= step[j];
+                if constexpr (D > 1) issue_next();
+                // the part of the last row that continues in the next work-item
+                s_cf[tid] = a[0];
+                s_hf[tid] = ends;
+                __syncthreads();
+                // The row of entry 0 may have started in earlier work-items:
+                // when it ends here, add their carries (nearest first, back to
+                // the last one with a row end, whose carry belongs to this row
+                // too).
+                if (ends) {
+                    float cin = 0.0f;
+                    for (int jj = (int)tid - 1; jj >= 0; --jj) {
+                        cin += s_cf[jj];
+                        if (s_hf[jj]) break;
                     }
+                    acc[key0] += cin;
                 }
             }
         }
-        uint32_t ends = end7 ? 1u : 0u;
-#pragma unroll
-        for (int j = 1; j < 8; ++j) ends |= step[j];
-        if (kMasked && !lane_valid) ends = 0;
-        // the part of the last row that continues in the next work-item
-#pragma unroll
-        for (int b = 0; b < BC; ++b) s_cf[tid * BC + b] = a[b];
-        s_hf[tid] = ends;
-        __syncthreads();
-        // The row of entry 0 may have started in earlier work-items: when it
-        // ends here, add their carries (nearest first, back to the last one
-        // with a row end, whose carry belongs to this row too).
-        if (ends) {
-            float cin[BC];
-#pragma unroll
-            for (int b = 0; b < BC; ++b) cin[b] = 0.0f;
-            for (int jj = (int)tid - 1; jj >= 0; --jj) {
-#pragma unroll
-                for (int b = 0; b < BC; ++b) cin[b] += s_cf[jj * BC + b];
-                if (s_hf[jj]) break;
-            }
-#pragma unroll
-            for (int b = 0; b < BC; ++b) acc[key0 * BC + b] += cin[b];
-        }
-        cur = nxt;
     }
     __syncthreads();
     float *pw = part + (uint64_t)blockIdx.x * R * BC;
     for (uint32_t i = tid; i < R * BC; i += kWG) pw[i] = acc[i];
+}
+
+// Several right-hand sides: quad groups in block-stream order (the groups of
+// a row block's tiles one after another, values then group words).  A window
+// is kSNT consecutive groups of the stream, one per work-item, from at most
+// two tiles (A, then B from group `split` on); the x segments of the tiles
+// sit in two LDS slots by the parity of the tile's place in the block's tile
+// list, so a window does not stop at a tile boundary.
+// wl[w] = {row block, first window, windows}; win[i] = {first group,
+// groups | split << 16 | slot of A << 31, segment of A, segment of B}.
+
+// Lane i of the wave gets v from lane i - (shift given by the DPP control
+// CTRL) of the rows in ROWM, 0 where there is none.
+template <int CTRL, int ROWM>
+__device__ __forceinline__ uint32_t dpp_get(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWM, 0xf, false);
+}
+
+// One step of a segmented inclusive scan across the wave: (v, f) combined
+// with the running value of an earlier lane, where f marks that a segment
+// starts at or before this lane (the earlier part then does not count).
+template <int BC, int CTRL, int ROWM>
+__device__ __forceinline__ void seg_scan_step(float (&v)[BC], uint32_t &f) {
+    const uint32_t nf = dpp_get<CTRL, ROWM>(f);
+#pragma unroll
+    for (int b = 0; b < BC; ++b) {
+        const float nv = __uint_as_float(dpp_get<CTRL, ROWM>(__float_as_uint(v[b])));
+        v[b] = f ? v[b] : nv + v[b];
+    }
+    f |= nf;
+}
+
+template <int W, int VT, int BC>
+__global__ __launch_bounds__(kSNT) void stream_spmv_kernel(uint32_t R, uint32_t nseg,
+                                                           const uint64_t *__restrict__ tile_off,
+                                                           const uint4 *__restrict__ wl,
+                                                           const uint4 *__restrict__ win,
+                                                           const uint8_t *__restrict__ tdata,
+                                                           const float *__restrict__ x, float *__restrict__ part) {
+    constexpr int VB = VT == VAL_F32 ? 4 : 2;  // bytes per stored value
+    constexpr int XF = W * BC / kSNT;          // floats of an x segment per work-item
+    constexpr int CB = col_bits<W>();
+    constexpr uint32_t XS = (W + 4) * BC;      // floats per x slot (column W holds 0)
+    constexpr int NW = kSNT / 64;              // waves
+    static_assert(BC % 4 == 0 && XF % 4 == 0, "x segments move in 16-byte pieces");
+    extern __shared__ __attribute__((aligned(16))) float smem[];
+    __shared__ uint32_t s_k0[kSNT + 1];                         // first key of every work-item
+    __shared__ __attribute__((aligned(16))) float s_wt[NW][BC];  // carry out of every wave
+    __shared__ __attribute__((aligned(16))) uint32_t s_wh[NW];  // the wave ends a row
+    __shared__ uint4 s_win[kRunWindows];                        // the run's windows
+    float *xs = smem;                                           // [2][XS]
+    float *acc = xs + 2 * XS;                                   // [R * BC]
+
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
+    const uint4 w = wl[blockIdx.x];
+    const uint64_t e0 = tile_off[(uint64_t)w.x * nseg], e1 = tile_off[(uint64_t)(w.x + 1) * nseg];
+    const uint64_t G = (e1 - e0) / 8;  // groups of the row block
+    const uint8_t *vbase = tdata + (uint64_t)(VB + 2) * e0;
+    const uint8_t *gbase = vbase + (uint64_t)VB * 8 * G;
+    for (uint32_t i = tid; i < R * BC; i += kSNT) acc[i] = 0.0f;
+    for (uint32_t i = tid; i < w.z; i += kSNT) s_win[i] = win[w.y + i];
+    if (tid < BC) {
+        xs[W * BC + tid] = 0.0f;
+        xs[XS + W * BC + tid] = 0.0f;
+    }
+    if (tid == 0) s_k0[kSNT] = kNoKey;
+
+    uint32_t staged[2] = {~0u, ~0u};  // segment held by each x slot
+    // Loads of window c (its groups; the x segments of its tiles that are
+    // not staged yet: needA / needB).  Work-items past the window's end
+    // re-read its first group.
+    f32x4 pv0 = {}, pv1 = {};
+    u32x4 pq = {}, pg = {};
+    f32x4 pxa[XF / 4], pxb[XF / 4];
+    auto load = [&](const uint4 &c, bool needA, bool needB) {
+        const uint32_t ng = c.y & 0x3ffu;
+        const uint64_t g = c.x + (tid < ng ? tid : 0u);
+        if constexpr (VT == VAL_F32) {
+            pv0 = ldnt(reinterpret_cast<const f32x4 *>(vbase) + g);
+            pv1 = ldnt(reinterpret_cast<const f32x4 *>(vbase + 16 * G) + g);
+        } else {
+            pq = ldnt(reinterpret_cast<const u32x4 *>(vbase) + g);
+        }
+        pg = ldnt(reinterpret_cast<const u32x4 *>(gbase) + g);
+        // x is padded to whole segments (xlen >= nseg * W): no clamping.
+        if (needA) {
+            const f32x4 *xa = reinterpret_cast<const f32x4 *>(x + (uint64_t)c.z * W * BC);
+#pragma unroll
+            for (int q = 0; q < XF / 4; ++q) pxa[q] = xa[tid + kSNT * q];
+        }
+        if (needB) {
+            const f32x4 *xb = reinterpret_cast<const f32x4 *>(x + (uint64_t)c.w * W * BC);
+#pragma unroll
+            for (int q = 0; q < XF / 4; ++q) pxb[q] = xb[tid + kSNT * q];
+        }
+    };
+    auto uniform4 = [](const uint4 &t) {
+        return make_uint4(__builtin_amdgcn_readfirstlane(t.x), __builtin_amdgcn_readfirstlane(t.y),
+                          __builtin_amdgcn_readfirstlane(t.z), __builtin_amdgcn_readfirstlane(t.w));
+    };
+    uint4 cw = uniform4(win[w.y]);
+    bool needA = true, needB = ((cw.y >> 16) & 0x3ffu) < (cw.y & 0x3ffu);
+    load(cw, needA, needB);
+    __syncthreads();
+
+    for (uint32_t i = 0; i < w.z; ++i) {
+        const uint32_t ng = cw.y & 0x3ffu, split = (cw.y >> 16) & 0x3ffu, slotA = cw.y >> 31;
+        const f32x4 cv0 = pv0, cv1 = pv1;
+        const u32x4 cq = pq, cg = pg;
+        const uint4 nw = uniform4(s_win[i + 1 < w.z ? i + 1 : i]);
+        // Stage the new x segments (the slots' previous tiles are done: the
+        // last barrier of the previous window lies between).
+        if (needA) {
+#pragma unroll
+            for (int q = 0; q < XF / 4; ++q) *(reinterpret_cast<f32x4 *>(xs + slotA * XS) + tid + kSNT * q) = pxa[q];
+            staged[slotA] = cw.z;
+        }
+        if (needB) {
+#pragma unroll
+            for (int q = 0; q < XF / 4; ++q)
+                *(reinterpret_cast<f32x4 *>(xs + (slotA ^ 1u) * XS) + tid + kSNT * q) = pxb[q];
+            staged[slotA ^ 1u] = cw.w;
+        }
+        const bool lane_valid = tid < ng;
+        const uint32_t kA = cg.x & 0xffffu, kB = cg.x >> 16;
+        s_k0[tid] = lane_valid ? kA : kNoKey;
+        // The next window's loads, in flight while this one is computed.
+        if (i + 1 < w.z) {
+            const uint32_t sa = nw.y >> 31;
+            needA = staged[sa] != nw.z;
+            needB = ((nw.y >> 16) & 0x3ffu) < (nw.y & 0x3ffu) && staged[sa ^ 1u] != nw.w;
+            load(nw, needA, needB);
+            cw = nw;
+        }
+        __syncthreads();
+        const float *xw = xs + (tid < split ? slotA : slotA ^ 1u) * XS;
+        // Quads: entries 0-3 belong to row kA, 4-7 to row kB >= kA.  Each
+        // quad is summed on its own (padding reads column W, whose slot
+        // holds 0); work-items past the window's end skip the sums.  A row
+        // ends after quad A where kB != kA and after quad B where the next
+        // work-item starts another row.
+        float sq[2][BC];
+#pragma unroll
+        for (int h = 0; h < 2; ++h)
+#pragma unroll
+            for (int c = 0; c < BC; ++c) sq[h][c] = 0.0f;
+        if (lane_valid) {
+#pragma unroll
+            for (int h = 0; h < 2; ++h) {
+#pragma unroll
+                for (int j = 0; j < 4; ++j) {
+                    const int m = 4 * h + j;
+                    const float v = group_val<VT>(cv0, cv1, cq, m);
+                    const float *xg = xw + group_col<CB>(cg, m) * BC;
+#pragma unroll
+                    for (int q = 0; q < BC / 4; ++q) {
+                        const f32x4 t4 = *reinterpret_cast<const f32x4 *>(xg + 4 * q);
+#pragma unroll
+                        for (int bb = 0; bb < 4; ++bb)
+                            sq[h][4 * q + bb] = j == 0 ? v * t4[bb] : fmaf(v, t4[bb], sq[h][4 * q + bb]);
+                    }
+                }
+            }
+        }
+        const bool endA = lane_valid && kA != kB;
+        const bool endB = lane_valid && kB != s_k0[tid + 1];
+        const bool ends = endA || endB;
+        // Carry out: the part of the last row that continues in the next
+        // work-item.  The carry in of row kA is the sum of the carries of
+        // the work-items since the last one that ends a row (that one's
+        // included), or since the window's start: a segmented scan over the
+        // carries shifted by one work-item, in registers across the wave,
+        // then across the waves from their totals in LDS.
+        float a[BC], s[BC];
+        uint32_t f;
+#pragma unroll
+        for (int b = 0; b < BC; ++b) a[b] = endB ? 0.0f : (endA ? sq[1][b] : sq[0][b] + sq[1][b]);
+#pragma unroll
+        for (int b = 0; b < BC; ++b) s[b] = __uint_as_float(dpp_get<0x138, 0xf>(__float_as_uint(a[b])));  // wave_shr:1
+        f = dpp_get<0x138, 0xf>(ends ? 1u : 0u);
+        seg_scan_step<BC, 0x111, 0xf>(s, f);  // row_shr:1
+        seg_scan_step<BC, 0x112, 0xf>(s, f);  // row_shr:2
+        seg_scan_step<BC, 0x114, 0xf>(s, f);  // row_shr:4
+        seg_scan_step<BC, 0x118, 0xf>(s, f);  // row_shr:8
+        seg_scan_step<BC, 0x142, 0xa>(s, f);  // row_bcast:15
+        seg_scan_step<BC, 0x143, 0xc>(s, f);  // row_bcast:31
+        if (lane == 63) {  // the wave's carry out and whether it ends a row
+#pragma unroll
+            for (int b = 0; b < BC; ++b) s_wt[wv][b] = ends ? a[b] : s[b] + a[b];
+            s_wh[wv] = (f || ends) ? 1u : 0u;
+        }
+        __syncthreads();
+        // Every row that ends here is added to the accumulator once: kA
+        // with the carry in, kB (when a second row) on its own.  A row can
+        // end twice in a window that spans two tiles (once in each): the
+        // work-items of tile A update the accumulator first, a barrier, then
+        // those of tile B.
+        auto flush = [&]() {
+            const uint32_t r0 = (endA ? kA : kB) * BC, r1 = kB * BC;
+            f32x4 o0[BC / 4], o1[BC / 4];
+#pragma unroll
+            for (int q = 0; q < BC / 4; ++q) {
+                o0[q] = *reinterpret_cast<const f32x4 *>(acc + r0 + 4 * q);
+                o1[q] = *reinterpret_cast<const f32x4 *>(acc + r1 + 4 * q);
+            }
+            float cin[BC];
+#pragma unroll
+            for (int b = 0; b < BC; ++b) cin[b] = 0.0f;
+            if (!f) {  // the row started before this wave: the earlier waves' carries
+                const u32x4 h0 = *reinterpret_cast<const u32x4 *>(s_wh);
+                const u32x4 h1 = NW > 4 ? *reinterpret_cast<const u32x4 *>(s_wh + 4) : u32x4{};
+#pragma unroll
+                for (int u = 0; u < NW - 1; ++u) {
+                    if (u < (int)wv) {
+                        const uint32_t hu = u < 4 ? h0[u] : h1[u - 4];
+#pragma unroll
+                        for (int q = 0; q < BC / 4; ++q) {
+                            const f32x4 t4 = *reinterpret_cast<const f32x4 *>(&s_wt[u][4 * q]);
+#pragma unroll
+                            for (int bb = 0; bb < 4; ++bb) cin[4 * q + bb] = hu ? t4[bb] : cin[4 * q + bb] + t4[bb];
+                        }
+                    }
+                }
+            }
+#pragma unroll
+            for (int q = 0; q < BC / 4; ++q) {
+#pragma unroll
+                for (int bb = 0; bb < 4; ++bb) {
+                    const int c = 4 * q + bb;
+                    const float ci = f ? s[c] : cin[c] + s[c];
+                    o0[q][bb] += endA ? ci + sq[0][c] : (ci + sq[0][c]) + sq[1][c];
+                    o1[q][bb] += sq[1][c];
+                }
+                *reinterpret_cast<f32x4 *>(acc + r0 + 4 * q) = o0[q];
+            }
+            if (endA && endB) {
+#pragma unroll
+                for (int q = 0; q < BC / 4; ++q) *reinterpret_cast<f32x4 *>(acc + r1 + 4 * q) = o1[q];
+            }
+        };
+        if (ends && tid < split) flush();
+        if (split < ng) {
+            __syncthreads();
+            if (ends && tid >= split) flush();
+        }
+    }
+    __syncthreads();
+    float *pw = part + (uint64_t)blockIdx.x * R * BC;
+    for (uint32_t i = tid; i < R * BC; i += kSNT) pw[i] = acc[i];
 }
 
 // Row r of block b = r / R sums the partials of the block's workgroups
@@ -409,12 +667,13 @@ __global__ __launch_bounds__(kWG) void tile_count_kernel(const uint64_t *__restr
 
 // Per tile (one work-item per tile): rowpos[r*nseg+s] = position of row r's
 // run in tile (b, s); tile_nz = entries including step padding, padded to 8
-// in tile_padded.  A row whose key exceeds the previous non-empty row's by
-// more than kMaxRowStep starts a new group of 8 (the rest of the group is
-// padding), so every key inside a group is its predecessor plus 0..3.
+// in tile_padded.  Row steps: a row whose key exceeds the previous non-empty
+// row's by more than kMaxRowStep starts a new group of 8 (the rest of the
+// group is padding), so every key inside a group is its predecessor plus
+// 0..3.  Quads: every non-empty row starts at a multiple of 4.
 __global__ void tile_rowpos_kernel(const uint32_t *__restrict__ cnt, uint32_t rows, uint32_t R, uint32_t nseg,
-                                   uint64_t ntiles, uint32_t *__restrict__ rowpos, uint32_t *__restrict__ tile_nz,
-                                   uint64_t *__restrict__ tile_padded) {
+                                   uint64_t ntiles, bool quad, uint32_t *__restrict__ rowpos,
+                                   uint32_t *__restrict__ tile_nz, uint64_t *__restrict__ tile_padded) {
     const uint64_t t = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x;
     if (t >= ntiles) return;
     const uint32_t b = (uint32_t)(t / nseg), s = (uint32_t)(t % nseg);
@@ -425,7 +684,10 @@ __global__ void tile_rowpos_kernel(const uint32_t *__restrict__ cnt, uint32_t ro
         if (r >= rows) break;
         const uint32_t n = cnt[r * nseg + s];
         if (n) {
-            if (any && i - last > kMaxRowStep) run = (run + 7u) & ~7u;
+            if (quad)
+                run = (run + 3u) & ~3u;
+            else if (any && i - last > kMaxRowStep)
+                run = (run + 7u) & ~7u;
             any = true;
             last = i;
         }
@@ -508,7 +770,7 @@ __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__res
                                                            const uint32_t *__restrict__ segoff,
                                                            const uint32_t *__restrict__ rowpos,
                                                            const uint64_t *__restrict__ tile_off, uint32_t vt,
-                                                           const uint16_t *__restrict__ qval,
+                                                           bool stream, const uint16_t *__restrict__ qval,
                                                            uint8_t *__restrict__ tdata, uint16_t *__restrict__ ctmp,
                                                            uint16_t *__restrict__ rtmp) {
     const uint32_t row = blockIdx.x * (kWG / 64) + threadIdx.x / 64;
@@ -523,14 +785,25 @@ __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__res
         const uint64_t rsn = (uint64_t)row * nseg + s;
         const uint64_t off = tile_off[t];
         const uint64_t p = rowpos[rsn] + (k - rs - segoff[rsn]);  // position in the tile
-        const uint64_t u = p / kUE, q = p % kUE;
-        if (vt == VAL_F32) {
-            const uint64_t n8u = std::min<uint64_t>(kUE, tile_off[t + 1] - off - u * kUE);
-            // entry q of the unit = entry j of work-item q/8: half j/4 of the values
-            const uint64_t vq = (q % 8 / 4) * (n8u / 2) + q / 8 * 4 + q % 4;
-            reinterpret_cast<float *>(tdata + 6 * (off + u * kUE))[vq] = val[k];
+        // Entry q of a stretch of n8u entries stored from entry offset `base`:
+        // a unit of a tile, or (stream) the whole row block.
+        uint64_t base, q, n8u;
+        if (stream) {
+            base = tile_off[(uint64_t)b * nseg];
+            q = off - base + p;
+            n8u = tile_off[(uint64_t)(b + 1) * nseg] - base;
         } else {
-            reinterpret_cast<uint16_t *>(tdata + 4 * (off + u * kUE))[q] = qval[k];
+            const uint64_t u = p / kUE;
+            base = off + u * kUE;
+            q = p % kUE;
+            n8u = std::min<uint64_t>(kUE, tile_off[t + 1] - base);
+        }
+        if (vt == VAL_F32) {
+            // entry q = entry j of group q/8: half j/4 of the values
+            const uint64_t vq = (q % 8 / 4) * (n8u / 2) + q / 8 * 4 + q % 4;
+            reinterpret_cast<float *>(tdata + 6 * base)[vq] = val[k];
+        } else {
+            reinterpret_cast<uint16_t *>(tdata + 4 * base)[q] = qval[k];
         }
         ctmp[off + p] = (uint16_t)(c & ((1u << wshift) - 1u));
         rtmp[off + p] = (uint16_t)i;
@@ -540,13 +813,15 @@ __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__res
 // One work-item per group of 8 entries (entry offset 8*q): finds its tile,
 // packs the group word (layout at the top of this file) into its unit.
 // Padding (rtmp = kPadRow) repeats the previous key with column W.  bit 4
-// of *bad: a step above kMaxRowStep or a key not below R (the layout
-// guarantees neither; the SpMV kernel relies on both).
-template <int W>
+// of *bad: a step above kMaxRowStep, a key not below R or (quads) a quad
+// holding two rows or starting with padding (the layout rules all of them
+// out; the SpMV kernels rely on it).
+template <int W, bool Q>
 __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restrict__ ctmp,
                                                         const uint16_t *__restrict__ rtmp,
                                                         const uint64_t *__restrict__ tile_off, uint64_t ntiles,
-                                                        uint32_t R, uint32_t vb, uint8_t *__restrict__ tdata,
+                                                        uint64_t nseg, uint32_t R, uint32_t vb,
+                                                        uint8_t *__restrict__ tdata,
                                                         uint32_t *__restrict__ bad) {
     constexpr int CB = col_bits<W>();
     const uint64_t q = blockIdx.x * (uint64_t)kWG + threadIdx.x;
@@ -562,24 +837,43 @@ __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restri
         else
             hi = mid;
     }
-    const uint64_t off = tile_off[lo], n8 = tile_off[lo + 1] - off;
-    const uint64_t p = L - off, u = p / kUE, e = p % kUE;
-    const uint64_t n8u = std::min<uint64_t>(kUE, n8 - u * kUE);
+    uint64_t base, e, n8u;  // as in tile_scatter_kernel
+    if constexpr (Q) {
+        const uint64_t b = lo / nseg;
+        base = tile_off[b * nseg];
+        e = L - base;
+        n8u = tile_off[(b + 1) * nseg] - base;
+    } else {
+        const uint64_t off = tile_off[lo], p = L - off;
+        base = off + p / kUE * kUE;
+        e = p % kUE;
+        n8u = std::min<uint64_t>(kUE, tile_off[lo + 1] - base);
+    }
 
     const u16x8 r = *reinterpret_cast<const u16x8 *>(rtmp + L);
     const u16x8 c = *reinterpret_cast<const u16x8 *>(ctmp + L);
-    uint32_t prev = r[0] == kPadRow ? 0u : r[0];
-    uint32_t w0 = prev, big = prev >= R ? 1u : 0u;
+    uint32_t w0, big;
     uint32_t cols[8];
-    cols[0] = r[0] == kPadRow ? (uint32_t)W : c[0];
 #pragma unroll
-    for (int j = 1; j < 8; ++j) {
-        const uint32_t v = r[j];
-        const uint32_t d = v == kPadRow ? 0u : v - prev;
-        prev = v == kPadRow ? prev : v;
-        big |= (d > kMaxRowStep || prev >= R) ? 1u : 0u;
-        w0 |= (d & 3u) << (14 + 2 * j);
-        cols[j] = v == kPadRow ? (uint32_t)W : c[j];
+    for (int j = 0; j < 8; ++j) cols[j] = r[j] == kPadRow ? (uint32_t)W : c[j];
+    if constexpr (Q) {
+        const uint32_t kA = r[0], kB = r[4] == kPadRow ? kA : r[4];
+        big = (kA == kPadRow || kA >= R || kB >= R || kB < kA) ? 1u : 0u;
+#pragma unroll
+        for (int j = 1; j < 8; ++j) big |= (r[j] != kPadRow && r[j] != (j < 4 ? kA : kB)) ? 1u : 0u;
+        w0 = kA | (kB << 16);
+    } else {
+        uint32_t prev = r[0] == kPadRow ? 0u : r[0];
+        w0 = prev;
+        big = prev >= R ? 1u : 0u;
+#pragma unroll
+        for (int j = 1; j < 8; ++j) {
+            const uint32_t v = r[j];
+            const uint32_t d = v == kPadRow ? 0u : v - prev;
+            prev = v == kPadRow ? prev : v;
+            big |= (d > kMaxRowStep || prev >= R) ? 1u : 0u;
+            w0 |= (d & 3u) << (14 + 2 * j);
+        }
     }
     uint64_t lo64 = 0;
     uint32_t w3 = 0;
@@ -595,7 +889,7 @@ __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restri
     word.y = (uint32_t)lo64;
     word.z = (uint32_t)(lo64 >> 32);
     word.w = w3;
-    *reinterpret_cast<u32x4 *>(tdata + (vb + 2) * (off + u * kUE) + vb * n8u + 2 * e) = word;
+    *reinterpret_cast<u32x4 *>(tdata + (vb + 2) * base + vb * n8u + 2 * e) = word;
     if (big) atomicOr(bad, 4u);
 }
 
@@ -611,22 +905,37 @@ size_t lds_bytes(uint32_t W, uint32_t R, uint32_t span, uint32_t BC) {
     return ((size_t)W + 4 + R + (BC == 1 ? kWG : 0) + kWG) * BC * 4 + (size_t)span * 16;
 }
 
+// stream_spmv_kernel: two x slots, the accumulator, the carries.
+size_t stream_lds_bytes(uint32_t W, uint32_t R, uint32_t BC) {
+    return (2 * ((size_t)W + 4) + R) * BC * 4;
+}
+
 template <int W, int VT, int BC>
 hipError_t launch_k(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
                     hipStream_t s) {
-    hipLaunchKernelGGL((blocked_spmv_kernel<W, VT, BC>), dim3(wk.nwg), dim3(kWG), lds_bytes(W, p.R, wk.span, BC), s,
+    hipLaunchKernelGGL((blocked_spmv_kernel<W, VT, BC, pipeline_depth<W, VT, BC>()>), dim3(wk.nwg), dim3(kWG),
+                       lds_bytes(W, p.R, wk.span, BC), s,
                        p.R, wk.span, wk.ntl, wk.wl, wk.tiles, tm.tdata, x, part);
+    return hipGetLastError();
+}
+
+template <int W, int VT, int BC>
+hipError_t launch_stream(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
+                         hipStream_t s) {
+    hipLaunchKernelGGL((stream_spmv_kernel<W, VT, BC>), dim3(wk.nwg), dim3(kSNT), stream_lds_bytes(W, p.R, BC), s, p.R,
+                       p.nseg, tm.tile_off, wk.wl, wk.tiles, tm.tdata, x, part);
     return hipGetLastError();
 }
 
 template <int VT>
 hipError_t launch_vt(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
                      uint32_t BC, hipStream_t s) {
+    if (p.quad != (BC > 1)) return hipErrorInvalidValue;  // quads (block streams) exactly for several
     if (BC == 1 && p.W == 1024) return launch_k<1024, VT, 1>(p, tm, wk, x, part, s);
     if (BC == 1 && p.W == 2048) return launch_k<2048, VT, 1>(p, tm, wk, x, part, s);
-    if (BC == 4 && p.W == 512) return launch_k<512, VT, 4>(p, tm, wk, x, part, s);
-    if (BC == 4 && p.W == 1024) return launch_k<1024, VT, 4>(p, tm, wk, x, part, s);
-    if (BC == 8 && p.W == 512) return launch_k<512, VT, 8>(p, tm, wk, x, part, s);
+    if (BC == 4 && p.W == 512) return launch_stream<512, VT, 4>(p, tm, wk, x, part, s);
+    if (BC == 4 && p.W == 1024) return launch_stream<1024, VT, 4>(p, tm, wk, x, part, s);
+    if (BC == 8 && p.W == 512) return launch_stream<512, VT, 8>(p, tm, wk, x, part, s);
     return hipErrorInvalidValue;
 }
 
@@ -670,9 +979,10 @@ CountScratch count_scratch(const TilePlan &p, void *scratch) {
 
 }  // namespace
 
-TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t vt) {
+TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t vt, bool quad) {
     TilePlan p;
     p.vt = vt;
+    p.quad = quad;
     p.rows = rows;
     p.cols = cols;
     p.nnz = nnz;
@@ -690,7 +1000,9 @@ bool valid_tile_config(uint32_t W, uint32_t BC) {
 }
 bool valid_tile_rows(uint32_t R) { return R == 512 || R == 1024 || R == 2048 || R == 4096 || R == 8192 || R == 16384; }
 
-size_t tiled_lds_bytes(const TilePlan &p, uint32_t span, uint32_t BC) { return lds_bytes(p.W, p.R, span, BC); }
+size_t tiled_lds_bytes(const TilePlan &p, uint32_t span, uint32_t BC) {
+    return p.quad ? stream_lds_bytes(p.W, p.R, BC) : lds_bytes(p.W, p.R, span, BC);
+}
 
 size_t tiled_matrix_bytes(const TilePlan &p) {
     return align256(8 * (p.ntiles + 1)) + align256(4 * p.ntiles) + align256((p.vb() + 2) * p.cap) +
@@ -701,7 +1013,67 @@ size_t tiled_part_bytes(const TilePlan &p, uint32_t nwg, uint32_t BC) {
     return sizeof(float) * (size_t)nwg * p.R * BC;
 }
 
+namespace {
+// Block streams (quad tiles): windows of up to kSNT groups from at most two
+// consecutive non-empty tiles of a row block, then runs of consecutive
+// windows of nearly equal group counts, none leaving its row block.
+TileWorkList stream_work_list(const TilePlan &p, const uint64_t *off, uint32_t target_wgs) {
+    TileWorkList L;
+    const uint64_t total = off[p.ntiles] / 8;
+    const uint64_t per_run = std::max<uint64_t>(1, (total + std::max(1u, target_wgs) - 1) / std::max(1u, target_wgs));
+    L.wl_first.assign((size_t)p.nblk + 1, 0);
+    std::vector<uint64_t> gs;    // first group of each non-empty tile of the block, then the block's groups
+    std::vector<uint32_t> segs;  // its segment
+    std::vector<uint64_t> wg;    // groups of each window of the block
+    for (uint32_t b = 0; b < p.nblk; ++b) {
+        L.wl_first[b] = L.nwg;
+        const uint64_t base = off[(uint64_t)b * p.nseg];
+        gs.clear();
+        segs.clear();
+        for (uint32_t sg = 0; sg < p.nseg; ++sg) {
+            const uint64_t t = (uint64_t)b * p.nseg + sg;
+            if (off[t + 1] == off[t]) continue;
+            gs.push_back((off[t] - base) / 8);
+            segs.push_back(sg);
+        }
+        const uint64_t G = (off[(uint64_t)(b + 1) * p.nseg] - base) / 8;
+        const size_t nt = segs.size();
+        gs.push_back(G);
+        const uint32_t first_win = L.ntl;
+        wg.clear();
+        size_t ta = 0;
+        for (uint64_t pos = 0; pos < G;) {
+            while (gs[ta + 1] <= pos) ++ta;  // the tile holding group pos
+            const uint64_t end = std::min<uint64_t>(pos + kSNT, gs[std::min(ta + 2, nt)]);
+            const uint64_t split = std::min(gs[ta + 1], end) - pos;
+            const uint32_t sb = ta + 1 < nt ? segs[ta + 1] : segs[ta];
+            L.tiles.insert(L.tiles.end(), {(uint32_t)pos, (uint32_t)((end - pos) | (split << 16) | ((ta & 1) << 31)),
+                                           segs[ta], sb});
+            ++L.ntl;
+            wg.push_back(end - pos);
+            pos = end;
+        }
+        // Runs of nearly equal group counts over the block's windows.
+        uint64_t left = G, runs = (G + per_run - 1) / per_run;
+        size_t wi = 0;
+        while (wi < wg.size()) {
+            const uint64_t want = (left + runs - 1) / runs;
+            const size_t w0 = wi;
+            uint64_t got = 0;
+            while (wi < wg.size() && wi - w0 < kRunWindows && (got == 0 || got + wg[wi] / 2 <= want)) got += wg[wi++];
+            L.wl.insert(L.wl.end(), {b, first_win + (uint32_t)w0, (uint32_t)(wi - w0), 0u});
+            ++L.nwg;
+            left -= got;
+            runs = runs > 1 ? runs - 1 : 1;
+        }
+    }
+    L.wl_first[p.nblk] = L.nwg;
+    return L;
+}
+}  // namespace
+
 TileWorkList tile_work_list(const TilePlan &p, const uint32_t *nz, const uint64_t *off, uint32_t target_wgs) {
+    if (p.quad) return stream_work_list(p, off, target_wgs);
     const uint64_t UE = kUE;
     TileWorkList L;
     uint64_t total = 0;
@@ -799,7 +1171,7 @@ hipError_t launch_tile_count(TilePlan &p, const uint64_t *rowptr, const uint32_t
     if (*flags_host) return hipSuccess;
     if (p.ntiles) {
         hipLaunchKernelGGL(tile_rowpos_kernel, dim3(blocks_for(p.ntiles, kWG)), dim3(kWG), 0, s, c.cnt, p.rows, p.R,
-                           p.nseg, p.ntiles, c.rowpos, c.tile_nz, c.tile_padded);
+                           p.nseg, p.ntiles, p.quad, c.rowpos, c.tile_nz, c.tile_padded);
         if ((e = hipGetLastError()) != hipSuccess) return e;
     }
     if ((e = hipMemsetAsync(c.tile_padded + p.ntiles, 0, sizeof(uint64_t), s)) != hipSuccess) return e;
@@ -845,20 +1217,27 @@ hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uin
     if (p.rows) {
         hipLaunchKernelGGL(row_segscan_kernel, dim3(blocks_for(p.rows, kWG)), dim3(kWG), 0, s, c.cnt, p.rows, p.nseg);
         hipLaunchKernelGGL(tile_scatter_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, col, val,
-                           p.rows, p.R, p.nseg, wshift, c.cnt, c.rowpos, tm.tile_off, p.vt, qval, tm.tdata, ctmp,
-                           rtmp);
+                           p.rows, p.R, p.nseg, wshift, c.cnt, c.rowpos, tm.tile_off, p.vt, p.quad, qval, tm.tdata,
+                           ctmp, rtmp);
     }
     if (p.ntiles) {
         const dim3 grid(blocks_for(p.cap / 8, kWG));
-        if (p.W == 512)
-            hipLaunchKernelGGL(tile_pack_kernel<512>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles, p.R,
-                               p.vb(), tm.tdata, c.bad);
+#define MLEM_PACK(WW, QQ)                                                                                       \
+    hipLaunchKernelGGL((tile_pack_kernel<WW, QQ>), grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles, p.nseg, p.R, \
+                       p.vb(), tm.tdata, c.bad)
+        if (p.W == 512 && p.quad)
+            MLEM_PACK(512, true);
+        else if (p.W == 512)
+            MLEM_PACK(512, false);
+        else if (p.W == 1024 && p.quad)
+            MLEM_PACK(1024, true);
         else if (p.W == 1024)
-            hipLaunchKernelGGL(tile_pack_kernel<1024>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles, p.R,
-                               p.vb(), tm.tdata, c.bad);
+            MLEM_PACK(1024, false);
+        else if (p.quad)
+            MLEM_PACK(2048, true);
         else
-            hipLaunchKernelGGL(tile_pack_kernel<2048>, grid, dim3(kWG), 0, s, ctmp, rtmp, tm.tile_off, p.ntiles, p.R,
-                               p.vb(), tm.tdata, c.bad);
+            MLEM_PACK(2048, false);
+#undef MLEM_PACK
     }
     if ((e = hipGetLastError()) != hipSuccess) return e;
     if ((e = hipMemcpyAsync(flags_host, c.bad, sizeof(uint32_t), hipMemcpyDeviceToHost, s)) != hipSuccess) return e;
