@@ -28,13 +28,15 @@
 //   so an entry costs 6 bytes (CSR: 8) and a unit is one 12 KiB stream.
 //   tile_off u64[ntiles+1] (entry offsets, padded to 8), tile_nz u32[ntiles]
 //   (entries including padding).
-//   Quad groups (the kernels for several right-hand sides): every row of a
-//   tile starts at a multiple of 4 entries, so each half of a group (a quad)
-//   holds one row; the group word's bits 0-15 hold the row of entries 0-3,
-//   bits 16-31 the row of entries 4-7 (the columns as above).  A row then
-//   ends only after entry 3 or 7 of a work-item, which is where the
-//   accumulator is touched; the price is the padding to whole quads (about
-//   1.5 entries per row segment).
+//   Quad groups in block streams (the kernel for several right-hand sides,
+//   stream_spmv_kernel): every row of a tile starts at a multiple of 4
+//   entries, so each half of a group (a quad) holds one row; the group
+//   word's bits 0-15 hold the row of entries 0-3, bits 16-31 the row of
+//   entries 4-7 (the columns as above).  A row then ends only after entry 3
+//   or 7 of a work-item, which is where the accumulator is touched; the
+//   price is the padding to whole quads (about 1.5 entries per row
+//   segment).  There are no units: the groups of a row block (its tiles in
+//   segment order) are stored as one stream, values then group words.
 //
 // Work: work-item k owns entries [8k, 8k+8) of a unit (two 16-byte value
 // loads, one group word).  The units of a row block (its non-empty tiles in
@@ -147,30 +149,17 @@ constexpr int min_waves() {
     return BC == 1 && VT == VAL_F32 ? (W == 1024 ? 8 : 7) : 1;
 }
 
-// Units whose loads are in flight per workgroup.  One right-hand side: 8
-// workgroups per CU fit in LDS, one unit each is enough.  Several: the
-// accumulator of BC values per row limits a CU to 2-3 workgroups, so each
-// keeps more units in flight.
+// Units whose loads are in flight per workgroup (the kernel takes any depth):
+// eight workgroups per CU fit in LDS, one unit each keeps HBM busy.
 template <int W, int VT, int BC>
 constexpr int pipeline_depth() {
-    return BC == 1 ? 1 : 3;
+    return 1;
 }
 
-// Adds the BC sums sv into the accumulator at slot (16-byte pieces).
-template <int BC>
-__device__ __forceinline__ void acc_add(float *acc, uint32_t slot, const float *sv) {
-#pragma unroll
-    for (int q = 0; q < BC / 4; ++q) {
-        f32x4 o = *reinterpret_cast<const f32x4 *>(acc + slot + 4 * q);
-#pragma unroll
-        for (int b = 0; b < 4; ++b) o[b] += sv[4 * q + b];
-        *reinterpret_cast<f32x4 *>(acc + slot + 4 * q) = o;
-    }
-}
-
-// wl[w] = {row block, first tile (index into tiles), first unit in it, units}
-// tiles[i] = {segment, entries, entry offset lo, hi} of the non-empty tiles.
-// BC right-hand sides at once: x is [nseg*W][BC], part [runs][R][BC].
+// One right-hand side.  wl[w] = {row block, first tile (index into tiles),
+// first unit in it, units}; tiles[i] = {segment, entries, entry offset lo,
+// hi} of the non-empty tiles; part [runs][R].  (The template keeps BC for
+// the shared layout arithmetic; it is 1.)
 template <int W, int VT, int BC, int D>
 __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_kernel(
     uint32_t R, uint32_t span, uint32_t ntl, const uint4 *__restrict__ wl, const uint4 *__restrict__ tiles,
@@ -261,9 +250,9 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
             // The next unit's loads go into set k.  One set: issued now, in
             // flight while this unit is computed.  Several: issued once this
             // unit's registers are used up (after the walk), in flight while
-            // the next D-1 units are computed; loading into registers that
+            // the next D-1 units are computed (loading into registers that
             // still hold live data would make the compiler copy them, and a
-            // copy waits for the loads.
+            // copy waits for the loads).
             auto issue_next = [&]() {
                 iss = advance(iss);
                 su[k] = iss;

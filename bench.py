@@ -295,7 +295,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0,
                     help="batched mode: B sinograms (seeded phantoms, counts log-uniform 1e6-1e9) over the matrix")
     ap.add_argument("--per-pass", type=int, default=4, help="batched mode: sinograms per pass over the matrix")
-    ap.add_argument("--values", default="f32", choices=["f32", "bf16", "fp16"], help="stored matrix values")
+    ap.add_argument("--values", default=None, choices=["f32", "bf16", "fp16"],
+                    help="stored matrix values (default: f32 for one reconstruction, fp16 for --batch and --serve)")
     ap.add_argument("--recon-iters", type=int, default=100, help="iterations per reconstruction (recon/s)")
     ap.add_argument("--serve", default="",
                     help="serving mode over these workloads (comma list of c1, mid, c2): continuous batching")
@@ -309,6 +310,8 @@ def main():
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
+    if args.values is None:
+        args.values = "fp16" if (args.batch or args.serve) else "f32"
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:

@@ -1,5 +1,5 @@
 """End to end through the command-line front end (the reference's argv
-contract) and through the ctypes stub of INTEGRATION.md, on the 2D ring
+contract) and through a minimal ctypes binding of the C ABI, on the 2D ring
 workload, against the FP64 oracle."""
 import ctypes as C
 import os
@@ -77,14 +77,14 @@ def test_cli_file_errors(c1, files):
 
 
 def test_integration_ctypes_stub(c1):
-    """The Python stub printed in INTEGRATION.md, verbatim in spirit."""
+    """A minimal ctypes binding of the C ABI (what a maintainer would add to call mlem_run)."""
     hip = C.CDLL(os.path.join(ROOT, "bachelorarbeit-mlem_amd", "lib", "libmlem_hip.so"))
     hip.mlem_last_error.restype = C.c_char_p
 
     class Opts(C.Structure):
         _fields_ = [(n, C.c_uint32) for n in ("spmv", "using_trans", "secsize_fp", "secsize_bp", "fp_lanes",
                                               "bp_lanes", "profile", "use_graph", "bp_spmv", "tile_width",
-                                              "tile_rows", "tile_groups")]
+                                              "tile_rows", "tile_groups", "values")]
 
     assert C.sizeof(Opts) == C.sizeof(M.Opts)
     P = lambda a: a.ctypes.data_as(C.c_void_p)  # noqa: E731
