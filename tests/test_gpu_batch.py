@@ -118,3 +118,33 @@ def test_batch_argument_errors(setup, c1):
     with M.Batch(setup["m"], 2) as b:
         with pytest.raises(M.MlemError):
             b.set(2, setup["sinos"][0], setup["f0"])
+
+
+@pytest.mark.parametrize("per_pass,width,rows,groups", [(4, 512, 512, 0), (4, 1024, 2048, 1), (8, 512, 1024, 100000)])
+def test_batch_edge_matrices(per_pass, width, rows, groups):
+    """Adversarial matrices through the batched kernels (f32 values): empty
+    rows and columns, one 60,000-entry row (carry chains across whole
+    windows), heavy tails, duplicate columns, no entries, one row, rows with
+    gaps; one run per row block up to one window per run; two passes, the
+    second one partly filled.  Every slot within 1e-5 of the FP64 oracle."""
+    from matrices import make
+    for kind in ["empty_rows", "one_huge_row", "heavy", "dups", "all_empty", "single_row", "row_gaps"]:
+        A = make(kind, seed=7)
+        AT = M.transpose(A)
+        rng = np.random.default_rng(8)
+        slots = per_pass + 1
+        sinos = [rng.integers(0, 30, A.rows).astype(np.int32) for _ in range(slots)]
+        with M.ResidentMatrix(A, AT) as m:
+            s32, _ = m.sensitivity()
+            f0 = np.ones(A.cols, np.float32)
+            print(f"batch edge {kind} per_pass={per_pass} W={width} R={rows} groups={groups}", flush=True)
+            with M.Batch(m, slots, values=0, per_pass=per_pass, tile_width=width, tile_rows=rows,
+                         tile_groups=groups) as b:
+                for k in range(slots):
+                    b.set(k, sinos[k], f0)
+                b.iterate(8)
+                b.sync()
+                for k in range(slots):
+                    ref = O.mlem_f64(A.rowptr, A.col, A.val, A.rows, A.cols, sinos[k], s32.astype(np.float64), f0, 8)
+                    img = b.image(k)
+                    assert np.linalg.norm(ref) == 0 and np.all(img == 0) or rel_l2(img, ref) <= 1e-5, (kind, k)
