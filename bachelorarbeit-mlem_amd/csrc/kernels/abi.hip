@@ -371,8 +371,12 @@ int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t sec
         const uint32_t R = o.tile_rows == 0 ? 2048u : o.tile_rows;
         const int st = setup_tiled(m, which, W, R, o.values, 1, o.tile_groups, tiled, work, part);
         if (st == MLEM_OK) return MLEM_OK;
-        // explicit width or 16-bit values: no fallback
-        if (st != MLEM_ERR_UNSUPPORTED || o.tile_width != 0 || o.values != VAL_F32) return st;
+        // Automatic width with f32 values: a matrix the tiles do not suit, or
+        // whose tiles do not fit beside the resident copies (the 100 GB
+        // matrices), takes the plain merge path; otherwise no fallback.
+        if ((st != MLEM_ERR_UNSUPPORTED && st != MLEM_ERR_OUT_OF_MEMORY) || o.tile_width != 0 || o.values != VAL_F32)
+            return st;
+        (void)hipGetLastError();  // the failed allocation's error is handled here
     }
     if (o.values != VAL_F32)
         return set_err(MLEM_ERR_UNSUPPORTED, "16-bit matrix values need the blocked merge-based kernels");
@@ -947,18 +951,41 @@ int mlem_matrix_load_csr4(int device, const char *path, int threads, mlem_matrix
     return MLEM_OK;
 }
 
-int mlem_matrix_build_transpose(mlem_matrix *m) {
+int mlem_matrix_build_transpose(mlem_matrix *m) { return mlem_matrix_build_transpose_limited(m, 0, nullptr); }
+
+int mlem_matrix_build_transpose_limited(mlem_matrix *m, uint64_t scratch_bytes, uint32_t *ranges) {
     ARGCHK(m, "mlem_matrix_build_transpose: null matrix");
     HIPCHK(hipSetDevice(m->device));
+    if (ranges) *ranges = 0;
     DevBuf scratch, rp, ci, va;
     HIPCHK(rp.alloc(sizeof(uint64_t) * ((size_t)m->cols + 1)));
     HIPCHK(ci.alloc(sizeof(uint32_t) * m->nnz));
     HIPCHK(va.alloc(sizeof(float) * m->nnz));
-    HIPCHK(scratch.alloc(transpose_scratch_bytes(m->rows, m->cols, m->nnz)));
-    HIPCHK(launch_transpose(m->rows, m->cols, m->nnz, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(),
-                            m->val.as<float>(), rp.as<uint64_t>(), ci.as<uint32_t>(), va.as<float>(), scratch.p,
-                            nullptr));
-    HIPCHK(hipStreamSynchronize(nullptr));
+    const size_t whole = transpose_scratch_bytes(m->rows, m->cols, m->nnz);
+    size_t budget = scratch_bytes;
+    if (budget == 0) {  // automatic: one sort when its scratch fits beside a margin, else ranges in what is free
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        const size_t margin = (size_t)1 << 30;
+        budget = free_b > margin ? free_b - margin : 0;
+    }
+    if (whole <= budget) {
+        HIPCHK(scratch.alloc(whole));
+        HIPCHK(launch_transpose(m->rows, m->cols, m->nnz, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(),
+                                m->val.as<float>(), rp.as<uint64_t>(), ci.as<uint32_t>(), va.as<float>(), scratch.p,
+                                nullptr));
+        HIPCHK(hipStreamSynchronize(nullptr));
+    } else {
+        uint32_t nr = 0;
+        const hipError_t e = transpose_in_ranges(m->rows, m->cols, m->nnz, m->rowptr.as<uint64_t>(),
+                                                 m->col.as<uint32_t>(), m->val.as<float>(), rp.as<uint64_t>(),
+                                                 ci.as<uint32_t>(), va.as<float>(), budget, &nr, nullptr);
+        if (e == hipErrorOutOfMemory)
+            return set_err(MLEM_ERR_OUT_OF_MEMORY,
+                           "mlem_matrix_build_transpose: %zu bytes of scratch cannot hold one column range", budget);
+        HIPCHK(e);
+        if (ranges) *ranges = nr;
+    }
     std::lock_guard<std::mutex> lk(m->mu);
     std::swap(m->rowptrT.p, rp.p);
     std::swap(m->rowptrT.bytes, rp.bytes);

@@ -172,5 +172,11 @@ size_t transpose_scratch_bytes(uint32_t rows, uint32_t cols, uint64_t nnz);
 hipError_t launch_transpose(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr, const uint32_t *col,
                             const float *val, uint64_t *colptrT, uint32_t *rowT, float *valT, void *scratch,
                             hipStream_t s);
+// The same result with at most `budget` bytes of scratch (allocated and freed
+// here; the call waits): column ranges sorted one at a time.  *ranges = how
+// many; hipErrorOutOfMemory when one column alone does not fit.
+hipError_t transpose_in_ranges(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
+                               const uint32_t *col, const float *val, uint64_t *colptrT, uint32_t *rowT, float *valT,
+                               size_t budget, uint32_t *ranges, hipStream_t s);
 
 }  // namespace mlem

@@ -67,6 +67,7 @@ def hip_lib() -> C.CDLL:
             "mlem_opts_default": (None, [P(Opts)]),
             "mlem_matrix_create": (cint, [cint, u32, u32, u64, vp, vp, vp, vp, vp, vp, P(vp)]),
             "mlem_matrix_build_transpose": (cint, [vp]),
+            "mlem_matrix_build_transpose_limited": (cint, [vp, u64, P(u32)]),
             "mlem_matrix_load_csr4": (cint, [cint, C.c_char_p, cint, P(vp), vp]),
             "mlem_batch_opts_default": (None, [vp]),
             "mlem_batch_create": (cint, [vp, u32, vp, P(vp)]),
@@ -508,8 +509,12 @@ class ResidentMatrix:
             _check(lib.mlem_memcpy_d2h(_ptr(val), va, val.nbytes, None))
         return Csr(self.rows, self.cols, rowptr, col, val)
 
-    def build_transpose(self):
-        _check(hip_lib().mlem_matrix_build_transpose(self.h))
+    def build_transpose(self, scratch_bytes: int = 0) -> int:
+        """A^T on the device (bit-exact with the host transpose) within scratch_bytes of scratch (0: what is
+        free).  Returns the column ranges sorted one at a time (0: one sort)."""
+        n = u32()
+        _check(hip_lib().mlem_matrix_build_transpose_limited(self.h, scratch_bytes, C.byref(n)))
+        return n.value
 
     def info(self):
         r, c, n, t, b = u32(), u32(), u64(), cint(), u64()

@@ -84,6 +84,9 @@ def workload(name, threads):
         geo, counts, shapes, label = M.RING3D_576_MID(), 1e7, 6, "3D MLEM FP32, 2 rings x 288 crystals, 96x96x24 voxels"
     elif name == "c1":
         geo, counts, shapes, label = M.RING2D_288(), 1e6, 5, "2D MLEM FP32, 288-crystal ring, 64x64 voxels"
+    elif name == "c3":
+        geo, counts, shapes, label = (M.RING3D_1152_FINE(), 1e8, 8,
+                                      "3D MLEM FP32, 2 rings x 576 crystals, 280x280x80 voxels, 128 sub-rays")
     else:
         raise SystemExit(f"unknown workload {name}")
     t = time.time()
@@ -290,7 +293,9 @@ def main():
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=100, help="timed MLEM iterations")
     ap.add_argument("--warmup", type=int, default=5)
-    ap.add_argument("--workload", default="c2", choices=["c2", "mid", "c1"])
+    ap.add_argument("--workload", default="c2", choices=["c2", "mid", "c1", "c3"],
+                    help="c3: the 100 GB-class matrix (A and A^T resident, plain merge / CSR-vector kernels; "
+                         "no CPU baseline: both copies do not fit the host memory of one command)")
     ap.add_argument("--fp", default="merge", choices=["merge", "vector"])
     ap.add_argument("--bp", default="merge", choices=["merge", "vector"])
     ap.add_argument("--secsize", type=int, default=8)
@@ -340,9 +345,19 @@ def main():
     t = time.time()
     m = M.ResidentMatrix(A)
     log(f"upload A {time.time() - t:.2f} s")
+    if args.workload == "c3":  # the host copy is not needed any more (and is ~120 GB)
+        class _Dims:
+            pass
+        dims = _Dims()
+        dims.rows, dims.cols, dims.nnz = A.rows, A.cols, A.nnz
+        A = dims
+        args.no_cpu = True
+        args.tile = 1  # the tiles do not fit in HBM beside both CSR copies: plain merge-path kernels
+        log("host copy of A released; CPU baseline skipped (A and A^T do not fit the host memory of one command)")
     t = time.time()
-    m.build_transpose()
-    log(f"device transpose (stable counting sort) {time.time() - t:.2f} s")
+    ranges = m.build_transpose()
+    log(f"device transpose (stable counting sort{f', {ranges} column ranges' if ranges else ''}) "
+        f"{time.time() - t:.2f} s")
     t = time.time()
     s32, _ = m.sensitivity()
     log(f"FP64 sensitivity {time.time() - t:.2f} s")
@@ -362,6 +377,7 @@ def main():
         log(f"reconstruction set-up (tiles of A and A^T, work lists) {time.time() - t:.2f} s")
         r.iterate(args.warmup)
         r.sync()
+        ll0 = r.loglik() if args.workload == "c3" else None
         r.reset_times()
         M.device_sync()
         t0 = time.perf_counter()
@@ -371,6 +387,10 @@ def main():
         t1 = time.perf_counter()
         phase, n_timed = r.phase_times()
         img = r.image()
+        if ll0 is not None:  # MLEM does not decrease the Poisson log-likelihood (a size-independent check)
+            ll1 = r.loglik()
+            log(f"log-likelihood after {args.warmup} iterations {ll0:.9e}, after {args.warmup + args.steps}: "
+                f"{ll1:.9e} ({'increased' if ll1 >= ll0 else 'DECREASED'})")
     elapsed = t1 - t0
     its = args.steps / elapsed
     rows, cols, nnz = A.rows, A.cols, A.nnz

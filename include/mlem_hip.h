@@ -217,8 +217,16 @@ int mlem_matrix_create(int device, uint32_t rows, uint32_t cols, uint64_t nnz,
                        const uint32_t *colT, const float *valT,
                        mlem_matrix **out);
 /* Build A^T on the device from the resident A (stable counting sort,
- * bit-exact with the host transpose) instead of uploading one. */
+ * bit-exact with the host transpose) instead of uploading one.  Replaces
+ * sptrans_scanTrans_specialized (sptrans.h:397-545) as called at
+ * main.cu:99-105.  The one sort needs 20 bytes of scratch per nonzero; when
+ * that is not free (the 100 GB matrices) the sort runs over column ranges
+ * within the free memory, with the same result. */
 int mlem_matrix_build_transpose(mlem_matrix *m);
+/* The same within scratch_bytes of scratch (0 = what is free); *ranges (may
+ * be NULL) = column ranges used, 0 for the single sort.  MLEM_ERR_OUT_OF_MEMORY
+ * when one column's nonzeros do not fit. */
+int mlem_matrix_build_transpose_limited(mlem_matrix *m, uint64_t scratch_bytes, uint32_t *ranges);
 
 /* Load A straight from a csr4 file into a resident matrix (Csr4Matrix +
  * mapRows + csr_format_for_cuda + the H2D copies of main.cu:71-79,263-272,

@@ -210,6 +210,30 @@ def test_device_transpose_bit_exact(mats):
         m.close()
 
 
+@pytest.mark.parametrize("budget", [2 << 20, 4 << 20, 64 << 20])
+def test_device_transpose_in_column_ranges_bit_exact(mats, budget):
+    """The transpose within a small scratch budget (column ranges sorted one
+    at a time, as for the 100 GB matrices) is bit-identical to the host
+    transpose; a budget too small for the fixed part is refused."""
+    many = 0
+    for name, A in mats:
+        host = M.transpose(A)
+        with M.ResidentMatrix(A) as m:
+            ranges = m.build_transpose(budget)
+            print(f"transpose in ranges {name} budget={budget} ranges={ranges}", flush=True)
+            many = max(many, ranges)
+            dev = m.download_transpose()
+        assert np.array_equal(dev.rowptr, host.rowptr)
+        assert np.array_equal(dev.col, host.col)
+        assert np.array_equal(dev.val.view(np.uint32), host.val.view(np.uint32))
+    if budget == 2 << 20:
+        assert many > 1
+    A = mats[0][1]
+    with M.ResidentMatrix(A) as m:
+        with pytest.raises(M.MlemError):
+            m.build_transpose(4096)
+
+
 def test_bad_arguments_rejected():
     A = make("tiny")
     d_rp, d_c, d_v = upload(A)
