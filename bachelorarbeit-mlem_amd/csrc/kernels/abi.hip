@@ -17,6 +17,7 @@
 #include <cstdio>
 #include <cstring>
 #include <map>
+#include <set>
 #include <tuple>
 #include <memory>
 #include <mutex>
@@ -155,6 +156,7 @@ struct Tiled {
     TilePlan plan;
     DevBuf mem;
     TiledMatrix view;
+    DevBuf sens;  // A^T tiles of 16-bit values: the quantized matrix's column sums (f32)
     std::map<uint32_t, std::unique_ptr<TileWorkDev>> works;  // by target workgroup count
 };
 
@@ -268,6 +270,36 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bo
     const uint64_t *rp = which == 0 ? m->rowptr.as<uint64_t>() : m->rowptrT.as<uint64_t>();
     const uint32_t *ci = which == 0 ? m->col.as<uint32_t>() : m->colT.as<uint32_t>();
     const float *va = which == 0 ? m->val.as<float>() : m->valT.as<float>();
+    // 16-bit values on A^T: the codes of A's quantization, transposed, stored
+    // as they are, so that the forward and back projections use one quantized
+    // matrix (and the sensitivity is its column sums).
+    DevBuf codesT;
+    if (which == 1 && vt != VAL_F32) {
+        DevBuf scl, codes, q;
+        HIPCHK(scl.alloc(sizeof(float) * std::max<size_t>(1, m->rows)));
+        HIPCHK(codes.alloc(sizeof(float) * std::max<uint64_t>(1, m->nnz)));
+        HIPCHK(q.alloc(sizeof(uint16_t) * std::max<uint64_t>(1, m->nnz)));
+        HIPCHK(launch_quantize_codes(m->rowptr.as<uint64_t>(), m->val.as<float>(), m->rows, m->nnz, vt,
+                                     scl.as<float>(), codes.as<float>(), q.as<uint16_t>(), nullptr));
+        q.reset();
+        DevBuf tcp, trow;
+        HIPCHK(codesT.alloc(sizeof(float) * std::max<uint64_t>(1, m->nnz)));
+        HIPCHK(tcp.alloc(sizeof(uint64_t) * ((size_t)m->cols + 1)));
+        HIPCHK(trow.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, m->nnz)));
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        const size_t margin = (size_t)1 << 30;
+        uint32_t nr = 0;
+        HIPCHK(transpose_in_ranges(m->rows, m->cols, m->nnz, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(),
+                                   codes.as<float>(), tcp.as<uint64_t>(), trow.as<uint32_t>(), codesT.as<float>(),
+                                   free_b > margin ? free_b - margin : 0, &nr, nullptr));
+        HIPCHK(t->sens.alloc(sizeof(float) * std::max<size_t>(1, m->cols)));
+        HIPCHK(launch_colsum_weighted_f64(m->rowptrT.as<uint64_t>(), trow.as<uint32_t>(), codesT.as<float>(),
+                                          scl.as<float>(), m->cols, nullptr, t->sens.as<float>(), nullptr));
+        HIPCHK(hipStreamSynchronize(nullptr));
+        va = codesT.as<float>();
+        t->plan.unit_scale = true;
+    }
     DevBuf scratch;
     HIPCHK(scratch.alloc(tile_count_scratch_bytes(t->plan)));
     uint32_t fl = 0;
@@ -1128,6 +1160,9 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
         if (int st = setup_merge_side(m, 1, o, r->opts.secsize_bp, &r->tbp, &r->wbp, &r->part_bp, &r->pbp,
                                       &r->scratch_bp))
             return st;
+    if (o.values != VAL_F32 && r->tbp)  // the quantized matrix's own sensitivity
+        HIPCHK(hipMemcpyAsync(r->sens.p, r->tbp->sens.p, sizeof(float) * m->cols, hipMemcpyDeviceToDevice,
+                              r->stream));
     HIPCHK(r->llpart.alloc(sizeof(double) * std::max<uint32_t>(1, loglik_blocks(m->rows))));
     HIPCHK(hipStreamSynchronize(r->stream));
     guard.armed = false;
@@ -1249,6 +1284,135 @@ int mlem_recon_destroy(mlem_recon *r) {
     return MLEM_OK;
 }
 
+// -------------------------------------------------------------- autotuner
+// Times the blocked kernels of both sides over a few tile shapes and
+// workgroup counts and reports the fastest.  Candidate tiles that did not
+// exist before and were not chosen are freed again.
+int mlem_autotune(mlem_matrix *m, uint32_t per_pass, uint32_t values, mlem_tune_result *out) {
+    ARGCHK(m && out, "mlem_autotune: null argument");
+    std::memset(out, 0, sizeof(*out));
+    const uint32_t BC = per_pass ? per_pass : 1u;
+    ARGCHK(BC == 1 || BC == 4 || BC == 8, "mlem_autotune: per_pass must be 1, 4 or 8");
+    ARGCHK(values <= VAL_F16, "mlem_autotune: values must be 0 (f32), 1 (bf16) or 2 (fp16)");
+    ARGCHK(m->has_T, "mlem_autotune: the transpose must be resident");
+    HIPCHK(hipSetDevice(m->device));
+    int dev = 0, cus = 256;
+    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
+    std::vector<std::pair<uint32_t, uint32_t>> shapes;  // (W, R)
+    std::vector<uint32_t> groups;                       // per CU
+    if (BC == 1) {
+        shapes = {{1024, 1024}, {1024, 2048}, {1024, 4096}, {2048, 1024}, {2048, 2048}};
+        groups = {16, 24, 32};
+    } else if (BC == 4) {
+        shapes = {{512, 1024}, {512, 2048}, {1024, 1024}};
+        groups = {8, 16};
+    } else {
+        shapes = {{512, 512}, {512, 1024}};
+        groups = {8, 16};
+    }
+    std::set<mlem_matrix::TileKey> before;
+    {
+        std::lock_guard<std::mutex> lk(m->mu);
+        for (const auto &t : m->tiles) before.insert(t.first);
+    }
+    const size_t rows_x = ((size_t)m->rows + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
+    const size_t cols_x = ((size_t)m->cols + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
+    DevBuf xf, xb, yf, yb;
+    HIPCHK(xf.alloc(sizeof(float) * cols_x * BC));
+    HIPCHK(xb.alloc(sizeof(float) * rows_x * BC));
+    HIPCHK(yf.alloc(sizeof(float) * rows_x * BC));
+    HIPCHK(yb.alloc(sizeof(float) * cols_x * BC));
+    hipStream_t s = nullptr;
+    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    auto cleanup = [&]() {
+        if (e0) (void)hipEventDestroy(e0);
+        if (e1) (void)hipEventDestroy(e1);
+        (void)hipStreamSynchronize(s);
+        (void)hipStreamDestroy(s);
+    };
+    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
+        cleanup();
+        return set_err(MLEM_ERR_HIP, "mlem_autotune: event creation failed");
+    }
+    if (launch_fill(xf.as<float>(), 1.0f, cols_x * BC, s) != hipSuccess ||
+        launch_fill(xb.as<float>(), 1.0f, rows_x * BC, s) != hipSuccess) {
+        cleanup();
+        return set_err(MLEM_ERR_HIP, "mlem_autotune: fill failed");
+    }
+    double best = 1e300;
+    uint32_t bestW = 0, bestR = 0, bestG = 0, tried = 0;
+    double best_fp = 0, best_bp = 0;
+    // One side's time per launch (partials + reduce), mean of three after a warm-up.
+    auto time_side = [&](const Tiled *t, const TileWorkDev *w, DevBuf &part, const float *x, float *y, double *ms) {
+        EpiArgs ea{};
+        ea.out = y;
+        for (int rep = 0; rep < 4; ++rep) {
+            if (rep == 1 && hipEventRecord(e0, s) != hipSuccess) return hipErrorUnknown;
+            hipError_t e = launch_tiled_partials(t->plan, t->view, w->view, x, part.as<float>(), BC, s);
+            if (e == hipSuccess) e = launch_tiled_reduce(t->plan, t->view, w->view, part.as<float>(), BC, EPI_PLAIN, ea, s);
+            if (e != hipSuccess) return e;
+        }
+        hipError_t e = hipEventRecord(e1, s);
+        if (e == hipSuccess) e = hipEventSynchronize(e1);
+        float f = 0.0f;
+        if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
+        *ms = f / 3.0;
+        return e;
+    };
+    for (const auto &sh : shapes) {
+        for (uint32_t g : groups) {
+            const Tiled *tf = nullptr, *tb = nullptr;
+            const TileWorkDev *wf = nullptr, *wb = nullptr;
+            DevBuf pf, pb;
+            int st = setup_tiled(m, 0, sh.first, sh.second, values, BC, g * (uint32_t)cus, &tf, &wf, &pf);
+            if (st == MLEM_OK) st = setup_tiled(m, 1, sh.first, sh.second, values, BC, g * (uint32_t)cus, &tb, &wb, &pb);
+            if (st == MLEM_ERR_UNSUPPORTED || st == MLEM_ERR_OUT_OF_MEMORY) {
+                (void)hipGetLastError();
+                continue;  // this shape does not suit or fit the matrix
+            }
+            if (st != MLEM_OK) {
+                cleanup();
+                return st;
+            }
+            double fp = 0, bp = 0;
+            hipError_t e = time_side(tf, wf, pf, xf.as<float>(), yf.as<float>(), &fp);
+            if (e == hipSuccess) e = time_side(tb, wb, pb, xb.as<float>(), yb.as<float>(), &bp);
+            if (e != hipSuccess) {
+                cleanup();
+                return hip_err(e, "mlem_autotune: timing", __LINE__);
+            }
+            ++tried;
+            if (fp + bp < best) {
+                best = fp + bp;
+                bestW = sh.first;
+                bestR = sh.second;
+                bestG = g;
+                best_fp = fp;
+                best_bp = bp;
+            }
+        }
+    }
+    cleanup();
+    {  // free the candidate tiles that were built here and not chosen
+        std::lock_guard<std::mutex> lk(m->mu);
+        for (auto it = m->tiles.begin(); it != m->tiles.end();) {
+            const auto &k = it->first;
+            const bool chosen = std::get<1>(k) == bestW && std::get<2>(k) == bestR && std::get<3>(k) == values &&
+                                std::get<4>(k) == (BC > 1);
+            it = (!before.count(k) && !chosen) ? m->tiles.erase(it) : std::next(it);
+        }
+    }
+    if (!tried) return set_err(MLEM_ERR_UNSUPPORTED, "mlem_autotune: no tile shape suits or fits this matrix");
+    out->tile_width = bestW;
+    out->tile_rows = bestR;
+    out->tile_groups = bestG * (uint32_t)cus;
+    out->candidates = tried;
+    out->fp_ms = best_fp;
+    out->bp_ms = best_bp;
+    return MLEM_OK;
+}
+
 // ---------------------------------------------------------------- batches
 void mlem_batch_opts_default(mlem_batch_opts *o) {
     if (!o) return;
@@ -1299,7 +1463,11 @@ int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opt
     HIPCHK(launch_fill(b->f.as<float>(), 0.0f, b->cols_x * per, b->stream));
     HIPCHK(launch_fill(b->active.as<float>(), 0.0f, per, b->stream));
     b->active_host.assign(per, 0);
-    if (int st = sens_on_device(m, b->sens.as<float>(), nullptr, b->stream)) return st;
+    if (o.values != VAL_F32)  // the quantized matrix's own sensitivity
+        HIPCHK(hipMemcpyAsync(b->sens.p, b->tbp->sens.p, sizeof(float) * m->cols, hipMemcpyDeviceToDevice,
+                              b->stream));
+    else if (int st = sens_on_device(m, b->sens.as<float>(), nullptr, b->stream))
+        return st;
     HIPCHK(hipStreamSynchronize(b->stream));
     *out = b.release();
     return MLEM_OK;

@@ -75,6 +75,9 @@ hipError_t launch_colsum_rows_f64(const uint64_t *rowptrT, const float *valT, ui
                                   hipStream_t s);
 hipError_t launch_colsum_scatter_f64(const uint64_t *rowptr, const uint32_t *col, const float *val, uint32_t rows,
                                      double *s64, hipStream_t s);
+// s[c] = sum_k valT[k] * w[rowT[k]] over row c of A^T, in FP64.
+hipError_t launch_colsum_weighted_f64(const uint64_t *rowptrT, const uint32_t *rowT, const float *valT, const float *w,
+                                      uint32_t cols, double *s64, float *s32, hipStream_t s);
 hipError_t launch_f64_to_f32(const double *a, float *b, uint32_t n, hipStream_t s);
 // Poisson log-likelihood partial sums over rows of A x: per-block FP64
 // partials into part[grid], grid = loglik_blocks(rows).
@@ -106,6 +109,9 @@ struct TilePlan {
     // the single right-hand-side kernels), true = quads (every 4 entries
     // belong to one row, the several right-hand-side kernels).
     bool quad = false;
+    // 16-bit values that are codes already (the transposed tiles hold the
+    // codes of A's tiles, so both sides use one quantized matrix): scale 1.
+    bool unit_scale = false;
     uint32_t vb() const { return vt == VAL_F32 ? 4u : 2u; }  // bytes per stored value
 };
 struct TiledMatrix {
@@ -130,6 +136,11 @@ struct TileWork {
     const uint32_t *wl_first = nullptr;
 };
 TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t vt, bool quad);
+// The 16-bit quantization of a CSR's values as the tiles store it: per-row
+// scale (largest |value|, times 2^-14 for FP16), codes rounded to nearest
+// (q, 2 bytes per nonzero of scratch) and decoded to f32 without the scale.
+hipError_t launch_quantize_codes(const uint64_t *rowptr, const float *val, uint32_t rows, uint64_t nnz, uint32_t vt,
+                                 float *scale, float *codes, uint16_t *q, hipStream_t s);
 bool valid_tile_width(uint32_t W);
 // Kernel shapes built: (W, right-hand sides per pass) in {1024, 2048} x 1,
 // {512, 1024} x 4, 512 x 8.

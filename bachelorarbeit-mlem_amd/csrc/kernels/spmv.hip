@@ -402,6 +402,25 @@ __global__ __launch_bounds__(kWG) void colsum_rows_kernel(const uint64_t *__rest
     }
 }
 
+// s[c] = sum over row c of A^T of valT[k] * w[rowT[k]] (FP64): the column
+// sums of a matrix given as row weights times codes.
+__global__ __launch_bounds__(kWG) void colsum_weighted_kernel(const uint64_t *__restrict__ rowptrT,
+                                                              const uint32_t *__restrict__ rowT,
+                                                              const float *__restrict__ valT,
+                                                              const float *__restrict__ w, uint32_t cols,
+                                                              double *__restrict__ s64, float *__restrict__ s32) {
+    const uint32_t row = blockIdx.x * (uint32_t)(kWG / 64) + threadIdx.x / 64;
+    const uint32_t lane = threadIdx.x % 64;
+    if (row >= cols) return;
+    double acc = 0.0;
+    for (uint64_t k = rowptrT[row] + lane; k < rowptrT[row + 1]; k += 64) acc += (double)valT[k] * (double)w[rowT[k]];
+    acc = group_sum_f64<64>(acc);
+    if (lane == 0) {
+        if (s64) s64[row] = acc;
+        if (s32) s32[row] = (float)acc;
+    }
+}
+
 __global__ __launch_bounds__(kWG) void colsum_scatter_kernel(const uint64_t *__restrict__ rowptr,
                                                              const uint32_t *__restrict__ col,
                                                              const float *__restrict__ val, uint32_t rows,
@@ -643,6 +662,14 @@ hipError_t launch_colsum_rows_f64(const uint64_t *rowptrT, const float *valT, ui
     if (cols == 0) return hipSuccess;
     hipLaunchKernelGGL(colsum_rows_kernel, dim3(blocks_for(cols, kWG / 64)), dim3(kWG), 0, s, rowptrT, valT, cols,
                        s64, s32);
+    return hipGetLastError();
+}
+
+hipError_t launch_colsum_weighted_f64(const uint64_t *rowptrT, const uint32_t *rowT, const float *valT, const float *w,
+                                      uint32_t cols, double *s64, float *s32, hipStream_t s) {
+    if (cols == 0) return hipSuccess;
+    hipLaunchKernelGGL(colsum_weighted_kernel, dim3(blocks_for(cols, kWG / 64)), dim3(kWG), 0, s, rowptrT, rowT, valT,
+                       w, cols, s64, s32);
     return hipGetLastError();
 }
 

@@ -621,7 +621,10 @@ __global__ void blocked_reduce_kernel(const float *__restrict__ part, const uint
         if constexpr (EPI == EPI_PLAIN) {
             ea.out[k] = yq;
         } else if constexpr (EPI == EPI_RATIO) {
-            ea.out[k] = (yq != 0.0f) ? (float)ea.g[k] / yq : yq;
+            // 16-bit values: the back projection's tiles hold the same codes
+            // without the row scales, so the ratio g / (scale * sum) goes on
+            // multiplied by the scale, that is g / sum.
+            ea.out[k] = (y[q] != 0.0f) ? (float)ea.g[k] / y[q] : 0.0f;
         } else {
             const float v = ea.f[k] * yq;
             const float n = ea.norm[r];
@@ -700,10 +703,12 @@ __global__ void row_segscan_kernel(uint32_t *__restrict__ cnt, uint32_t rows, ui
     }
 }
 
-// scale[r] = largest |value| of row r (1 for a row without nonzero values):
-// 16-bit tiles store value / scale.
+// scale[r] = mult x the largest |value| of row r (mult for a row without
+// nonzero values): 16-bit tiles store value / scale.  FP16 takes mult =
+// 2^-14, so a row's largest value is stored as 16384 and even values 1e-8 of
+// it stay normal numbers (11-bit significand); BF16 has the range anyway.
 __global__ __launch_bounds__(kWG) void row_absmax_kernel(const uint64_t *__restrict__ rowptr,
-                                                         const float *__restrict__ val, uint32_t rows,
+                                                         const float *__restrict__ val, uint32_t rows, float mult,
                                                          float *__restrict__ scale) {
     const uint32_t row = blockIdx.x * (kWG / 64) + threadIdx.x / 64;
     const uint32_t lane = threadIdx.x % 64;
@@ -711,7 +716,7 @@ __global__ __launch_bounds__(kWG) void row_absmax_kernel(const uint64_t *__restr
     float m = 0.0f;
     for (uint64_t k = rowptr[row] + lane; k < rowptr[row + 1]; k += 64) m = fmaxf(m, fabsf(val[k]));
     for (int o = 32; o > 0; o >>= 1) m = fmaxf(m, __shfl_xor(m, o));
-    if (lane == 0) scale[row] = m > 0.0f ? m : 1.0f;
+    if (lane == 0) scale[row] = (m > 0.0f ? m : 1.0f) * mult;
 }
 
 // f32 -> bf16 bits, round to nearest even (finite inputs).
@@ -722,31 +727,22 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 
 __device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
-// 16-bit values of every row, one work-item per row in column order, with
-// error feedback: each value is rounded together with the rounding error
-// carried from the previous entry of its row, so the running sums of a row's
-// stored values stay within one rounding of the true ones (a smooth x then
-// sees almost no quantization error in the row sum).
+// 16-bit values, each value / scale of its row rounded to nearest (even).
+// Rounding every value on its own keeps each entry within half a unit in the
+// last place; carrying the rounding error along the row (error feedback)
+// keeps row sums closer but can move a small entry by many times itself,
+// which the back projection and the sensitivity of the voxels behind it do
+// not forgive (measured: rel-L2 up to 0.13 after 100 iterations on the 3D
+// scanner matrix, against 1e-3 without it).
 __global__ void quantize_rows_kernel(const uint64_t *__restrict__ rowptr, const float *__restrict__ val,
                                      uint32_t rows, const float *__restrict__ scale, uint32_t vt,
                                      uint16_t *__restrict__ q) {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= rows) return;
     const float inv = 1.0f / scale[row];
-    float e = 0.0f;
     for (uint64_t k = rowptr[row]; k < rowptr[row + 1]; ++k) {
-        const float t = val[k] * inv + e;
-        uint16_t b;
-        float back;
-        if (vt == VAL_BF16) {
-            b = to_bf16(t);
-            back = from_bf16(b);
-        } else {
-            b = __half_as_ushort(__float2half_rn(t));
-            back = __half2float(__ushort_as_half(b));
-        }
-        q[k] = b;
-        e = t - back;
+        const float t = val[k] * inv;
+        q[k] = vt == VAL_BF16 ? to_bf16(t) : __half_as_ushort(__float2half_rn(t));
     }
 }
 
@@ -880,6 +876,12 @@ __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restri
     word.w = w3;
     *reinterpret_cast<u32x4 *>(tdata + (vb + 2) * base + vb * n8u + 2 * e) = word;
     if (big) atomicOr(bad, 4u);
+}
+
+// codes[k] = the 16-bit value q[k] as f32 (without its row's scale).
+__global__ void decode_codes_kernel(const uint16_t *__restrict__ q, uint64_t n, uint32_t vt, float *__restrict__ codes) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x)
+        codes[k] = vt == VAL_BF16 ? from_bf16(q[k]) : __half2float(__ushort_as_half(q[k]));
 }
 
 __global__ void fill_u16_kernel(uint16_t *__restrict__ p, uint16_t v, uint64_t n) {
@@ -1196,8 +1198,12 @@ hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uin
     uint16_t *qval = nullptr;
     if (tm.scale && p.rows) {
         qval = reinterpret_cast<uint16_t *>(static_cast<char *>(fill_scratch) + 2 * align256(2 * p.cap));
-        hipLaunchKernelGGL(row_absmax_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, val,
-                           p.rows, tm.scale);
+        if (p.unit_scale) {  // the values are 16-bit codes already: stored as they are
+            if ((e = launch_fill(tm.scale, 1.0f, p.rows, s)) != hipSuccess) return e;
+        } else {
+            hipLaunchKernelGGL(row_absmax_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, val,
+                               p.rows, p.vt == VAL_F16 ? 1.0f / 16384.0f : 1.0f, tm.scale);
+        }
         hipLaunchKernelGGL(quantize_rows_kernel, dim3(blocks_for(p.rows, 64)), dim3(64), 0, s, rowptr, val, p.rows,
                            tm.scale, p.vt, qval);
     }
@@ -1277,6 +1283,20 @@ hipError_t launch_tiled_reduce(const TilePlan &p, const TiledMatrix &tm, const T
         case 8: return launch_reduce_bc<8>(p, wk, part, tm.scale, epi, ea, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_quantize_codes(const uint64_t *rowptr, const float *val, uint32_t rows, uint64_t nnz, uint32_t vt,
+                                 float *scale, float *codes, uint16_t *q, hipStream_t s) {
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(row_absmax_kernel, dim3(blocks_for(rows, kWG / 64)), dim3(kWG), 0, s, rowptr, val, rows,
+                       vt == VAL_F16 ? 1.0f / 16384.0f : 1.0f, scale);
+    hipLaunchKernelGGL(quantize_rows_kernel, dim3(blocks_for(rows, 64)), dim3(64), 0, s, rowptr, val, rows, scale, vt,
+                       q);
+    if (nnz) {
+        const uint64_t blocks = std::min<uint64_t>((nnz + kWG - 1) / kWG, 65536);
+        hipLaunchKernelGGL(decode_codes_kernel, dim3((uint32_t)blocks), dim3(kWG), 0, s, q, nnz, vt, codes);
+    }
+    return hipGetLastError();
 }
 
 }  // namespace mlem

@@ -68,6 +68,7 @@ def hip_lib() -> C.CDLL:
             "mlem_matrix_create": (cint, [cint, u32, u32, u64, vp, vp, vp, vp, vp, vp, P(vp)]),
             "mlem_matrix_build_transpose": (cint, [vp]),
             "mlem_matrix_build_transpose_limited": (cint, [vp, u64, P(u32)]),
+            "mlem_autotune": (cint, [vp, u32, u32, vp]),
             "mlem_matrix_load_csr4": (cint, [cint, C.c_char_p, cint, P(vp), vp]),
             "mlem_batch_opts_default": (None, [vp]),
             "mlem_batch_create": (cint, [vp, u32, vp, P(vp)]),
@@ -469,6 +470,11 @@ class Server:
             pass
 
 
+class TuneResult(C.Structure):
+    _fields_ = [("tile_width", u32), ("tile_rows", u32), ("tile_groups", u32), ("candidates", u32),
+                ("fp_ms", f64), ("bp_ms", f64)]
+
+
 class LoadStats(C.Structure):
     _fields_ = [("bytes", u64), ("open_s", f64), ("stream_s", f64), ("total_s", f64)]
 
@@ -515,6 +521,14 @@ class ResidentMatrix:
         n = u32()
         _check(hip_lib().mlem_matrix_build_transpose_limited(self.h, scratch_bytes, C.byref(n)))
         return n.value
+
+    def autotune(self, per_pass: int = 1, values: int = 0) -> dict:
+        """The fastest blocked-kernel tiling for this matrix (mlem_autotune): tile_width, tile_rows, tile_groups
+        for opts() / Batch, plus the candidates timed and the chosen one's FP and BP ms per launch."""
+        r = TuneResult()
+        _check(hip_lib().mlem_autotune(self.h, per_pass, values, C.byref(r)))
+        return dict(tile_width=r.tile_width, tile_rows=r.tile_rows, tile_groups=r.tile_groups,
+                    candidates=r.candidates, fp_ms=r.fp_ms, bp_ms=r.bp_ms)
 
     def info(self):
         r, c, n, t, b = u32(), u32(), u64(), cint(), u64()

@@ -137,6 +137,27 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label):
         b.sync()
         M.device_sync()
         elapsed = time.perf_counter() - t0
+    # Accuracy: the first few sinograms reconstructed in full (recon_iters iterations) by the batch and by the
+    # one-at-a-time FP32 path (itself within 1e-5 rel-L2 of the FP64 oracle), compared per sinogram.
+    errs = []
+    K = min(B, args.batch_check)
+    if K:
+        with M.Batch(m, K, values=VALUES[args.values], per_pass=args.per_pass,
+                     tile_width=args.tile if args.tile > 1 else 0, tile_rows=args.tile_rows,
+                     tile_groups=args.tile_groups) as b2:
+            for k in range(K):
+                b2.set(k, sinos[k], f0)
+            b2.iterate(args.recon_iters)
+            b2.sync()
+            imgs = [b2.image(k) for k in range(K)]
+        for k in range(K):
+            with M.Recon(m, sinos[k], f0, sens=s32, options=M.opts(spmv=0, using_trans=0)) as r:
+                r.iterate(args.recon_iters)
+                r.sync()
+                ref = r.image().astype(np.float64)
+            errs.append(float(np.linalg.norm(imgs[k] - ref) / np.linalg.norm(ref)))
+        log(f"rel-L2 of the batched images vs the one-at-a-time FP32 images after {args.recon_iters} iterations: "
+            + ", ".join(f"{e:.2e}" for e in errs))
     t_iter = elapsed / args.steps
     recon_s = B / (args.recon_iters * t_iter)
     single_recon_s = 1.0 / (args.recon_iters * t_single)
@@ -153,6 +174,9 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label):
         "one_at_a_time_f32": {"ms_per_iteration": round(t_single * 1e3, 4), "recon_s": round(single_recon_s, 3)},
         "speedup_vs_one_at_a_time_f32": round(recon_s / single_recon_s, 3),
     }
+    if errs:
+        out["rel_l2_vs_one_at_a_time_f32"] = {"iterations": args.recon_iters, "per_sinogram": [round(e, 8) for e in errs],
+                                              "max": round(max(errs), 8)}
     print(json.dumps(out), flush=True)
     m.close()
 
@@ -303,6 +327,8 @@ def main():
     ap.add_argument("--tile-rows", type=int, default=0, help="rows per tile block (0: default)")
     ap.add_argument("--tile-groups", type=int, default=0, help="workgroups per blocked SpMV (0: default)")
     ap.add_argument("--graph", type=int, default=0)
+    ap.add_argument("--autotune", action="store_true",
+                    help="time the candidate tile shapes and workgroup counts first (mlem_autotune) and use the best")
     ap.add_argument("--threads", type=int, default=0, help="host threads (0: all)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
@@ -312,6 +338,8 @@ def main():
     ap.add_argument("--values", default=None, choices=["f32", "bf16", "fp16"],
                     help="stored matrix values (default: f32 for one reconstruction, fp16 for --batch and --serve)")
     ap.add_argument("--recon-iters", type=int, default=100, help="iterations per reconstruction (recon/s)")
+    ap.add_argument("--batch-check", type=int, default=4,
+                    help="batched mode: sinograms whose full reconstructions are compared with the FP32 path")
     ap.add_argument("--serve", default="",
                     help="serving mode over these workloads (comma list of c1, mid, c2): continuous batching")
     ap.add_argument("--serve-requests", type=int, default=48, help="serving mode: number of requests")
@@ -362,6 +390,13 @@ def main():
     s32, _ = m.sensitivity()
     log(f"FP64 sensitivity {time.time() - t:.2f} s")
     f0, init = M.initial_image(g, A.rows, s32)
+    if args.autotune and args.tile != 1:
+        t = time.time()
+        tune = m.autotune(per_pass=args.per_pass if args.batch else 1,
+                          values=VALUES[args.values] if args.batch else 0)
+        args.tile, args.tile_rows, args.tile_groups = tune["tile_width"], tune["tile_rows"], tune["tile_groups"]
+        log(f"autotune {time.time() - t:.1f} s over {tune['candidates']} candidates: tiles {args.tile} x "
+            f"{args.tile_rows}, {args.tile_groups} workgroups (FP {tune['fp_ms']:.3f} ms, BP {tune['bp_ms']:.3f} ms)")
 
     if args.batch:
         return run_batch(args, M, geo, A, g, m, s32, threads, label)

@@ -199,8 +199,14 @@ typedef struct mlem_opts {
                              0 = 24 per CU                                   */
     uint32_t values;      /* stored matrix values of the blocked kernels:
                              0 = f32, 1 = bf16, 2 = fp16 (16-bit values are
-                             value / rowmax with an f32 scale per row; the
-                             products, sums, ratio and update stay f32)     */
+                             value / scale rounded to nearest, with an f32
+                             scale per row of A (the row's largest value, or
+                             2^-14 of it for fp16); the
+                             back projection uses the same codes, and the
+                             sensitivity is this quantized matrix's column
+                             sums in FP64 (a sensitivity passed in is not
+                             used); products, sums, ratio and update stay
+                             f32)                                            */
 } mlem_opts;
 
 /* Fill opts with the defaults: merge-based FP and BP (argv[8] = 0),
@@ -312,7 +318,8 @@ typedef struct mlem_batch_opts {
     uint32_t tile_groups; /* workgroups per blocked SpMV, 0 = automatic    */
 } mlem_batch_opts;
 void mlem_batch_opts_default(mlem_batch_opts *opts);
-/* The sensitivity (shared by all slots) is the matrix's FP64 column sums. */
+/* The sensitivity (shared by all slots) is the matrix's FP64 column sums
+ * (with 16-bit values: those of the quantized matrix). */
 int mlem_batch_create(mlem_matrix *m, uint32_t slots,
                       const mlem_batch_opts *opts, mlem_batch **out);
 /* Load a reconstruction into a slot (g int32[rows], f0 f32[cols]) and make
@@ -329,6 +336,24 @@ int mlem_batch_info(const mlem_batch *b, uint32_t *slots, uint32_t *per_pass,
                     uint32_t *passes, uint64_t *device_bytes);
 void *mlem_batch_stream(mlem_batch *b);
 int mlem_batch_destroy(mlem_batch *b);
+
+/* ------------------------------------------------------------------------ */
+/* 2b'. autotuner                                                           */
+/* ------------------------------------------------------------------------ */
+
+/* Times the blocked FP and BP kernels of the matrix over a few tile shapes
+ * (width x rows) and workgroup counts for `per_pass` right-hand sides (1, 4
+ * or 8) and `values` (0 f32, 1 bf16, 2 fp16), and reports the fastest for
+ * mlem_opts / mlem_batch_opts (tile_width, tile_rows, tile_groups).  Tiles
+ * built for candidates that were not chosen are freed again.  No counterpart
+ * in the reference (its section sizes come from argv[5]/[6]). */
+typedef struct mlem_tune_result {
+    uint32_t tile_width, tile_rows, tile_groups;
+    uint32_t candidates;  /* shape x workgroup-count pairs timed            */
+    double fp_ms, bp_ms;  /* per launch of the chosen one, partials + reduce */
+} mlem_tune_result;
+int mlem_autotune(mlem_matrix *m, uint32_t per_pass, uint32_t values,
+                  mlem_tune_result *out);
 
 /* ------------------------------------------------------------------------ */
 /* 2c. serving: continuous batching over resident matrices                  */

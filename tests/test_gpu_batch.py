@@ -148,3 +148,25 @@ def test_batch_edge_matrices(per_pass, width, rows, groups):
                     ref = O.mlem_f64(A.rowptr, A.col, A.val, A.rows, A.cols, sinos[k], s32.astype(np.float64), f0, 8)
                     img = b.image(k)
                     assert np.linalg.norm(ref) == 0 and np.all(img == 0) or rel_l2(img, ref) <= 1e-5, (kind, k)
+
+
+@pytest.mark.parametrize("per_pass,values", [(1, 0), (4, 2)])
+def test_autotune_choice_reconstructs(setup, per_pass, values):
+    """The autotuner returns a shape among its candidates; reconstructions with it meet the bars."""
+    t = setup["m"].autotune(per_pass=per_pass, values=values)
+    print(f"autotune per_pass={per_pass} values={values}: {t}", flush=True)
+    assert t["candidates"] >= 2 and t["fp_ms"] > 0 and t["bp_ms"] > 0
+    assert t["tile_width"] in (512, 1024, 2048) and t["tile_rows"] in (512, 1024, 2048, 4096)
+    if per_pass == 1:
+        o = M.opts(spmv=0, tile_width=t["tile_width"], tile_rows=t["tile_rows"], tile_groups=t["tile_groups"])
+        img = M.mlem_run(setup["m"], setup["sinos"][0], setup["f0"], ITERS, sens=setup["s32"], options=o)
+        assert rel_l2(img, setup["refs"][0]) <= BAR[0]
+    else:
+        with M.Batch(setup["m"], per_pass, values=values, per_pass=per_pass, tile_width=t["tile_width"],
+                     tile_rows=t["tile_rows"], tile_groups=t["tile_groups"]) as b:
+            for k in range(per_pass):
+                b.set(k, setup["sinos"][k], setup["f0"])
+            b.iterate(ITERS)
+            b.sync()
+            for k in range(per_pass):
+                assert rel_l2(b.image(k), setup["refs"][k]) <= BAR[values]
