@@ -3,9 +3,9 @@ resident matrix) against the FP64 oracle, on the 2D ring matrix.
 
 Bars: f32 values 1e-5 rel-L2; fp16 values 1e-3; bf16 values 2e-3 (8-bit
 significand: the stored matrix differs from the FP32 one by up to 2^-9 per
-value, rounded with per-row error feedback; the products, sums, ratio and
-update stay FP32); a slot's image is bitwise the same alone and in a full
-batch."""
+value, each value rounded to nearest after the per-row scale; the products,
+sums, ratio and update stay FP32); a slot's image is bitwise the same alone
+and in a full batch."""
 import numpy as np
 import pytest
 
