@@ -190,6 +190,7 @@ struct IterEvents {
 struct mlem_batch {
     mlem_matrix *m = nullptr;
     uint32_t slots = 0, BC = 4, passes = 0;
+    uint32_t nt = 0;                // work-items per workgroup of the stream kernel (0: its default)
     size_t rows_x = 0, cols_x = 0;  // padded vector lengths
     hipStream_t stream = nullptr;
     // [passes][rows_x][BC] counts and ratios, [passes][cols_x][BC] images
@@ -341,6 +342,7 @@ int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t target, const TileWorkDev *
         HIPCHK(hipMemcpy(off.data(), t->view.tile_off, 8 * (p.ntiles + 1), hipMemcpyDeviceToHost));
     }
     const TileWorkList L = tile_work_list(p, nz.data(), off.data(), target);
+    if (L.bad) return set_err(MLEM_ERR_FORMAT, "tiles (width %u, rows %u) are not aligned for the stream kernel", p.W, p.R);
     auto w = std::make_unique<TileWorkDev>();
     const size_t b_wl = 4 * L.wl.size(), b_tl = 4 * L.tiles.size(), b_first = 4 * L.wl_first.size();
     const size_t o_tl = (b_wl + 255) & ~size_t(255), o_first = o_tl + ((b_tl + 255) & ~size_t(255));
@@ -430,7 +432,7 @@ hipError_t enqueue_iteration(mlem_recon *r, IterEvents *ev) {
     fa.g = r->g.as<int32_t>();
     if (r->tfp) {
         if ((e = launch_tiled_partials(r->tfp->plan, r->tfp->view, r->wfp->view, r->f.as<float>(),
-                                       r->part_fp.as<float>(), 1, s)) != hipSuccess)
+                                       r->part_fp.as<float>(), 1, 0, s)) != hipSuccess)
             return e;
         if ((e = mark(1)) != hipSuccess) return e;
         e = launch_tiled_reduce(r->tfp->plan, r->tfp->view, r->wfp->view, r->part_fp.as<float>(), 1, EPI_RATIO, fa, s);
@@ -452,7 +454,7 @@ hipError_t enqueue_iteration(mlem_recon *r, IterEvents *ev) {
     if (r->opts.using_trans == 0) {
         if (r->tbp) {
             if ((e = launch_tiled_partials(r->tbp->plan, r->tbp->view, r->wbp->view, r->ratio.as<float>(),
-                                           r->part_bp.as<float>(), 1, s)) != hipSuccess)
+                                           r->part_bp.as<float>(), 1, 0, s)) != hipSuccess)
                 return e;
             if ((e = mark(3)) != hipSuccess) return e;
             e = launch_tiled_reduce(r->tbp->plan, r->tbp->view, r->wbp->view, r->part_bp.as<float>(), 1, EPI_UPDATE,
@@ -1300,6 +1302,8 @@ int mlem_autotune(mlem_matrix *m, uint32_t per_pass, uint32_t values, mlem_tune_
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
     std::vector<std::pair<uint32_t, uint32_t>> shapes;  // (W, R)
     std::vector<uint32_t> groups;                       // per CU
+    std::vector<uint32_t> threads = {0};                // work-items per workgroup (several right-hand sides)
+    if (BC > 1) threads = {256, 512};  // 128 was slower at every shape timed on the 3D scanner matrix
     if (BC == 1) {
         shapes = {{1024, 1024}, {1024, 2048}, {1024, 4096}, {2048, 1024}, {2048, 2048}};
         groups = {16, 24, 32};
@@ -1341,15 +1345,16 @@ int mlem_autotune(mlem_matrix *m, uint32_t per_pass, uint32_t values, mlem_tune_
         return set_err(MLEM_ERR_HIP, "mlem_autotune: fill failed");
     }
     double best = 1e300;
-    uint32_t bestW = 0, bestR = 0, bestG = 0, tried = 0;
+    uint32_t bestW = 0, bestR = 0, bestG = 0, bestT = 0, tried = 0;
     double best_fp = 0, best_bp = 0;
     // One side's time per launch (partials + reduce), mean of three after a warm-up.
-    auto time_side = [&](const Tiled *t, const TileWorkDev *w, DevBuf &part, const float *x, float *y, double *ms) {
+    auto time_side = [&](const Tiled *t, const TileWorkDev *w, DevBuf &part, const float *x, float *y, uint32_t nt,
+                         double *ms) {
         EpiArgs ea{};
         ea.out = y;
         for (int rep = 0; rep < 4; ++rep) {
             if (rep == 1 && hipEventRecord(e0, s) != hipSuccess) return hipErrorUnknown;
-            hipError_t e = launch_tiled_partials(t->plan, t->view, w->view, x, part.as<float>(), BC, s);
+            hipError_t e = launch_tiled_partials(t->plan, t->view, w->view, x, part.as<float>(), BC, nt, s);
             if (e == hipSuccess) e = launch_tiled_reduce(t->plan, t->view, w->view, part.as<float>(), BC, EPI_PLAIN, ea, s);
             if (e != hipSuccess) return e;
         }
@@ -1375,21 +1380,24 @@ int mlem_autotune(mlem_matrix *m, uint32_t per_pass, uint32_t values, mlem_tune_
                 cleanup();
                 return st;
             }
-            double fp = 0, bp = 0;
-            hipError_t e = time_side(tf, wf, pf, xf.as<float>(), yf.as<float>(), &fp);
-            if (e == hipSuccess) e = time_side(tb, wb, pb, xb.as<float>(), yb.as<float>(), &bp);
-            if (e != hipSuccess) {
-                cleanup();
-                return hip_err(e, "mlem_autotune: timing", __LINE__);
-            }
-            ++tried;
-            if (fp + bp < best) {
-                best = fp + bp;
-                bestW = sh.first;
-                bestR = sh.second;
-                bestG = g;
-                best_fp = fp;
-                best_bp = bp;
+            for (uint32_t nt : threads) {
+                double fp = 0, bp = 0;
+                hipError_t e = time_side(tf, wf, pf, xf.as<float>(), yf.as<float>(), nt, &fp);
+                if (e == hipSuccess) e = time_side(tb, wb, pb, xb.as<float>(), yb.as<float>(), nt, &bp);
+                if (e != hipSuccess) {
+                    cleanup();
+                    return hip_err(e, "mlem_autotune: timing", __LINE__);
+                }
+                ++tried;
+                if (fp + bp < best) {
+                    best = fp + bp;
+                    bestW = sh.first;
+                    bestR = sh.second;
+                    bestG = g;
+                    bestT = nt;
+                    best_fp = fp;
+                    best_bp = bp;
+                }
             }
         }
     }
@@ -1407,6 +1415,7 @@ int mlem_autotune(mlem_matrix *m, uint32_t per_pass, uint32_t values, mlem_tune_
     out->tile_width = bestW;
     out->tile_rows = bestR;
     out->tile_groups = bestG * (uint32_t)cus;
+    out->tile_threads = bestT;
     out->candidates = tried;
     out->fp_ms = best_fp;
     out->bp_ms = best_bp;
@@ -1437,11 +1446,13 @@ int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opt
     const uint32_t R = o.tile_rows ? o.tile_rows : (BC == 8 ? 1024u : 2048u);
     ARGCHK(valid_tile_config(W, BC), "mlem_batch_create: no blocked kernel for width %u with %u per pass", W, BC);
     ARGCHK(valid_tile_rows(R), "mlem_batch_create: tile_rows must be 512 ... 16384 (a power of two)");
+    ARGCHK(valid_stream_threads(o.tile_threads), "mlem_batch_create: tile_threads must be 0, 128, 256 or 512");
     HIPCHK(hipSetDevice(m->device));
     auto b = std::make_unique<mlem_batch>();
     b->m = m;
     b->slots = slots;
     b->BC = BC;
+    b->nt = o.tile_threads;
     b->passes = (slots + BC - 1) / BC;
     b->rows_x = (m->rows + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
     b->cols_x = (m->cols + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
@@ -1518,7 +1529,8 @@ int mlem_batch_iterate(mlem_batch *b, uint32_t iters) {
             EpiArgs fa{};
             fa.out = ratio;
             fa.g = b->g.as<int32_t>() + (size_t)p * b->rows_x * BC;
-            HIPCHK(launch_tiled_partials(b->tfp->plan, b->tfp->view, b->wfp->view, f, b->part_fp.as<float>(), BC, s));
+            HIPCHK(launch_tiled_partials(b->tfp->plan, b->tfp->view, b->wfp->view, f, b->part_fp.as<float>(), BC,
+                                         b->nt, s));
             HIPCHK(launch_tiled_reduce(b->tfp->plan, b->tfp->view, b->wfp->view, b->part_fp.as<float>(), BC, EPI_RATIO,
                                        fa, s));
             EpiArgs ba{};
@@ -1526,7 +1538,7 @@ int mlem_batch_iterate(mlem_batch *b, uint32_t iters) {
             ba.f = f;
             ba.active = b->active.as<uint32_t>() + (size_t)p * BC;
             HIPCHK(launch_tiled_partials(b->tbp->plan, b->tbp->view, b->wbp->view, ratio, b->part_bp.as<float>(), BC,
-                                         s));
+                                         b->nt, s));
             HIPCHK(launch_tiled_reduce(b->tbp->plan, b->tbp->view, b->wbp->view, b->part_bp.as<float>(), BC,
                                        EPI_UPDATE, ba, s));
         }

@@ -126,6 +126,7 @@ struct TiledMatrix {
 // row block b (nblk+1 entries).
 struct TileWorkList {
     uint32_t span = 1, nwg = 0, ntl = 0;
+    uint32_t bad = 0;  // quad tiles not aligned as the stream kernel needs
     std::vector<uint32_t> wl, tiles, wl_first;
 };
 // The same on the device.
@@ -170,8 +171,11 @@ TileWorkList tile_work_list(const TilePlan &p, const uint32_t *tile_nz, const ui
 // reduction with the row scales and the epilogue.  Vectors are [n][BC]
 // (right-hand side fastest); X must be readable (and finite) up to
 // nseg * W rows: callers pad their vectors to whole segments (kTileXPad).
+// nt: work-items per workgroup of the kernel for several right-hand sides
+// (128, 256 or 512; 0 = 512), each taking 512 / nt groups of a window.
 hipError_t launch_tiled_partials(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x,
-                                 float *part, uint32_t BC, hipStream_t s);
+                                 float *part, uint32_t BC, uint32_t nt, hipStream_t s);
+bool valid_stream_threads(uint32_t nt);
 hipError_t launch_tiled_reduce(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *part,
                                uint32_t BC, Epi epi, const EpiArgs &ea, hipStream_t s);
 

@@ -75,7 +75,8 @@ __device__ __forceinline__ T ldnt(const T *p) {
 }
 
 constexpr uint32_t kE = kTileE;          // entries per work-item of a unit
-constexpr uint32_t kSNT = 512;           // work-items of a stream_spmv_kernel workgroup (groups per window)
+constexpr uint32_t kSNT = 512;           // groups per window of stream_spmv_kernel
+constexpr uint32_t kQuadAlign = 4;       // quad tiles are padded to a multiple of this many groups
 constexpr uint32_t kRunWindows = 64;     // windows of a stream run at most
 constexpr uint32_t kUE = kTileUnit;      // entries per unit
 constexpr uint32_t kNoKey = 0x10000u;    // past every u16 row key
@@ -347,11 +348,15 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
 }
 
 // Several right-hand sides: quad groups in block-stream order (the groups of
-// a row block's tiles one after another, values then group words).  A window
-// is kSNT consecutive groups of the stream, one per work-item, from at most
-// two tiles (A, then B from group `split` on); the x segments of the tiles
-// sit in two LDS slots by the parity of the tile's place in the block's tile
-// list, so a window does not stop at a tile boundary.
+// a row block's tiles one after another, values then group words; every
+// tile padded to a multiple of kQuadAlign groups).  A window is kSNT
+// consecutive groups of the stream from at most two tiles (A, then B from
+// group `split` on); work-item t of an NT-wide workgroup takes the GPW =
+// kSNT / NT consecutive groups [t*GPW, t*GPW + GPW) of it, all in one tile
+// (window starts, ends and splits are multiples of kQuadAlign groups).  The
+// x segments of the tiles sit in two LDS slots by the parity of the tile's
+// place in the block's tile list, so a window does not stop at a tile
+// boundary.
 // wl[w] = {row block, first window, windows}; win[i] = {first group,
 // groups | split << 16 | slot of A << 31, segment of A, segment of B}.
 
@@ -376,68 +381,77 @@ __device__ __forceinline__ void seg_scan_step(float (&v)[BC], uint32_t &f) {
     f |= nf;
 }
 
-template <int W, int VT, int BC>
-__global__ __launch_bounds__(kSNT) void stream_spmv_kernel(uint32_t R, uint32_t nseg,
-                                                           const uint64_t *__restrict__ tile_off,
-                                                           const uint4 *__restrict__ wl,
-                                                           const uint4 *__restrict__ win,
-                                                           const uint8_t *__restrict__ tdata,
-                                                           const float *__restrict__ x, float *__restrict__ part) {
+template <int W, int VT, int BC, int NT>
+__global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t nseg,
+                                                         const uint64_t *__restrict__ tile_off,
+                                                         const uint4 *__restrict__ wl,
+                                                         const uint4 *__restrict__ win,
+                                                         const uint8_t *__restrict__ tdata,
+                                                         const float *__restrict__ x, float *__restrict__ part) {
+    constexpr int GPW = kSNT / NT;             // groups per work-item and window
+    constexpr int NQ = 2 * GPW;                // quads per work-item
     constexpr int VB = VT == VAL_F32 ? 4 : 2;  // bytes per stored value
-    constexpr int XF = W * BC / kSNT;          // floats of an x segment per work-item
+    constexpr int XF = W * BC / NT;            // floats of an x segment per work-item
     constexpr int CB = col_bits<W>();
     constexpr uint32_t XS = (W + 4) * BC;      // floats per x slot (column W holds 0)
-    constexpr int NW = kSNT / 64;              // waves
+    constexpr int NW = NT / 64;                // waves
+    static_assert(kSNT % NT == 0 && kQuadAlign % GPW == 0, "a work-item's groups stay in one tile");
     static_assert(BC % 4 == 0 && XF % 4 == 0, "x segments move in 16-byte pieces");
     extern __shared__ __attribute__((aligned(16))) float smem[];
-    __shared__ uint32_t s_k0[kSNT + 1];                         // first key of every work-item
+    __shared__ uint32_t s_k0[NT + 1];                           // first key of every work-item
     __shared__ __attribute__((aligned(16))) float s_wt[NW][BC];  // carry out of every wave
-    __shared__ __attribute__((aligned(16))) uint32_t s_wh[NW];  // the wave ends a row
+    __shared__ uint32_t s_wh[NW];                               // the wave ends a row
     __shared__ uint4 s_win[kRunWindows];                        // the run's windows
     float *xs = smem;                                           // [2][XS]
     float *acc = xs + 2 * XS;                                   // [R * BC]
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
+    const uint32_t g0 = tid * GPW;  // the work-item's first group in a window
     const uint4 w = wl[blockIdx.x];
     const uint64_t e0 = tile_off[(uint64_t)w.x * nseg], e1 = tile_off[(uint64_t)(w.x + 1) * nseg];
     const uint64_t G = (e1 - e0) / 8;  // groups of the row block
     const uint8_t *vbase = tdata + (uint64_t)(VB + 2) * e0;
     const uint8_t *gbase = vbase + (uint64_t)VB * 8 * G;
-    for (uint32_t i = tid; i < R * BC; i += kSNT) acc[i] = 0.0f;
-    for (uint32_t i = tid; i < w.z; i += kSNT) s_win[i] = win[w.y + i];
+    for (uint32_t i = tid; i < R * BC; i += NT) acc[i] = 0.0f;
+    for (uint32_t i = tid; i < w.z; i += NT) s_win[i] = win[w.y + i];
     if (tid < BC) {
         xs[W * BC + tid] = 0.0f;
         xs[XS + W * BC + tid] = 0.0f;
     }
-    if (tid == 0) s_k0[kSNT] = kNoKey;
+    if (tid == 0) s_k0[NT] = kNoKey;
 
     uint32_t staged[2] = {~0u, ~0u};  // segment held by each x slot
     // Loads of window c (its groups; the x segments of its tiles that are
     // not staged yet: needA / needB).  Work-items past the window's end
     // re-read its first group.
-    f32x4 pv0 = {}, pv1 = {};
-    u32x4 pq = {}, pg = {};
+    f32x4 pv0[GPW], pv1[GPW];
+    u32x4 pq[GPW], pg[GPW];
     f32x4 pxa[XF / 4], pxb[XF / 4];
     auto load = [&](const uint4 &c, bool needA, bool needB) {
         const uint32_t ng = c.y & 0x3ffu;
-        const uint64_t g = c.x + (tid < ng ? tid : 0u);
-        if constexpr (VT == VAL_F32) {
-            pv0 = ldnt(reinterpret_cast<const f32x4 *>(vbase) + g);
-            pv1 = ldnt(reinterpret_cast<const f32x4 *>(vbase + 16 * G) + g);
-        } else {
-            pq = ldnt(reinterpret_cast<const u32x4 *>(vbase) + g);
+        const bool valid = g0 < ng;
+        const uint64_t gb = c.x + (valid ? g0 : 0u);
+#pragma unroll
+        for (int j = 0; j < GPW; ++j) {
+            const uint64_t g = gb + (valid ? j : 0);
+            if constexpr (VT == VAL_F32) {
+                pv0[j] = ldnt(reinterpret_cast<const f32x4 *>(vbase) + g);
+                pv1[j] = ldnt(reinterpret_cast<const f32x4 *>(vbase + 16 * G) + g);
+            } else {
+                pq[j] = ldnt(reinterpret_cast<const u32x4 *>(vbase) + g);
+            }
+            pg[j] = ldnt(reinterpret_cast<const u32x4 *>(gbase) + g);
         }
-        pg = ldnt(reinterpret_cast<const u32x4 *>(gbase) + g);
         // x is padded to whole segments (xlen >= nseg * W): no clamping.
         if (needA) {
             const f32x4 *xa = reinterpret_cast<const f32x4 *>(x + (uint64_t)c.z * W * BC);
 #pragma unroll
-            for (int q = 0; q < XF / 4; ++q) pxa[q] = xa[tid + kSNT * q];
+            for (int q = 0; q < XF / 4; ++q) pxa[q] = xa[tid + NT * q];
         }
         if (needB) {
             const f32x4 *xb = reinterpret_cast<const f32x4 *>(x + (uint64_t)c.w * W * BC);
 #pragma unroll
-            for (int q = 0; q < XF / 4; ++q) pxb[q] = xb[tid + kSNT * q];
+            for (int q = 0; q < XF / 4; ++q) pxb[q] = xb[tid + NT * q];
         }
     };
     auto uniform4 = [](const uint4 &t) {
@@ -451,25 +465,40 @@ __global__ __launch_bounds__(kSNT) void stream_spmv_kernel(uint32_t R, uint32_t 
 
     for (uint32_t i = 0; i < w.z; ++i) {
         const uint32_t ng = cw.y & 0x3ffu, split = (cw.y >> 16) & 0x3ffu, slotA = cw.y >> 31;
-        const f32x4 cv0 = pv0, cv1 = pv1;
-        const u32x4 cq = pq, cg = pg;
+        f32x4 cv0[GPW], cv1[GPW];
+        u32x4 cq[GPW], cg[GPW];
+#pragma unroll
+        for (int j = 0; j < GPW; ++j) {
+            cv0[j] = pv0[j];
+            cv1[j] = pv1[j];
+            cq[j] = pq[j];
+            cg[j] = pg[j];
+        }
         const uint4 nw = uniform4(s_win[i + 1 < w.z ? i + 1 : i]);
         // Stage the new x segments (the slots' previous tiles are done: the
         // last barrier of the previous window lies between).
         if (needA) {
 #pragma unroll
-            for (int q = 0; q < XF / 4; ++q) *(reinterpret_cast<f32x4 *>(xs + slotA * XS) + tid + kSNT * q) = pxa[q];
+            for (int q = 0; q < XF / 4; ++q) *(reinterpret_cast<f32x4 *>(xs + slotA * XS) + tid + NT * q) = pxa[q];
             staged[slotA] = cw.z;
         }
         if (needB) {
 #pragma unroll
             for (int q = 0; q < XF / 4; ++q)
-                *(reinterpret_cast<f32x4 *>(xs + (slotA ^ 1u) * XS) + tid + kSNT * q) = pxb[q];
+                *(reinterpret_cast<f32x4 *>(xs + (slotA ^ 1u) * XS) + tid + NT * q) = pxb[q];
             staged[slotA ^ 1u] = cw.w;
         }
-        const bool lane_valid = tid < ng;
-        const uint32_t kA = cg.x & 0xffffu, kB = cg.x >> 16;
-        s_k0[tid] = lane_valid ? kA : kNoKey;
+        const bool lane_valid = g0 < ng;
+        const bool inA = g0 < split;
+        // Quad 2j holds entries 0-3 of group j (row key bits 0-15), quad
+        // 2j+1 entries 4-7 (bits 16-31); keys do not decrease inside a tile.
+        uint32_t kq[NQ];
+#pragma unroll
+        for (int j = 0; j < GPW; ++j) {
+            kq[2 * j] = cg[j].x & 0xffffu;
+            kq[2 * j + 1] = cg[j].x >> 16;
+        }
+        s_k0[tid] = lane_valid ? kq[0] : kNoKey;
         // The next window's loads, in flight while this one is computed.
         if (i + 1 < w.z) {
             const uint32_t sa = nw.y >> 31;
@@ -479,48 +508,58 @@ __global__ __launch_bounds__(kSNT) void stream_spmv_kernel(uint32_t R, uint32_t 
             cw = nw;
         }
         __syncthreads();
-        const float *xw = xs + (tid < split ? slotA : slotA ^ 1u) * XS;
-        // Quads: entries 0-3 belong to row kA, 4-7 to row kB >= kA.  Each
-        // quad is summed on its own (padding reads column W, whose slot
-        // holds 0); work-items past the window's end skip the sums.  A row
-        // ends after quad A where kB != kA and after quad B where the next
-        // work-item starts another row.
-        float sq[2][BC];
+        const float *xw = xs + (inA ? slotA : slotA ^ 1u) * XS;
+        // Every quad is summed on its own (padding reads column W, whose
+        // slot holds 0); work-items past the window's end skip the sums.
+        float sq[NQ][BC];
 #pragma unroll
-        for (int h = 0; h < 2; ++h)
+        for (int q = 0; q < NQ; ++q)
 #pragma unroll
-            for (int c = 0; c < BC; ++c) sq[h][c] = 0.0f;
+            for (int c = 0; c < BC; ++c) sq[q][c] = 0.0f;
         if (lane_valid) {
 #pragma unroll
-            for (int h = 0; h < 2; ++h) {
+            for (int q = 0; q < NQ; ++q) {
 #pragma unroll
-                for (int j = 0; j < 4; ++j) {
-                    const int m = 4 * h + j;
-                    const float v = group_val<VT>(cv0, cv1, cq, m);
-                    const float *xg = xw + group_col<CB>(cg, m) * BC;
+                for (int jj = 0; jj < 4; ++jj) {
+                    const int m = 4 * (q & 1) + jj;
+                    const float v = group_val<VT>(cv0[q / 2], cv1[q / 2], cq[q / 2], m);
+                    const float *xg = xw + group_col<CB>(cg[q / 2], m) * BC;
 #pragma unroll
-                    for (int q = 0; q < BC / 4; ++q) {
-                        const f32x4 t4 = *reinterpret_cast<const f32x4 *>(xg + 4 * q);
+                    for (int qq = 0; qq < BC / 4; ++qq) {
+                        const f32x4 t4 = *reinterpret_cast<const f32x4 *>(xg + 4 * qq);
 #pragma unroll
                         for (int bb = 0; bb < 4; ++bb)
-                            sq[h][4 * q + bb] = j == 0 ? v * t4[bb] : fmaf(v, t4[bb], sq[h][4 * q + bb]);
+                            sq[q][4 * qq + bb] = jj == 0 ? v * t4[bb] : fmaf(v, t4[bb], sq[q][4 * qq + bb]);
                     }
                 }
             }
         }
-        const bool endA = lane_valid && kA != kB;
-        const bool endB = lane_valid && kB != s_k0[tid + 1];
-        const bool ends = endA || endB;
+        // A row ends after quad q where the next quad (the next work-item's
+        // first, after the last) holds another row.  sq[q] becomes the sum
+        // of the quads since the work-item's previous row end.
+        const uint32_t knext = s_k0[tid + 1];
+        bool endq[NQ];
+        bool ends = false;
+#pragma unroll
+        for (int q = 0; q < NQ; ++q) {
+            endq[q] = lane_valid && kq[q] != (q + 1 < NQ ? kq[q + 1] : knext);
+            ends = ends || endq[q];
+        }
+#pragma unroll
+        for (int q = 1; q < NQ; ++q)
+#pragma unroll
+            for (int c = 0; c < BC; ++c) sq[q][c] = endq[q - 1] ? sq[q][c] : sq[q - 1][c] + sq[q][c];
         // Carry out: the part of the last row that continues in the next
-        // work-item.  The carry in of row kA is the sum of the carries of
-        // the work-items since the last one that ends a row (that one's
-        // included), or since the window's start: a segmented scan over the
-        // carries shifted by one work-item, in registers across the wave,
-        // then across the waves from their totals in LDS.
+        // work-item.  The carry in of the work-item's first row is the sum
+        // of the carries of the work-items since the last one that ends a
+        // row (that one's included), or since the window's start: a
+        // segmented scan over the carries shifted by one work-item, in
+        // registers across the wave, then across the waves from their
+        // totals in LDS.
         float a[BC], s[BC];
         uint32_t f;
 #pragma unroll
-        for (int b = 0; b < BC; ++b) a[b] = endB ? 0.0f : (endA ? sq[1][b] : sq[0][b] + sq[1][b]);
+        for (int b = 0; b < BC; ++b) a[b] = endq[NQ - 1] ? 0.0f : sq[NQ - 1][b];
 #pragma unroll
         for (int b = 0; b < BC; ++b) s[b] = __uint_as_float(dpp_get<0x138, 0xf>(__float_as_uint(a[b])));  // wave_shr:1
         f = dpp_get<0x138, 0xf>(ends ? 1u : 0u);
@@ -536,63 +575,50 @@ __global__ __launch_bounds__(kSNT) void stream_spmv_kernel(uint32_t R, uint32_t 
             s_wh[wv] = (f || ends) ? 1u : 0u;
         }
         __syncthreads();
-        // Every row that ends here is added to the accumulator once: kA
-        // with the carry in, kB (when a second row) on its own.  A row can
-        // end twice in a window that spans two tiles (once in each): the
-        // work-items of tile A update the accumulator first, a barrier, then
-        // those of tile B.
+        // Every row that ends here is added to the accumulator once, the
+        // work-item's first with the carry in.  A row can end twice in a
+        // window that spans two tiles (once in each): the work-items of
+        // tile A update the accumulator first, a barrier, then those of
+        // tile B.
         auto flush = [&]() {
-            const uint32_t r0 = (endA ? kA : kB) * BC, r1 = kB * BC;
-            f32x4 o0[BC / 4], o1[BC / 4];
+            float c[BC];
 #pragma unroll
-            for (int q = 0; q < BC / 4; ++q) {
-                o0[q] = *reinterpret_cast<const f32x4 *>(acc + r0 + 4 * q);
-                o1[q] = *reinterpret_cast<const f32x4 *>(acc + r1 + 4 * q);
-            }
-            float cin[BC];
-#pragma unroll
-            for (int b = 0; b < BC; ++b) cin[b] = 0.0f;
+            for (int b = 0; b < BC; ++b) c[b] = s[b];
             if (!f) {  // the row started before this wave: the earlier waves' carries
-                const u32x4 h0 = *reinterpret_cast<const u32x4 *>(s_wh);
-                const u32x4 h1 = NW > 4 ? *reinterpret_cast<const u32x4 *>(s_wh + 4) : u32x4{};
 #pragma unroll
                 for (int u = 0; u < NW - 1; ++u) {
                     if (u < (int)wv) {
-                        const uint32_t hu = u < 4 ? h0[u] : h1[u - 4];
+                        const uint32_t hu = s_wh[u];
 #pragma unroll
-                        for (int q = 0; q < BC / 4; ++q) {
-                            const f32x4 t4 = *reinterpret_cast<const f32x4 *>(&s_wt[u][4 * q]);
-#pragma unroll
-                            for (int bb = 0; bb < 4; ++bb) cin[4 * q + bb] = hu ? t4[bb] : cin[4 * q + bb] + t4[bb];
-                        }
+                        for (int b = 0; b < BC; ++b) c[b] = hu ? s_wt[u][b] + s[b] : c[b] + s_wt[u][b];
                     }
                 }
             }
 #pragma unroll
-            for (int q = 0; q < BC / 4; ++q) {
+            for (int q = 0; q < NQ; ++q) {
+                if (endq[q]) {
+                    const uint32_t r = kq[q] * BC;
 #pragma unroll
-                for (int bb = 0; bb < 4; ++bb) {
-                    const int c = 4 * q + bb;
-                    const float ci = f ? s[c] : cin[c] + s[c];
-                    o0[q][bb] += endA ? ci + sq[0][c] : (ci + sq[0][c]) + sq[1][c];
-                    o1[q][bb] += sq[1][c];
+                    for (int qq = 0; qq < BC / 4; ++qq) {
+                        f32x4 o = *reinterpret_cast<const f32x4 *>(acc + r + 4 * qq);
+#pragma unroll
+                        for (int bb = 0; bb < 4; ++bb) o[bb] += c[4 * qq + bb] + sq[q][4 * qq + bb];
+                        *reinterpret_cast<f32x4 *>(acc + r + 4 * qq) = o;
+                    }
+#pragma unroll
+                    for (int b = 0; b < BC; ++b) c[b] = 0.0f;
                 }
-                *reinterpret_cast<f32x4 *>(acc + r0 + 4 * q) = o0[q];
-            }
-            if (endA && endB) {
-#pragma unroll
-                for (int q = 0; q < BC / 4; ++q) *reinterpret_cast<f32x4 *>(acc + r1 + 4 * q) = o1[q];
             }
         };
-        if (ends && tid < split) flush();
+        if (ends && inA) flush();
         if (split < ng) {
             __syncthreads();
-            if (ends && tid >= split) flush();
+            if (ends && !inA) flush();
         }
     }
     __syncthreads();
     float *pw = part + (uint64_t)blockIdx.x * R * BC;
-    for (uint32_t i = tid; i < R * BC; i += kSNT) pw[i] = acc[i];
+    for (uint32_t i = tid; i < R * BC; i += NT) pw[i] = acc[i];
 }
 
 // Row r of block b = r / R sums the partials of the block's workgroups
@@ -662,7 +688,8 @@ __global__ __launch_bounds__(kWG) void tile_count_kernel(const uint64_t *__restr
 // in tile_padded.  Row steps: a row whose key exceeds the previous non-empty
 // row's by more than kMaxRowStep starts a new group of 8 (the rest of the
 // group is padding), so every key inside a group is its predecessor plus
-// 0..3.  Quads: every non-empty row starts at a multiple of 4.
+// 0..3.  Quads: every non-empty row starts at a multiple of 4, and the tile
+// is padded to a multiple of kQuadAlign groups.
 __global__ void tile_rowpos_kernel(const uint32_t *__restrict__ cnt, uint32_t rows, uint32_t R, uint32_t nseg,
                                    uint64_t ntiles, bool quad, uint32_t *__restrict__ rowpos,
                                    uint32_t *__restrict__ tile_nz, uint64_t *__restrict__ tile_padded) {
@@ -687,7 +714,8 @@ __global__ void tile_rowpos_kernel(const uint32_t *__restrict__ cnt, uint32_t ro
         run += n;
     }
     tile_nz[t] = run;
-    tile_padded[t] = (run + 7u) & ~7u;
+    const uint32_t align = quad ? 8 * kQuadAlign : 8;
+    tile_padded[t] = (run + align - 1) / align * align;
 }
 
 // Per row: exclusive prefix of its segment counts (in place).
@@ -799,8 +827,8 @@ __global__ __launch_bounds__(kWG) void tile_scatter_kernel(const uint64_t *__res
 // packs the group word (layout at the top of this file) into its unit.
 // Padding (rtmp = kPadRow) repeats the previous key with column W.  bit 4
 // of *bad: a step above kMaxRowStep, a key not below R or (quads) a quad
-// holding two rows or starting with padding (the layout rules all of them
-// out; the SpMV kernels rely on it).
+// holding two rows or starting with padding outside a tile's end padding
+// (the layout rules all of them out; the SpMV kernels rely on it).
 template <int W, bool Q>
 __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restrict__ ctmp,
                                                         const uint16_t *__restrict__ rtmp,
@@ -842,7 +870,15 @@ __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restri
 #pragma unroll
     for (int j = 0; j < 8; ++j) cols[j] = r[j] == kPadRow ? (uint32_t)W : c[j];
     if constexpr (Q) {
-        const uint32_t kA = r[0], kB = r[4] == kPadRow ? kA : r[4];
+        uint32_t kA = r[0], kB = r[4] == kPadRow ? kA : r[4];
+        if (kA == kPadRow) {
+            // A whole group of the padding at a tile's end: the key of the
+            // tile's last row (its last quad starts with an entry), so the
+            // padding extends that row.
+            uint64_t pq = L;
+            while (pq > tile_off[lo] && rtmp[pq - 4] == kPadRow) pq -= 4;
+            if (pq > tile_off[lo]) kA = kB = rtmp[pq - 4];
+        }
         big = (kA == kPadRow || kA >= R || kB >= R || kB < kA) ? 1u : 0u;
 #pragma unroll
         for (int j = 1; j < 8; ++j) big |= (r[j] != kPadRow && r[j] != (j < 4 ? kA : kB)) ? 1u : 0u;
@@ -910,23 +946,35 @@ hipError_t launch_k(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk
     return hipGetLastError();
 }
 
+template <int W, int VT, int BC, int NT>
+hipError_t launch_stream_nt(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
+                            hipStream_t s) {
+    hipLaunchKernelGGL((stream_spmv_kernel<W, VT, BC, NT>), dim3(wk.nwg), dim3(NT), stream_lds_bytes(W, p.R, BC), s,
+                       p.R, p.nseg, tm.tile_off, wk.wl, wk.tiles, tm.tdata, x, part);
+    return hipGetLastError();
+}
+
 template <int W, int VT, int BC>
 hipError_t launch_stream(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
-                         hipStream_t s) {
-    hipLaunchKernelGGL((stream_spmv_kernel<W, VT, BC>), dim3(wk.nwg), dim3(kSNT), stream_lds_bytes(W, p.R, BC), s, p.R,
-                       p.nseg, tm.tile_off, wk.wl, wk.tiles, tm.tdata, x, part);
-    return hipGetLastError();
+                         uint32_t nt, hipStream_t s) {
+    switch (nt) {
+        case 128: return launch_stream_nt<W, VT, BC, 128>(p, tm, wk, x, part, s);
+        case 256: return launch_stream_nt<W, VT, BC, 256>(p, tm, wk, x, part, s);
+        case 0:
+        case 512: return launch_stream_nt<W, VT, BC, 512>(p, tm, wk, x, part, s);
+        default: return hipErrorInvalidValue;
+    }
 }
 
 template <int VT>
 hipError_t launch_vt(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
-                     uint32_t BC, hipStream_t s) {
+                     uint32_t BC, uint32_t nt, hipStream_t s) {
     if (p.quad != (BC > 1)) return hipErrorInvalidValue;  // quads (block streams) exactly for several
     if (BC == 1 && p.W == 1024) return launch_k<1024, VT, 1>(p, tm, wk, x, part, s);
     if (BC == 1 && p.W == 2048) return launch_k<2048, VT, 1>(p, tm, wk, x, part, s);
-    if (BC == 4 && p.W == 512) return launch_stream<512, VT, 4>(p, tm, wk, x, part, s);
-    if (BC == 4 && p.W == 1024) return launch_stream<1024, VT, 4>(p, tm, wk, x, part, s);
-    if (BC == 8 && p.W == 512) return launch_stream<512, VT, 8>(p, tm, wk, x, part, s);
+    if (BC == 4 && p.W == 512) return launch_stream<512, VT, 4>(p, tm, wk, x, part, nt, s);
+    if (BC == 4 && p.W == 1024) return launch_stream<1024, VT, 4>(p, tm, wk, x, part, nt, s);
+    if (BC == 8 && p.W == 512) return launch_stream<512, VT, 8>(p, tm, wk, x, part, nt, s);
     return hipErrorInvalidValue;
 }
 
@@ -1023,6 +1071,7 @@ TileWorkList stream_work_list(const TilePlan &p, const uint64_t *off, uint32_t t
         segs.clear();
         for (uint32_t sg = 0; sg < p.nseg; ++sg) {
             const uint64_t t = (uint64_t)b * p.nseg + sg;
+            if (off[t] % (8 * kQuadAlign)) L.bad = 1;
             if (off[t + 1] == off[t]) continue;
             gs.push_back((off[t] - base) / 8);
             segs.push_back(sg);
@@ -1240,14 +1289,16 @@ hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uin
 }
 
 hipError_t launch_tiled_partials(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x,
-                                 float *part, uint32_t BC, hipStream_t s) {
+                                 float *part, uint32_t BC, uint32_t nt, hipStream_t s) {
     if (p.rows == 0 || wk.nwg == 0) return hipSuccess;
     switch (p.vt) {
-        case VAL_F32: return launch_vt<VAL_F32>(p, tm, wk, x, part, BC, s);
-        case VAL_BF16: return launch_vt<VAL_BF16>(p, tm, wk, x, part, BC, s);
-        default: return launch_vt<VAL_F16>(p, tm, wk, x, part, BC, s);
+        case VAL_F32: return launch_vt<VAL_F32>(p, tm, wk, x, part, BC, nt, s);
+        case VAL_BF16: return launch_vt<VAL_BF16>(p, tm, wk, x, part, BC, nt, s);
+        default: return launch_vt<VAL_F16>(p, tm, wk, x, part, BC, nt, s);
     }
 }
+
+bool valid_stream_threads(uint32_t nt) { return nt == 0 || nt == 128 || nt == 256 || nt == 512; }
 
 namespace {
 template <int BC>

@@ -343,17 +343,18 @@ def colsum_scatter_f64(rowptr, col, val, rows, cols, nnz, sens64=None, sens32=No
 
 # ---------------------------------------------------------------- handles
 class BatchOpts(C.Structure):
-    _fields_ = [("values", u32), ("per_pass", u32), ("tile_width", u32), ("tile_rows", u32), ("tile_groups", u32)]
+    _fields_ = [("values", u32), ("per_pass", u32), ("tile_width", u32), ("tile_rows", u32), ("tile_groups", u32),
+                ("tile_threads", u32)]
 
 
 class Batch:
     """Many reconstructions over one resident matrix (A and A^T), iterated together (mlem_batch_*)."""
 
     def __init__(self, matrix: "ResidentMatrix", slots: int, values: int = 0, per_pass: int = 4, tile_width: int = 0,
-                 tile_rows: int = 0, tile_groups: int = 0):
+                 tile_rows: int = 0, tile_groups: int = 0, tile_threads: int = 0):
         self.h = vp()
         self.matrix = matrix
-        o = BatchOpts(values, per_pass, tile_width, tile_rows, tile_groups)
+        o = BatchOpts(values, per_pass, tile_width, tile_rows, tile_groups, tile_threads)
         _check(hip_lib().mlem_batch_create(matrix.h, slots, C.byref(o), C.byref(self.h)))
         self.slots = slots
 
@@ -408,9 +409,9 @@ class Server:
     between two iterations and leave when their iteration count is reached."""
 
     def __init__(self, hbm_budget_bytes: int = 0, values: int = 0, per_pass: int = 4, tile_width: int = 0,
-                 tile_rows: int = 0, tile_groups: int = 0):
+                 tile_rows: int = 0, tile_groups: int = 0, tile_threads: int = 0):
         self.h = vp()
-        o = BatchOpts(values, per_pass, tile_width, tile_rows, tile_groups)
+        o = BatchOpts(values, per_pass, tile_width, tile_rows, tile_groups, tile_threads)
         _check(hip_lib().mlem_server_create(hbm_budget_bytes, C.byref(o), C.byref(self.h)))
         self.cols = []
         self.rows = []
@@ -471,8 +472,8 @@ class Server:
 
 
 class TuneResult(C.Structure):
-    _fields_ = [("tile_width", u32), ("tile_rows", u32), ("tile_groups", u32), ("candidates", u32),
-                ("fp_ms", f64), ("bp_ms", f64)]
+    _fields_ = [("tile_width", u32), ("tile_rows", u32), ("tile_groups", u32), ("tile_threads", u32),
+                ("candidates", u32), ("fp_ms", f64), ("bp_ms", f64)]
 
 
 class LoadStats(C.Structure):
@@ -524,11 +525,12 @@ class ResidentMatrix:
 
     def autotune(self, per_pass: int = 1, values: int = 0) -> dict:
         """The fastest blocked-kernel tiling for this matrix (mlem_autotune): tile_width, tile_rows, tile_groups
-        for opts() / Batch, plus the candidates timed and the chosen one's FP and BP ms per launch."""
+        for opts() / Batch (and tile_threads for Batch), plus the candidates timed and the chosen one's FP and BP
+        ms per launch."""
         r = TuneResult()
         _check(hip_lib().mlem_autotune(self.h, per_pass, values, C.byref(r)))
         return dict(tile_width=r.tile_width, tile_rows=r.tile_rows, tile_groups=r.tile_groups,
-                    candidates=r.candidates, fp_ms=r.fp_ms, bp_ms=r.bp_ms)
+                    tile_threads=r.tile_threads, candidates=r.candidates, fp_ms=r.fp_ms, bp_ms=r.bp_ms)
 
     def info(self):
         r, c, n, t, b = u32(), u32(), u64(), cint(), u64()

@@ -124,7 +124,7 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label):
     t = time.time()
     with M.Batch(m, B, values=VALUES[args.values], per_pass=args.per_pass,
                  tile_width=args.tile if args.tile > 1 else 0, tile_rows=args.tile_rows,
-                 tile_groups=args.tile_groups) as b:
+                 tile_groups=args.tile_groups, tile_threads=args.tile_threads) as b:
         for k in range(B):
             b.set(k, sinos[k], f0)
         info = b.info()
@@ -144,7 +144,7 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label):
     if K:
         with M.Batch(m, K, values=VALUES[args.values], per_pass=args.per_pass,
                      tile_width=args.tile if args.tile > 1 else 0, tile_rows=args.tile_rows,
-                     tile_groups=args.tile_groups) as b2:
+                     tile_groups=args.tile_groups, tile_threads=args.tile_threads) as b2:
             for k in range(K):
                 b2.set(k, sinos[k], f0)
             b2.iterate(args.recon_iters)
@@ -326,6 +326,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0, help="merge-path tiling: 0 auto, 1 off, else segment width")
     ap.add_argument("--tile-rows", type=int, default=0, help="rows per tile block (0: default)")
     ap.add_argument("--tile-groups", type=int, default=0, help="workgroups per blocked SpMV (0: default)")
+    ap.add_argument("--tile-threads", type=int, default=0,
+                    help="work-items per workgroup of the kernel for 4 or 8 per pass: 128, 256, 512 (0: default)")
     ap.add_argument("--graph", type=int, default=0)
     ap.add_argument("--autotune", action="store_true",
                     help="time the candidate tile shapes and workgroup counts first (mlem_autotune) and use the best")
@@ -395,8 +397,10 @@ def main():
         tune = m.autotune(per_pass=args.per_pass if args.batch else 1,
                           values=VALUES[args.values] if args.batch else 0)
         args.tile, args.tile_rows, args.tile_groups = tune["tile_width"], tune["tile_rows"], tune["tile_groups"]
+        args.tile_threads = tune["tile_threads"]
         log(f"autotune {time.time() - t:.1f} s over {tune['candidates']} candidates: tiles {args.tile} x "
-            f"{args.tile_rows}, {args.tile_groups} workgroups (FP {tune['fp_ms']:.3f} ms, BP {tune['bp_ms']:.3f} ms)")
+            f"{args.tile_rows}, {args.tile_groups} workgroups of {args.tile_threads or 'default'} work-items "
+            f"(FP {tune['fp_ms']:.3f} ms, BP {tune['bp_ms']:.3f} ms)")
 
     if args.batch:
         return run_batch(args, M, geo, A, g, m, s32, threads, label)

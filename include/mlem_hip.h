@@ -316,6 +316,10 @@ typedef struct mlem_batch_opts {
     uint32_t tile_width;  /* 0 = automatic (1024 for 1 per pass, else 512) */
     uint32_t tile_rows;   /* 0 = automatic                                 */
     uint32_t tile_groups; /* workgroups per blocked SpMV, 0 = automatic    */
+    uint32_t tile_threads; /* work-items per workgroup of the kernel for
+                              4 or 8 per pass: 128, 256 or 512 (each takes
+                              512 / tile_threads groups of 8 entries of a
+                              window), 0 = automatic (512)               */
 } mlem_batch_opts;
 void mlem_batch_opts_default(mlem_batch_opts *opts);
 /* The sensitivity (shared by all slots) is the matrix's FP64 column sums
@@ -349,6 +353,7 @@ int mlem_batch_destroy(mlem_batch *b);
  * in the reference (its section sizes come from argv[5]/[6]). */
 typedef struct mlem_tune_result {
     uint32_t tile_width, tile_rows, tile_groups;
+    uint32_t tile_threads; /* for mlem_batch_opts (0 for 1 per pass)       */
     uint32_t candidates;  /* shape x workgroup-count pairs timed            */
     double fp_ms, bp_ms;  /* per launch of the chosen one, partials + reduce */
 } mlem_tune_result;

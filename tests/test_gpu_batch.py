@@ -6,6 +6,8 @@ significand: the stored matrix differs from the FP32 one by up to 2^-9 per
 value, each value rounded to nearest after the per-row scale; the products,
 sums, ratio and update stay FP32); a slot's image is bitwise the same alone
 and in a full batch."""
+import itertools
+
 import numpy as np
 import pytest
 
@@ -55,12 +57,14 @@ def test_16bit_values_need_the_blocked_path(setup):
         M.Recon(setup["m"], setup["sinos"][0], setup["f0"], sens=setup["s32"], options=M.opts(spmv=1, values=1))
 
 
-@pytest.mark.parametrize("per_pass,width,values", [(1, 1024, 0), (4, 512, 0), (4, 1024, 0), (8, 512, 0), (4, 512, 1),
-                                                   (4, 512, 2), (8, 512, 2), (4, 1024, 1), (1, 1024, 2)])
-def test_batch_matches_oracle(setup, per_pass, width, values):
-    print(f"batch per_pass={per_pass} W={width} values={values}", flush=True)
+@pytest.mark.parametrize("per_pass,width,values,threads",
+                         [(1, 1024, 0, 0), (4, 512, 0, 0), (4, 1024, 0, 0), (8, 512, 0, 0), (4, 512, 1, 0),
+                          (4, 512, 2, 0), (8, 512, 2, 0), (4, 1024, 1, 0), (1, 1024, 2, 0), (4, 512, 0, 256),
+                          (4, 512, 0, 512), (8, 512, 2, 256), (4, 1024, 1, 512)])
+def test_batch_matches_oracle(setup, per_pass, width, values, threads):
+    print(f"batch per_pass={per_pass} W={width} values={values} threads={threads}", flush=True)
     n = len(setup["sinos"])
-    with M.Batch(setup["m"], n, values=values, per_pass=per_pass, tile_width=width) as b:
+    with M.Batch(setup["m"], n, values=values, per_pass=per_pass, tile_width=width, tile_threads=threads) as b:
         for k, g in enumerate(setup["sinos"]):
             b.set(k, g, setup["f0"])
         b.iterate(ITERS)
@@ -112,6 +116,8 @@ def test_batch_argument_errors(setup, c1):
         M.Batch(setup["m"], 4, per_pass=3)
     with pytest.raises(M.MlemError):
         M.Batch(setup["m"], 4, per_pass=8, tile_width=1024)
+    with pytest.raises(M.MlemError):
+        M.Batch(setup["m"], 4, per_pass=4, tile_threads=64)
     with M.ResidentMatrix(c1["A"]) as no_t:
         with pytest.raises(M.MlemError):
             M.Batch(no_t, 4)
@@ -126,9 +132,12 @@ def test_batch_edge_matrices(per_pass, width, rows, groups):
     rows and columns, one 60,000-entry row (carry chains across whole
     windows), heavy tails, duplicate columns, no entries, one row, rows with
     gaps; one run per row block up to one window per run; two passes, the
-    second one partly filled.  Every slot within 1e-5 of the FP64 oracle."""
+    second one partly filled; workgroups of 128, 256 and 512 work-items (4,
+    2 and 1 groups of a window each).  Every slot within 1e-5 of the FP64
+    oracle."""
     from matrices import make
-    for kind in ["empty_rows", "one_huge_row", "heavy", "dups", "all_empty", "single_row", "row_gaps"]:
+    for kind, threads in itertools.product(["empty_rows", "one_huge_row", "heavy", "dups", "all_empty", "single_row",
+                                            "row_gaps"], [128, 256, 512]):
         A = make(kind, seed=7)
         AT = M.transpose(A)
         rng = np.random.default_rng(8)
@@ -137,9 +146,10 @@ def test_batch_edge_matrices(per_pass, width, rows, groups):
         with M.ResidentMatrix(A, AT) as m:
             s32, _ = m.sensitivity()
             f0 = np.ones(A.cols, np.float32)
-            print(f"batch edge {kind} per_pass={per_pass} W={width} R={rows} groups={groups}", flush=True)
+            print(f"batch edge {kind} per_pass={per_pass} W={width} R={rows} groups={groups} threads={threads}",
+                  flush=True)
             with M.Batch(m, slots, values=0, per_pass=per_pass, tile_width=width, tile_rows=rows,
-                         tile_groups=groups) as b:
+                         tile_groups=groups, tile_threads=threads) as b:
                 for k in range(slots):
                     b.set(k, sinos[k], f0)
                 b.iterate(8)
@@ -157,13 +167,14 @@ def test_autotune_choice_reconstructs(setup, per_pass, values):
     print(f"autotune per_pass={per_pass} values={values}: {t}", flush=True)
     assert t["candidates"] >= 2 and t["fp_ms"] > 0 and t["bp_ms"] > 0
     assert t["tile_width"] in (512, 1024, 2048) and t["tile_rows"] in (512, 1024, 2048, 4096)
+    assert t["tile_threads"] == 0 if per_pass == 1 else t["tile_threads"] in (256, 512)
     if per_pass == 1:
         o = M.opts(spmv=0, tile_width=t["tile_width"], tile_rows=t["tile_rows"], tile_groups=t["tile_groups"])
         img = M.mlem_run(setup["m"], setup["sinos"][0], setup["f0"], ITERS, sens=setup["s32"], options=o)
         assert rel_l2(img, setup["refs"][0]) <= BAR[0]
     else:
         with M.Batch(setup["m"], per_pass, values=values, per_pass=per_pass, tile_width=t["tile_width"],
-                     tile_rows=t["tile_rows"], tile_groups=t["tile_groups"]) as b:
+                     tile_rows=t["tile_rows"], tile_groups=t["tile_groups"], tile_threads=t["tile_threads"]) as b:
             for k in range(per_pass):
                 b.set(k, setup["sinos"][k], setup["f0"])
             b.iterate(ITERS)
