@@ -11,6 +11,7 @@
 #include <hip/hip_runtime.h>
 
 #include <algorithm>
+#include <cstdlib>
 #include <array>
 #include <chrono>
 #include <cstdarg>
@@ -167,6 +168,12 @@ struct mlem_matrix {
     DevBuf rowptr, col, val;
     DevBuf rowptrT, colT, valT;
     bool has_T = false;
+    // Tiles only (mlem_matrix_create_tiled): no CSR copies on the device, the
+    // tiles of A and A^T (width tile_W, tile_R rows, f32) and the
+    // sensitivity taken while they were built.
+    bool tiles_only = false;
+    uint32_t tile_W = 0, tile_R = 0;
+    DevBuf sens32, sens64;
     std::mutex mu;
     std::map<std::pair<int, uint32_t>, std::unique_ptr<Partition>> parts;  // (0: A / 1: A^T, secsize)
     // Tiles keyed by (0: A / 1: A^T, width W, rows R, value type, quad groups).
@@ -175,7 +182,8 @@ struct mlem_matrix {
     std::map<TileKey, uint32_t> tile_refused;  // build flags of refused tilings
 
     uint64_t device_bytes() const {
-        uint64_t b = rowptr.bytes + col.bytes + val.bytes + rowptrT.bytes + colT.bytes + valT.bytes;
+        uint64_t b = rowptr.bytes + col.bytes + val.bytes + rowptrT.bytes + colT.bytes + valT.bytes + sens32.bytes +
+                     sens64.bytes;
         for (const auto &t : tiles) b += t.second->mem.bytes;
         return b;
     }
@@ -260,6 +268,9 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bo
         *out = it->second.get();
         return MLEM_OK;
     }
+    if (m->tiles_only)
+        return set_err(MLEM_ERR_UNSUPPORTED, "a tiles-only matrix holds tiles of width %u and %u rows (f32) only",
+                       m->tile_W, m->tile_R);
     auto ref = m->tile_refused.find(key);
     if (ref != m->tile_refused.end()) {
         *flags = ref->second;
@@ -323,6 +334,97 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bo
     m->tiles.emplace(key, std::move(t));
     return MLEM_OK;
 }
+
+// Progress of long builds on stderr when MLEM_VERBOSE is set (not empty, not 0).
+void progress(const char *fmt, ...) {
+    static const bool on = [] {
+        const char *v = std::getenv("MLEM_VERBOSE");
+        return v && *v && std::strcmp(v, "0") != 0;
+    }();
+    if (!on) return;
+    va_list ap;
+    va_start(ap, fmt);
+    std::fprintf(stderr, "[mlem] ");
+    std::vfprintf(stderr, fmt, ap);
+    std::fprintf(stderr, "\n");
+    std::fflush(stderr);
+    va_end(ap);
+}
+
+// Tiles (f32 values, one right-hand side) of a matrix that is not resident
+// as one CSR: given as chunks of whole row blocks, each a CSR of its own
+// (rows r0 .. r0+rows_c, row pointers from 0) on the device.  Two passes
+// over the same row blocks (the chunks may differ between them): the counts
+// give every tile's size and so the offsets of the whole layout; then, the
+// tiles allocated, each chunk fills its part.  The bytes are those that one
+// build over the whole CSR writes (every tile is laid out on its own).
+struct ChunkedTiles {
+    TilePlan P;
+    std::vector<uint64_t> size;  // padded entries per tile, then (allocate) offsets
+    std::vector<uint32_t> nz;    // entries per tile
+    Tiled *t = nullptr;
+
+    ChunkedTiles(const TilePlan &p, Tiled *tiled) : P(p), size(p.ntiles + 1, 0), nz(p.ntiles, 0), t(tiled) {}
+
+    // The chunk's own plan, its count into `scratch` (allocated here).
+    int count_chunk(uint32_t r0, uint32_t rows_c, uint64_t nnz_c, const uint64_t *rp, const uint32_t *ci,
+                    TilePlan *pc, DevBuf *scratch) {
+        *pc = tile_plan(rows_c, P.cols, nnz_c, P.W, P.R, VAL_F32, false);
+        HIPCHK(scratch->alloc(tile_count_scratch_bytes(*pc)));
+        uint32_t fl = 0;
+        HIPCHK(launch_tile_count(*pc, rp, ci, scratch->p, &fl, nullptr));
+        if (fl) return set_err(MLEM_ERR_UNSUPPORTED, "tiling refused: a row is not sorted by column (rows from %u)", r0);
+        (void)r0;
+        return MLEM_OK;
+    }
+
+    int count(uint32_t r0, uint32_t rows_c, uint64_t nnz_c, const uint64_t *rp, const uint32_t *ci) {
+        TilePlan pc;
+        DevBuf scratch;
+        if (int st = count_chunk(r0, rows_c, nnz_c, rp, ci, &pc, &scratch)) return st;
+        const uint64_t t0 = (uint64_t)(r0 / P.R) * P.nseg;
+        std::vector<uint64_t> off(pc.ntiles + 1);
+        HIPCHK(hipMemcpy(off.data(), tile_count_offsets(pc, scratch.p), 8 * (pc.ntiles + 1), hipMemcpyDeviceToHost));
+        if (pc.ntiles)
+            HIPCHK(hipMemcpy(nz.data() + t0, tile_count_sizes(pc, scratch.p), 4 * pc.ntiles, hipMemcpyDeviceToHost));
+        for (uint64_t i = 0; i < pc.ntiles; ++i) size[t0 + i] = off[i + 1] - off[i];
+        return MLEM_OK;
+    }
+
+    int allocate() {
+        uint64_t run = 0;
+        for (uint64_t i = 0; i <= P.ntiles; ++i) {
+            const uint64_t n = i < P.ntiles ? size[i] : 0;
+            size[i] = run;
+            run += n;
+        }
+        P.cap = run + 8;  // as one build: a multiple of 8, at least one group
+        t->plan = P;
+        HIPCHK(t->mem.alloc(tiled_matrix_bytes(P)));
+        t->view = tiled_matrix_view(P, t->mem.p);
+        HIPCHK(hipMemcpy(t->view.tile_off, size.data(), 8 * (P.ntiles + 1), hipMemcpyHostToDevice));
+        if (P.ntiles) HIPCHK(hipMemcpy(t->view.tile_nz, nz.data(), 4 * P.ntiles, hipMemcpyHostToDevice));
+        return MLEM_OK;
+    }
+
+    int fill(uint32_t r0, uint32_t rows_c, uint64_t nnz_c, const uint64_t *rp, const uint32_t *ci, const float *va) {
+        TilePlan pc;
+        DevBuf scratch, local_off, fill_scratch;
+        if (int st = count_chunk(r0, rows_c, nnz_c, rp, ci, &pc, &scratch)) return st;
+        const uint64_t t0 = (uint64_t)(r0 / P.R) * P.nseg;
+        HIPCHK(local_off.alloc(8 * (pc.ntiles + 1)));
+        HIPCHK(fill_scratch.alloc(tile_fill_scratch_bytes(pc)));
+        TiledMatrix v;
+        v.tile_off = local_off.as<uint64_t>();
+        v.tile_nz = t->view.tile_nz + t0;
+        v.tdata = t->view.tdata + (uint64_t)(P.vb() + 2) * size[t0];
+        v.scale = nullptr;
+        uint32_t fl = 0;
+        HIPCHK(launch_tile_fill_into(pc, rp, ci, va, v, scratch.p, fill_scratch.p, &fl, nullptr));
+        if (fl) return set_err(MLEM_ERR_FORMAT, "tiling: layout check failed (flags %u, rows from %u)", fl, r0);
+        return MLEM_OK;
+    }
+};
 
 // Work list of tiled matrix t for about `target` workgroups, built on first
 // use from the tile sizes.
@@ -404,7 +506,7 @@ int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t sec
         const uint32_t W = o.tile_width == 0 ? 1024u : o.tile_width;
         const uint32_t R = o.tile_rows == 0 ? 2048u : o.tile_rows;
         const int st = setup_tiled(m, which, W, R, o.values, 1, o.tile_groups, tiled, work, part);
-        if (st == MLEM_OK) return MLEM_OK;
+        if (st == MLEM_OK || m->tiles_only) return st;
         // Automatic width with f32 values: a matrix the tiles do not suit, or
         // whose tiles do not fit beside the resident copies (the 100 GB
         // matrices), takes the plain merge path; otherwise no fallback.
@@ -505,7 +607,10 @@ int destroy_graph(mlem_recon *r) {
 int sens_on_device(const mlem_matrix *m, float *s32, double *s64_or_null, hipStream_t s) {
     DevBuf tmp;
     double *s64 = s64_or_null;
-    if (m->has_T) {
+    if (m->tiles_only) {
+        if (s32) HIPCHK(hipMemcpyAsync(s32, m->sens32.p, m->sens32.bytes, hipMemcpyDeviceToDevice, s));
+        if (s64) HIPCHK(hipMemcpyAsync(s64, m->sens64.p, m->sens64.bytes, hipMemcpyDeviceToDevice, s));
+    } else if (m->has_T) {
         HIPCHK(launch_colsum_rows_f64(m->rowptrT.as<uint64_t>(), m->valT.as<float>(), m->cols, s64, s32, s));
     } else {
         if (!s64) {
@@ -884,6 +989,139 @@ int mlem_matrix_create(int device, uint32_t rows, uint32_t cols, uint64_t nnz, c
     return MLEM_OK;
 }
 
+int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
+                             const uint32_t *col, const float *val, uint32_t tile_width, uint32_t tile_rows,
+                             uint64_t chunk_bytes, mlem_matrix **out) {
+    ARGCHK(out, "mlem_matrix_create_tiled: null out");
+    *out = nullptr;
+    ARGCHK(rowptr && (nnz == 0 || (col && val)), "mlem_matrix_create_tiled: null matrix array");
+    ARGCHK(rows > 0 && cols > 0, "mlem_matrix_create_tiled: rows and cols must be > 0");
+    if (int st = check_csr_host("mlem_matrix_create_tiled", rows, cols, nnz, rowptr, col)) return st;
+    const uint32_t W = tile_width ? tile_width : 1024u, R = tile_rows ? tile_rows : 2048u;
+    ARGCHK(valid_tile_config(W, 1), "mlem_matrix_create_tiled: tile_width must be 0, 1024 or 2048");
+    ARGCHK(valid_tile_rows(R), "mlem_matrix_create_tiled: tile_rows must be 512 ... 16384 (a power of two)");
+    HIPCHK(hipSetDevice(device));
+    auto m = std::make_unique<mlem_matrix>();
+    m->device = device;
+    m->rows = rows;
+    m->cols = cols;
+    m->nnz = nnz;
+    m->tiles_only = true;
+    m->tile_W = W;
+    m->tile_R = R;
+    const size_t margin = (size_t)1 << 30;
+    auto avail = [&](size_t *b) -> int {
+        size_t free_b = 0, total_b = 0;
+        HIPCHK(hipMemGetInfo(&free_b, &total_b));
+        *b = free_b > margin ? free_b - margin : 0;
+        if (chunk_bytes) *b = std::min<size_t>(*b, chunk_bytes);
+        return MLEM_OK;
+    };
+    HIPCHK(m->sens32.alloc(sizeof(float) * cols));
+    HIPCHK(m->sens64.alloc(sizeof(double) * cols));
+    auto tT = std::make_unique<Tiled>();
+    auto tA = std::make_unique<Tiled>();
+    ChunkedTiles bT(tile_plan(cols, rows, nnz, W, R, VAL_F32, false), tT.get());
+    ChunkedTiles bA(tile_plan(rows, cols, nnz, W, R, VAL_F32, false), tA.get());
+    {
+        // A on the device; A^T's tiles from slabs of A^T's rows (column
+        // ranges of A, whole row blocks of A^T), the sensitivity from the
+        // slabs' row sums (FP64, as for a resident transpose).
+        DevBuf rp, ci, va, colptrT, srp;
+        HIPCHK(rp.alloc(sizeof(uint64_t) * ((size_t)rows + 1)));
+        HIPCHK(ci.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, nnz)));
+        HIPCHK(va.alloc(sizeof(float) * std::max<uint64_t>(1, nnz)));
+        HIPCHK(hipMemcpy(rp.p, rowptr, rp.bytes, hipMemcpyHostToDevice));
+        if (nnz) {
+            HIPCHK(hipMemcpy(ci.p, col, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice));
+            HIPCHK(hipMemcpy(va.p, val, sizeof(float) * nnz, hipMemcpyHostToDevice));
+        }
+        HIPCHK(colptrT.alloc(sizeof(uint64_t) * ((size_t)cols + 1)));
+        HIPCHK(srp.alloc(sizeof(uint64_t) * ((size_t)cols + 1)));
+        for (int pass = 0; pass < 2; ++pass) {
+            if (pass == 1)
+                if (int st = bT.allocate()) return st;
+            int st_sink = MLEM_OK;
+            RangeSink sink = [&](uint32_t c0, uint32_t c1, uint64_t off0, uint64_t n, const uint32_t *rowT,
+                                 const float *valT) -> hipError_t {
+                hipError_t e = launch_rebase(colptrT.as<uint64_t>() + c0, off0, c1 - c0, srp.as<uint64_t>(), nullptr);
+                if (e != hipSuccess) return e;
+                if (pass == 0) {
+                    e = launch_colsum_rows_f64(srp.as<uint64_t>(), valT, c1 - c0, m->sens64.as<double>() + c0,
+                                               m->sens32.as<float>() + c0, nullptr);
+                    if (e != hipSuccess) return e;
+                    st_sink = bT.count(c0, c1 - c0, n, srp.as<uint64_t>(), rowT);
+                } else {
+                    st_sink = bT.fill(c0, c1 - c0, n, srp.as<uint64_t>(), rowT, valT);
+                }
+                return st_sink == MLEM_OK ? hipSuccess : hipErrorUnknown;
+            };
+            // Half of what is free for the ranges' sorts, the rest for the
+            // slab's tiling scratch.
+            size_t budget = 0;
+            if (int st = avail(&budget)) return st;
+            uint32_t nr = 0;
+            const hipError_t e = transpose_ranges_to(rows, cols, nnz, rp.as<uint64_t>(), ci.as<uint32_t>(),
+                                                     va.as<float>(), colptrT.as<uint64_t>(), nullptr, nullptr,
+                                                     budget / 2, R, sink, &nr, nullptr);
+            if (st_sink != MLEM_OK) return st_sink;
+            if (e == hipErrorOutOfMemory)
+                return set_err(MLEM_ERR_OUT_OF_MEMORY, "mlem_matrix_create_tiled: %u rows of A^T do not fit", R);
+            HIPCHK(e);
+            progress("A^T tiles %s over %u column ranges (%.1f GB of range scratch)", pass ? "filled" : "counted", nr,
+                     budget / 2 / 1e9);
+        }
+        HIPCHK(hipDeviceSynchronize());
+    }
+    // A's tiles from row chunks of the host CSR (A's device copy is gone),
+    // whole row blocks whose CSR and tiling scratch fit.
+    const uint64_t nseg = bA.P.nseg;
+    progress("A^T tiles %.1f GB; A's device copy freed", tT->mem.bytes / 1e9);
+    for (int pass = 0; pass < 2; ++pass) {
+        if (pass == 1)
+            if (int st = bA.allocate()) return st;
+        size_t budget = 0;
+        if (int st = avail(&budget)) return st;
+        uint32_t chunks = 0;
+        for (uint32_t r0 = 0; r0 < rows; ++chunks) {
+            uint32_t r1 = std::min<uint64_t>((uint64_t)r0 + R, rows);
+            auto need = [&](uint32_t a, uint32_t b) {
+                const uint64_t n = rowptr[b] - rowptr[a];
+                return n * 14 + (uint64_t)(b - a) * (nseg * 8 + 8) + ((size_t)64 << 20);
+            };
+            while (r1 < rows) {
+                const uint32_t nx = (uint32_t)std::min<uint64_t>((uint64_t)r1 + R, rows);
+                if (need(r0, nx) > budget) break;
+                r1 = nx;
+            }
+            const uint32_t rc = r1 - r0;
+            const uint64_t n = rowptr[r1] - rowptr[r0];
+            std::vector<uint64_t> hrp(rc + 1);
+            for (uint32_t i = 0; i <= rc; ++i) hrp[i] = rowptr[r0 + i] - rowptr[r0];
+            DevBuf rp, ci, va;
+            HIPCHK(rp.alloc(sizeof(uint64_t) * (rc + 1)));
+            HIPCHK(ci.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, n)));
+            HIPCHK(hipMemcpy(rp.p, hrp.data(), rp.bytes, hipMemcpyHostToDevice));
+            if (n) HIPCHK(hipMemcpy(ci.p, col + rowptr[r0], sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+            if (pass == 0) {
+                if (int st = bA.count(r0, rc, n, rp.as<uint64_t>(), ci.as<uint32_t>())) return st;
+            } else {
+                HIPCHK(va.alloc(sizeof(float) * std::max<uint64_t>(1, n)));
+                if (n) HIPCHK(hipMemcpy(va.p, val + rowptr[r0], sizeof(float) * n, hipMemcpyHostToDevice));
+                if (int st = bA.fill(r0, rc, n, rp.as<uint64_t>(), ci.as<uint32_t>(), va.as<float>())) return st;
+            }
+            r0 = r1;
+        }
+        progress("A tiles %s over %u row chunks", pass ? "filled" : "counted", chunks);
+    }
+    HIPCHK(hipDeviceSynchronize());
+    progress("A tiles %.1f GB", tA->mem.bytes / 1e9);
+    m->tiles.emplace(std::make_tuple(0, W, R, (uint32_t)VAL_F32, false), std::move(tA));
+    m->tiles.emplace(std::make_tuple(1, W, R, (uint32_t)VAL_F32, false), std::move(tT));
+    *out = m.release();
+    return MLEM_OK;
+}
+
 int mlem_matrix_load_csr4(int device, const char *path, int threads, mlem_matrix **out, mlem_load_stats *stats) {
     ARGCHK(path && out, "mlem_matrix_load_csr4: null argument");
     *out = nullptr;
@@ -989,6 +1227,7 @@ int mlem_matrix_build_transpose(mlem_matrix *m) { return mlem_matrix_build_trans
 
 int mlem_matrix_build_transpose_limited(mlem_matrix *m, uint64_t scratch_bytes, uint32_t *ranges) {
     ARGCHK(m, "mlem_matrix_build_transpose: null matrix");
+    ARGCHK(!m->tiles_only, "mlem_matrix_build_transpose: a tiles-only matrix has no CSR");
     HIPCHK(hipSetDevice(m->device));
     if (ranges) *ranges = 0;
     DevBuf scratch, rp, ci, va;
@@ -1107,9 +1346,17 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
     ARGCHK(o.fp_lanes == 0 || valid_lanes(o.fp_lanes), "mlem_recon_create: fp_lanes must be 0,4,8,16,32,64");
     ARGCHK(o.bp_lanes == 0 || valid_lanes(o.bp_lanes), "mlem_recon_create: bp_lanes must be 0,4,8,16,32,64");
     ARGCHK(o.bp_spmv <= 2, "mlem_recon_create: bp_spmv must be 0 (as spmv), 1 (merge) or 2 (CSR-vector)");
-    if (o.using_trans == 0 && !m->has_T)
+    if (o.using_trans == 0 && !m->has_T && !m->tiles_only)
         return set_err(MLEM_ERR_INVALID_ARGUMENT,
                        "mlem_recon_create: using_trans == 0 needs the transpose (upload it or build it)");
+    if (m->tiles_only) {
+        ARGCHK(o.spmv == 0 && o.using_trans == 0 && o.bp_spmv <= 1 && o.values == VAL_F32,
+               "mlem_recon_create: a tiles-only matrix takes merge-based FP and BP on A^T with f32 values");
+        ARGCHK((o.tile_width == 0 || o.tile_width == m->tile_W) && (o.tile_rows == 0 || o.tile_rows == m->tile_R),
+               "mlem_recon_create: the matrix holds tiles of width %u and %u rows", m->tile_W, m->tile_R);
+        o.tile_width = m->tile_W;
+        o.tile_rows = m->tile_R;
+    }
     HIPCHK(hipSetDevice(m->device));
     auto r = std::make_unique<mlem_recon>();
     r->m = m;
@@ -1261,8 +1508,21 @@ int mlem_recon_loglik(mlem_recon *r, double *ll) {
     HIPCHK(hipSetDevice(r->m->device));
     const mlem_matrix *m = r->m;
     const uint32_t nb = loglik_blocks(m->rows);
-    HIPCHK(launch_loglik(m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(), r->f.as<float>(),
-                         r->g.as<int32_t>(), m->rows, r->llpart.as<double>(), r->stream));
+    if (m->tiles_only) {  // A f by the tiled forward projection
+        DevBuf y;
+        HIPCHK(y.alloc(sizeof(float) * std::max<size_t>(1, m->rows)));
+        EpiArgs ea{};
+        ea.out = y.as<float>();
+        HIPCHK(launch_tiled_partials(r->tfp->plan, r->tfp->view, r->wfp->view, r->f.as<float>(), r->part_fp.as<float>(),
+                                     1, 0, r->stream));
+        HIPCHK(launch_tiled_reduce(r->tfp->plan, r->tfp->view, r->wfp->view, r->part_fp.as<float>(), 1, EPI_PLAIN, ea,
+                                   r->stream));
+        HIPCHK(launch_loglik_y(y.as<float>(), r->g.as<int32_t>(), m->rows, r->llpart.as<double>(), r->stream));
+        HIPCHK(hipStreamSynchronize(r->stream));
+    } else {
+        HIPCHK(launch_loglik(m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(), r->f.as<float>(),
+                             r->g.as<int32_t>(), m->rows, r->llpart.as<double>(), r->stream));
+    }
     std::vector<double> part(nb);
     if (nb) HIPCHK(hipMemcpyAsync(part.data(), r->llpart.p, sizeof(double) * nb, hipMemcpyDeviceToHost, r->stream));
     HIPCHK(hipStreamSynchronize(r->stream));

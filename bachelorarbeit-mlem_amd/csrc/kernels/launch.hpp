@@ -7,6 +7,7 @@
 
 #include <cstddef>
 #include <cstdint>
+#include <functional>
 #include <vector>
 
 namespace mlem {
@@ -84,6 +85,8 @@ hipError_t launch_f64_to_f32(const double *a, float *b, uint32_t n, hipStream_t 
 uint32_t loglik_blocks(uint32_t rows);
 hipError_t launch_loglik(const uint64_t *rowptr, const uint32_t *col, const float *val, const float *x,
                          const int32_t *g, uint32_t rows, double *part, hipStream_t s);
+// The same from y = A x computed elsewhere (the tiled forward projection).
+hipError_t launch_loglik_y(const float *y, const int32_t *g, uint32_t rows, double *part, hipStream_t s);
 
 // Blocked SpMV (tiled.hip): tiles of R rows x W columns with the W-wide
 // segment of x staged in LDS, entries packed to 6 bytes in units of
@@ -162,6 +165,18 @@ hipError_t launch_tile_count(TilePlan &p, const uint64_t *rowptr, const uint32_t
                              uint32_t *flags_host, hipStream_t s);
 hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
                             void *mem, void *scratch, void *fill_scratch, uint32_t *flags_host, hipStream_t s);
+// The same into arrays given by a view: tile_off receives the count's
+// offsets (relative to tdata), tile_nz the sizes.  Tiles built in row
+// chunks (whole row blocks, each chunk a CSR of its own) fill the chunk's
+// part of the whole matrix's tiles: tdata shifted by the chunk's first entry
+// offset, tile_nz by its first tile, tile_off a scratch of ntiles + 1.
+hipError_t launch_tile_fill_into(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
+                                 const TiledMatrix &tm, void *scratch, void *fill_scratch, uint32_t *flags_host,
+                                 hipStream_t s);
+// After launch_tile_count: the tile offsets (ntiles + 1) and sizes (ntiles)
+// it left in `scratch` (device pointers).
+const uint64_t *tile_count_offsets(const TilePlan &p, void *scratch);
+const uint32_t *tile_count_sizes(const TilePlan &p, void *scratch);
 // Runs for about target_wgs workgroups from the host copies of tile_nz and
 // tile_off.
 TileWorkList tile_work_list(const TilePlan &p, const uint32_t *tile_nz, const uint64_t *tile_off,
@@ -193,5 +208,19 @@ hipError_t launch_transpose(uint32_t rows, uint32_t cols, uint64_t nnz, const ui
 hipError_t transpose_in_ranges(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
                                const uint32_t *col, const float *val, uint64_t *colptrT, uint32_t *rowT, float *valT,
                                size_t budget, uint32_t *ranges, hipStream_t s);
+// The same with the ranges' columns a multiple of `align` (but the last)
+// and, when `sink` is set, each range handed to it as a slab of A^T's rows
+// [c0, c1) (n entries from offset off0 = colptrT[c0] of A^T: rowT and valT
+// of the slab, its row pointers colptrT[c0..c1] minus off0) instead of
+// being written to rowT / valT (then null).  colptrT must still hold cols + 1
+// entries.  The slab's arrays are valid until the sink returns.
+using RangeSink = std::function<hipError_t(uint32_t c0, uint32_t c1, uint64_t off0, uint64_t n,
+                                           const uint32_t *rowT, const float *valT)>;
+hipError_t transpose_ranges_to(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
+                               const uint32_t *col, const float *val, uint64_t *colptrT, uint32_t *rowT, float *valT,
+                               size_t budget, uint32_t align, const RangeSink &sink, uint32_t *ranges_out,
+                               hipStream_t s);
+// dst[i] = src[i] - base for i = 0..n.
+hipError_t launch_rebase(const uint64_t *src, uint64_t base, uint32_t n, uint64_t *dst, hipStream_t s);
 
 }  // namespace mlem

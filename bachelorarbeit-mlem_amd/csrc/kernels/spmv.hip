@@ -683,6 +683,31 @@ hipError_t launch_colsum_scatter_f64(const uint64_t *rowptr, const uint32_t *col
 
 uint32_t loglik_blocks(uint32_t rows) { return blocks_for(rows, kWG / 16); }
 
+// The same terms from y = A x given (16 rows per block, as loglik_kernel<16>).
+__global__ void loglik_y_kernel(const float *__restrict__ y, const int32_t *__restrict__ g, uint32_t rows,
+                                double *__restrict__ part) {
+    __shared__ double s[16];
+    const uint32_t row = blockIdx.x * 16u + threadIdx.x;
+    double l = 0.0;
+    if (row < rows) {
+        const double yd = (double)y[row];
+        l = (yd > 0.0) ? (double)g[row] * log(yd) - yd : -yd;
+    }
+    s[threadIdx.x] = l;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        double acc = 0.0;
+        for (int q = 0; q < 16; ++q) acc += s[q];
+        part[blockIdx.x] = acc;
+    }
+}
+
+hipError_t launch_loglik_y(const float *y, const int32_t *g, uint32_t rows, double *part, hipStream_t s) {
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(loglik_y_kernel, dim3(loglik_blocks(rows)), dim3(16), 0, s, y, g, rows, part);
+    return hipGetLastError();
+}
+
 hipError_t launch_loglik(const uint64_t *rowptr, const uint32_t *col, const float *val, const float *x,
                          const int32_t *g, uint32_t rows, double *part, hipStream_t s) {
     if (rows == 0) return hipSuccess;

@@ -1227,10 +1227,19 @@ hipError_t launch_tile_count(TilePlan &p, const uint64_t *rowptr, const uint32_t
     return hipSuccess;
 }
 
+const uint64_t *tile_count_offsets(const TilePlan &p, void *scratch) { return count_scratch(p, scratch).tile_off; }
+const uint32_t *tile_count_sizes(const TilePlan &p, void *scratch) { return count_scratch(p, scratch).tile_nz; }
+
 hipError_t launch_tile_fill(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
                             void *mem, void *scratch, void *fill_scratch, uint32_t *flags_host, hipStream_t s) {
+    return launch_tile_fill_into(p, rowptr, col, val, tiled_matrix_view(p, mem), scratch, fill_scratch, flags_host,
+                                 s);
+}
+
+hipError_t launch_tile_fill_into(const TilePlan &p, const uint64_t *rowptr, const uint32_t *col, const float *val,
+                                 const TiledMatrix &tm, void *scratch, void *fill_scratch, uint32_t *flags_host,
+                                 hipStream_t s) {
     CountScratch c = count_scratch(p, scratch);
-    TiledMatrix tm = tiled_matrix_view(p, mem);
     uint16_t *ctmp = static_cast<uint16_t *>(fill_scratch);
     uint16_t *rtmp = reinterpret_cast<uint16_t *>(static_cast<char *>(fill_scratch) + align256(2 * p.cap));
     uint32_t wshift = 0;

@@ -169,8 +169,17 @@ hipError_t launch_transpose(uint32_t rows, uint32_t cols, uint64_t nnz, const ui
 hipError_t transpose_in_ranges(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
                                const uint32_t *col, const float *val, uint64_t *colptrT, uint32_t *rowT, float *valT,
                                size_t budget, uint32_t *ranges_out, hipStream_t s) {
+    return transpose_ranges_to(rows, cols, nnz, rowptr, col, val, colptrT, rowT, valT, budget, 1, RangeSink(),
+                               ranges_out, s);
+}
+
+hipError_t transpose_ranges_to(uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
+                               const uint32_t *col, const float *val, uint64_t *colptrT, uint32_t *rowT, float *valT,
+                               size_t budget, uint32_t align, const RangeSink &sink, uint32_t *ranges_out,
+                               hipStream_t s) {
     hipError_t e;
     *ranges_out = 0;
+    if (align == 0) align = 1;
     // colptrT: column counts, scanned (u32 counts, u64 offsets).
     size_t scan_bytes = 0;
     (void)rocprim::exclusive_scan(nullptr, scan_bytes, (const uint32_t *)nullptr, (uint64_t *)nullptr, uint64_t(0),
@@ -232,9 +241,11 @@ hipError_t transpose_in_ranges(uint32_t rows, uint32_t cols, uint64_t nnz, const
         return done(e);
     if ((e = hipStreamSynchronize(s)) != hipSuccess) return done(e);
     for (uint32_t c0 = 0; c0 < cols;) {
-        uint32_t c1 = c0 + 1;  // at least one column (it must fit)
+        // At least `align` columns (they must fit), else the furthest c1 with
+        // cp[c1] - cp[c0] <= max_n, a multiple of align or cols.
+        uint32_t c1 = (uint32_t)std::min<uint64_t>((uint64_t)c0 + align, cols);
         if (cp[c1] - cp[c0] > max_n) return done(hipErrorOutOfMemory);
-        {  // the furthest c1 with cp[c1] - cp[c0] <= max_n
+        {
             uint32_t a = c1, b = cols;
             while (a < b) {
                 const uint32_t mid = a + (b - a + 1) / 2;
@@ -243,7 +254,7 @@ hipError_t transpose_in_ranges(uint32_t rows, uint32_t cols, uint64_t nnz, const
                 else
                     b = mid - 1;
             }
-            c1 = a;
+            c1 = a == cols ? a : std::max(c1, a / align * align);
         }
         const uint64_t n = cp[c1] - cp[c0];
         ++*ranges_out;
@@ -262,13 +273,28 @@ hipError_t transpose_in_ranges(uint32_t rows, uint32_t cols, uint64_t nnz, const
                                                key_bits(c1 - c0), s)) != hipSuccess)
                 return done(e);
             const uint64_t blocks = std::min<uint64_t>((n + kWG - 1) / kWG, 65536);
-            hipLaunchKernelGGL(unpack_kernel, dim3((uint32_t)blocks), dim3(kWG), 0, s, packed_out, n, rowT + cp[c0],
-                               valT ? valT + cp[c0] : nullptr);
+            // With a sink the range goes to the freed sort inputs (a slab of
+            // A^T's rows c0..c1), else to its place in rowT / valT.
+            uint32_t *rT = sink ? keys_in : rowT + cp[c0];
+            float *vT = sink ? reinterpret_cast<float *>(packed_in) : (valT ? valT + cp[c0] : nullptr);
+            hipLaunchKernelGGL(unpack_kernel, dim3((uint32_t)blocks), dim3(kWG), 0, s, packed_out, n, rT, vT);
             if ((e = hipGetLastError()) != hipSuccess) return done(e);
         }
+        if (sink && (e = sink(c0, c1, cp[c0], n, keys_in, reinterpret_cast<const float *>(packed_in))) != hipSuccess)
+            return done(e);
         c0 = c1;
     }
     return done(hipSuccess);
+}
+
+__global__ void rebase_kernel(const uint64_t *__restrict__ src, uint64_t base, uint32_t n, uint64_t *__restrict__ dst) {
+    const uint32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i <= n) dst[i] = src[i] - base;
+}
+
+hipError_t launch_rebase(const uint64_t *src, uint64_t base, uint32_t n, uint64_t *dst, hipStream_t s) {
+    hipLaunchKernelGGL(rebase_kernel, dim3(n / kWG + 1), dim3(kWG), 0, s, src, base, n, dst);
+    return hipGetLastError();
 }
 
 }  // namespace mlem

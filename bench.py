@@ -318,8 +318,10 @@ def main():
     ap.add_argument("--steps", type=int, default=100, help="timed MLEM iterations")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=["c2", "mid", "c1", "c3"],
-                    help="c3: the 100 GB-class matrix (A and A^T resident, plain merge / CSR-vector kernels; "
-                         "no CPU baseline: both copies do not fit the host memory of one command)")
+                    help="c3: the 100 GB-class matrix, resident as the tiles of A and A^T only (no CPU baseline: "
+                         "A and A^T do not fit the host memory of one command)")
+    ap.add_argument("--c3-csr", action="store_true",
+                    help="c3: keep A and A^T resident as CSR instead (plain merge / CSR-vector kernels)")
     ap.add_argument("--fp", default="merge", choices=["merge", "vector"])
     ap.add_argument("--bp", default="merge", choices=["merge", "vector"])
     ap.add_argument("--secsize", type=int, default=8)
@@ -373,8 +375,15 @@ def main():
     geo, A, g, label = workload(args.workload, threads)
 
     t = time.time()
-    m = M.ResidentMatrix(A)
-    log(f"upload A {time.time() - t:.2f} s")
+    tiles_only = args.workload == "c3" and not args.c3_csr
+    if tiles_only:
+        m = M.ResidentMatrix.tiles_only(A, tile_width=args.tile if args.tile > 1 else 0, tile_rows=args.tile_rows)
+        log(f"tiles of A and A^T built without resident CSR copies (A^T's from column ranges of A on the device, "
+            f"A's from row chunks of the host CSR) {time.time() - t:.2f} s; "
+            f"{m.info()['device_bytes'] / 1e9:.1f} GB resident")
+    else:
+        m = M.ResidentMatrix(A)
+        log(f"upload A {time.time() - t:.2f} s")
     if args.workload == "c3":  # the host copy is not needed any more (and is ~120 GB)
         class _Dims:
             pass
@@ -382,17 +391,19 @@ def main():
         dims.rows, dims.cols, dims.nnz = A.rows, A.cols, A.nnz
         A = dims
         args.no_cpu = True
-        args.tile = 1  # the tiles do not fit in HBM beside both CSR copies: plain merge-path kernels
+        if not tiles_only:
+            args.tile = 1  # the tiles do not fit in HBM beside both CSR copies: plain merge-path kernels
         log("host copy of A released; CPU baseline skipped (A and A^T do not fit the host memory of one command)")
-    t = time.time()
-    ranges = m.build_transpose()
-    log(f"device transpose (stable counting sort{f', {ranges} column ranges' if ranges else ''}) "
-        f"{time.time() - t:.2f} s")
+    if not tiles_only:
+        t = time.time()
+        ranges = m.build_transpose()
+        log(f"device transpose (stable counting sort{f', {ranges} column ranges' if ranges else ''}) "
+            f"{time.time() - t:.2f} s")
     t = time.time()
     s32, _ = m.sensitivity()
     log(f"FP64 sensitivity {time.time() - t:.2f} s")
     f0, init = M.initial_image(g, A.rows, s32)
-    if args.autotune and args.tile != 1:
+    if args.autotune and args.tile != 1 and not tiles_only:
         t = time.time()
         tune = m.autotune(per_pass=args.per_pass if args.batch else 1,
                           values=VALUES[args.values] if args.batch else 0)
@@ -540,7 +551,8 @@ def main():
                    "bp": args.bp + (" blocked" if args.bp == "merge" and blocked else "") + " on A^T",
                    "tile_width": args.tile, "tile_rows": args.tile_rows, "tile_groups": args.tile_groups,
                    "graph": bool(args.graph),
-                   "parallelism": "single GPU, matrix and transpose resident"},
+                   "parallelism": "single GPU, " + ("tiles of A and A^T resident (no CSR copies)" if tiles_only
+                                                    else "matrix and transpose resident")},
         "roofline": roofline, "cpu_baseline": cpu,
     }
     if load:
