@@ -206,7 +206,13 @@ struct mlem_batch {
     const Tiled *tfp = nullptr, *tbp = nullptr;
     const TileWorkDev *wfp = nullptr, *wbp = nullptr;
     std::vector<uint32_t> active_host;
+    // One iteration captured as a graph (use_graph), for the passes that
+    // held an active slot when it was captured.
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint8_t> graph_passes;
     ~mlem_batch() {
+        if (gexec) (void)hipGraphExecDestroy(gexec);
         if (stream) {
             (void)hipStreamSynchronize(stream);
             (void)hipStreamDestroy(stream);
@@ -1713,6 +1719,7 @@ int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opt
     b->slots = slots;
     b->BC = BC;
     b->nt = o.tile_threads;
+    b->use_graph = o.use_graph != 0;
     b->passes = (slots + BC - 1) / BC;
     b->rows_x = (m->rows + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
     b->cols_x = (m->cols + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
@@ -1774,16 +1781,14 @@ int mlem_batch_deactivate(mlem_batch *b, uint32_t slot) {
     return MLEM_OK;
 }
 
-int mlem_batch_iterate(mlem_batch *b, uint32_t iters) {
-    ARGCHK(b, "mlem_batch_iterate: null handle");
-    HIPCHK(hipSetDevice(b->m->device));
+namespace {
+// One iteration of the passes marked in `passes` on the batch's stream.
+int enqueue_batch_iteration(mlem_batch *b, const std::vector<uint8_t> &passes) {
     hipStream_t s = b->stream;
     const uint32_t BC = b->BC;
-    for (uint32_t it = 0; it < iters; ++it) {
+    {
         for (uint32_t p = 0; p < b->passes; ++p) {
-            bool any = false;
-            for (uint32_t q = 0; q < BC; ++q) any = any || b->active_host[(size_t)p * BC + q];
-            if (!any) continue;
+            if (!passes[p]) continue;
             float *f = b->f.as<float>() + (size_t)p * b->cols_x * BC;
             float *ratio = b->ratio.as<float>() + (size_t)p * b->rows_x * BC;
             EpiArgs fa{};
@@ -1803,6 +1808,47 @@ int mlem_batch_iterate(mlem_batch *b, uint32_t iters) {
                                        EPI_UPDATE, ba, s));
         }
     }
+    return MLEM_OK;
+}
+}  // namespace
+
+int mlem_batch_iterate(mlem_batch *b, uint32_t iters) {
+    ARGCHK(b, "mlem_batch_iterate: null handle");
+    HIPCHK(hipSetDevice(b->m->device));
+    const uint32_t BC = b->BC;
+    std::vector<uint8_t> passes(b->passes, 0);
+    bool any = false;
+    for (uint32_t p = 0; p < b->passes; ++p) {
+        for (uint32_t q = 0; q < BC; ++q) passes[p] = passes[p] || b->active_host[(size_t)p * BC + q];
+        any = any || passes[p];
+    }
+    if (!any || iters == 0) return MLEM_OK;
+    if (!b->use_graph) {
+        for (uint32_t it = 0; it < iters; ++it)
+            if (int st = enqueue_batch_iteration(b, passes)) return st;
+        return MLEM_OK;
+    }
+    if (!b->gexec || passes != b->graph_passes) {  // (re)capture for this set of passes
+        if (b->gexec) {
+            HIPCHK(hipStreamSynchronize(b->stream));
+            HIPCHK(hipGraphExecDestroy(b->gexec));
+            b->gexec = nullptr;
+        }
+        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
+        const int st = enqueue_batch_iteration(b, passes);
+        hipGraph_t gr = nullptr;
+        const hipError_t e2 = hipStreamEndCapture(b->stream, &gr);
+        if (st != MLEM_OK) {
+            if (gr) (void)hipGraphDestroy(gr);
+            return st;
+        }
+        HIPCHK(e2);
+        const hipError_t e3 = hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0);
+        (void)hipGraphDestroy(gr);
+        HIPCHK(e3);
+        b->graph_passes = passes;
+    }
+    for (uint32_t it = 0; it < iters; ++it) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
     return MLEM_OK;
 }
 

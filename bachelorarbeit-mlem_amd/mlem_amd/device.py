@@ -345,17 +345,17 @@ def colsum_scatter_f64(rowptr, col, val, rows, cols, nnz, sens64=None, sens32=No
 # ---------------------------------------------------------------- handles
 class BatchOpts(C.Structure):
     _fields_ = [("values", u32), ("per_pass", u32), ("tile_width", u32), ("tile_rows", u32), ("tile_groups", u32),
-                ("tile_threads", u32)]
+                ("tile_threads", u32), ("use_graph", u32)]
 
 
 class Batch:
     """Many reconstructions over one resident matrix (A and A^T), iterated together (mlem_batch_*)."""
 
     def __init__(self, matrix: "ResidentMatrix", slots: int, values: int = 0, per_pass: int = 4, tile_width: int = 0,
-                 tile_rows: int = 0, tile_groups: int = 0, tile_threads: int = 0):
+                 tile_rows: int = 0, tile_groups: int = 0, tile_threads: int = 0, use_graph: int = 0):
         self.h = vp()
         self.matrix = matrix
-        o = BatchOpts(values, per_pass, tile_width, tile_rows, tile_groups, tile_threads)
+        o = BatchOpts(values, per_pass, tile_width, tile_rows, tile_groups, tile_threads, use_graph)
         _check(hip_lib().mlem_batch_create(matrix.h, slots, C.byref(o), C.byref(self.h)))
         self.slots = slots
 
@@ -410,9 +410,9 @@ class Server:
     between two iterations and leave when their iteration count is reached."""
 
     def __init__(self, hbm_budget_bytes: int = 0, values: int = 0, per_pass: int = 4, tile_width: int = 0,
-                 tile_rows: int = 0, tile_groups: int = 0, tile_threads: int = 0):
+                 tile_rows: int = 0, tile_groups: int = 0, tile_threads: int = 0, use_graph: int = 0):
         self.h = vp()
-        o = BatchOpts(values, per_pass, tile_width, tile_rows, tile_groups, tile_threads)
+        o = BatchOpts(values, per_pass, tile_width, tile_rows, tile_groups, tile_threads, use_graph)
         _check(hip_lib().mlem_server_create(hbm_budget_bytes, C.byref(o), C.byref(self.h)))
         self.cols = []
         self.rows = []

@@ -112,8 +112,8 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label):
         sinos.append(M.gen_sinogram(A, x, float(10 ** rng.uniform(6, 9)), seed=2000 + k, threads=threads))
     log(f"{B} sinograms (counts log-uniform 1e6-1e9) in {time.time() - t:.1f} s")
     f0 = np.ones(A.cols, np.float32)
-    # one-at-a-time FP32 reference path
-    o = M.opts(spmv=0, using_trans=0, profile=1)
+    # one-at-a-time FP32 reference path (a captured graph too when --graph)
+    o = M.opts(spmv=0, using_trans=0, profile=0, use_graph=args.graph)
     with M.Recon(m, g0, f0, sens=s32, options=o) as r:
         r.iterate(3)
         r.sync()
@@ -124,7 +124,7 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label):
     t = time.time()
     with M.Batch(m, B, values=VALUES[args.values], per_pass=args.per_pass,
                  tile_width=args.tile if args.tile > 1 else 0, tile_rows=args.tile_rows,
-                 tile_groups=args.tile_groups, tile_threads=args.tile_threads) as b:
+                 tile_groups=args.tile_groups, tile_threads=args.tile_threads, use_graph=args.graph) as b:
         for k in range(B):
             b.set(k, sinos[k], f0)
         info = b.info()
@@ -170,6 +170,7 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label):
         "vs_baseline": None, "dtype": f"{args.values} values, f32 arithmetic", "data": "synthetic",
         "config": {"workload": label + f", batch of {B} sinograms", "batch": B, "per_pass": args.per_pass,
                    "values": args.values, "iterations_per_recon": args.recon_iters, "passes": info["passes"],
+                   "graph": bool(args.graph),
                    "rows": A.rows, "cols": A.cols, "nnz": A.nnz},
         "one_at_a_time_f32": {"ms_per_iteration": round(t_single * 1e3, 4), "recon_s": round(single_recon_s, 3)},
         "speedup_vs_one_at_a_time_f32": round(recon_s / single_recon_s, 3),

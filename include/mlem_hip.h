@@ -340,6 +340,10 @@ typedef struct mlem_batch_opts {
                               4 or 8 per pass: 128, 256 or 512 (each takes
                               512 / tile_threads groups of 8 entries of a
                               window), 0 = automatic (512)               */
+    uint32_t use_graph;    /* 1: each mlem_batch_iterate call replays one
+                              iteration captured as a hipGraph (captured
+                              again when the set of passes holding an
+                              active slot changes)                        */
 } mlem_batch_opts;
 void mlem_batch_opts_default(mlem_batch_opts *opts);
 /* The sensitivity (shared by all slots) is the matrix's FP64 column sums

@@ -126,6 +126,41 @@ def test_batch_argument_errors(setup, c1):
             b.set(2, setup["sinos"][0], setup["f0"])
 
 
+@pytest.mark.parametrize("per_pass,values", [(4, 0), (4, 2), (1, 0)])
+def test_graph_replays_equal_eager_iterations(setup, per_pass, values):
+    """use_graph: every mlem_batch_iterate call replays one captured
+    iteration; after each call (eager image reads and host syncs between
+    the replays) every slot is bitwise the eager batch's, also after a slot
+    leaves, a pass empties (a new capture) and a slot is reused."""
+    n = 6
+    sinos, f0 = setup["sinos"], setup["f0"]
+    with M.Batch(setup["m"], n, values=values, per_pass=per_pass) as eager, \
+            M.Batch(setup["m"], n, values=values, per_pass=per_pass, use_graph=1) as graph:
+        for b in (eager, graph):
+            for k in range(n):
+                b.set(k, sinos[k], f0)
+
+        def step(iters, what):
+            for b in (eager, graph):
+                b.iterate(iters)
+                b.sync()
+            for k in range(n):
+                a, c = eager.image(k), graph.image(k)
+                assert np.array_equal(a.view(np.uint32), c.view(np.uint32)), (what, k)
+
+        for r in range(4):
+            step(1, f"replay {r}")
+        step(3, "three replays in one call")
+        for b in (eager, graph):
+            b.deactivate(n - 1)
+            b.deactivate(n - 2)  # with 4 per pass: the second pass empties
+        step(2, "after deactivation")
+        for b in (eager, graph):
+            b.set(n - 1, sinos[7], f0)
+        step(2, "after reuse")
+        assert not np.array_equal(graph.image(0), f0)  # it did iterate
+
+
 @pytest.mark.parametrize("per_pass,width,rows,groups", [(4, 512, 512, 0), (4, 1024, 2048, 1), (8, 512, 1024, 100000)])
 def test_batch_edge_matrices(per_pass, width, rows, groups):
     """Adversarial matrices through the batched kernels (f32 values): empty
