@@ -380,7 +380,6 @@ struct ChunkedTiles {
         uint32_t fl = 0;
         HIPCHK(launch_tile_count(*pc, rp, ci, scratch->p, &fl, nullptr));
         if (fl) return set_err(MLEM_ERR_UNSUPPORTED, "tiling refused: a row is not sorted by column (rows from %u)", r0);
-        (void)r0;
         return MLEM_OK;
     }
 
