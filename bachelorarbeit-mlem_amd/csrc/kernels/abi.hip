@@ -335,6 +335,7 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bo
         HIPCHK(fill_scratch.alloc(tile_fill_scratch_bytes(t->plan)));
         HIPCHK(launch_tile_fill(t->plan, rp, ci, va, t->mem.p, scratch.p, fill_scratch.p, &fl, nullptr));
     }
+    if (fl & 8u) return set_err(MLEM_ERR_INVALID_ARGUMENT, "uf16 values need a matrix without negative values");
     if (fl) return set_err(MLEM_ERR_FORMAT, "tiling (width %u, rows %u): layout check failed (flags %u)", W, R, fl);
     *out = t.get();
     m->tiles.emplace(key, std::move(t));
@@ -1401,7 +1402,7 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
     }
     ARGCHK(o.tile_width <= 1 || o.tile_width == 1024 || o.tile_width == 2048,
            "mlem_recon_create: tile_width must be 0 (auto), 1 (off), 1024 or 2048");
-    ARGCHK(o.values <= VAL_F16, "mlem_recon_create: values must be 0 (f32), 1 (bf16) or 2 (fp16)");
+    ARGCHK(o.values <= VAL_UF16, "mlem_recon_create: values must be 0 (f32), 1 (bf16), 2 (fp16) or 3 (uf16)");
     ARGCHK(o.values == VAL_F32 || (o.spmv == 0 && o.using_trans == 0 && r->bp_merge),
            "mlem_recon_create: 16-bit values need merge-based FP and BP on A^T (the blocked kernels)");
     ARGCHK(o.tile_rows == 0 || valid_tile_rows(o.tile_rows),
@@ -1560,7 +1561,7 @@ int mlem_autotune(mlem_matrix *m, uint32_t per_pass, uint32_t values, mlem_tune_
     std::memset(out, 0, sizeof(*out));
     const uint32_t BC = per_pass ? per_pass : 1u;
     ARGCHK(BC == 1 || BC == 4 || BC == 8, "mlem_autotune: per_pass must be 1, 4 or 8");
-    ARGCHK(values <= VAL_F16, "mlem_autotune: values must be 0 (f32), 1 (bf16) or 2 (fp16)");
+    ARGCHK(values <= VAL_UF16, "mlem_autotune: values must be 0 (f32), 1 (bf16), 2 (fp16) or 3 (uf16)");
     ARGCHK(m->has_T, "mlem_autotune: the transpose must be resident");
     HIPCHK(hipSetDevice(m->device));
     int dev = 0, cus = 256;
@@ -1705,7 +1706,7 @@ int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opt
         mlem_batch_opts_default(&o);
     const uint32_t BC = o.per_pass ? o.per_pass : 4u;
     ARGCHK(BC == 1 || BC == 4 || BC == 8, "mlem_batch_create: per_pass must be 1, 4 or 8");
-    ARGCHK(o.values <= VAL_F16, "mlem_batch_create: values must be 0 (f32), 1 (bf16) or 2 (fp16)");
+    ARGCHK(o.values <= VAL_UF16, "mlem_batch_create: values must be 0 (f32), 1 (bf16), 2 (fp16) or 3 (uf16)");
     ARGCHK(m->has_T, "mlem_batch_create: the transpose must be resident (upload it or build it)");
     const uint32_t W = o.tile_width ? o.tile_width : (BC == 1 ? 1024u : 512u);
     const uint32_t R = o.tile_rows ? o.tile_rows : (BC == 8 ? 1024u : 2048u);

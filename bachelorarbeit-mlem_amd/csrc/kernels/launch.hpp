@@ -100,7 +100,16 @@ constexpr uint32_t kTileE = 8;           // entries per work-item of a unit
 constexpr uint32_t kTileUnit = kWG * kTileE;
 // Stored value type of a tiled matrix: f32, or bf16 / fp16 holding
 // value / scale[row] with an f32 scale per row (the largest |value|).
-enum ValType : uint32_t { VAL_F32 = 0, VAL_BF16 = 1, VAL_F16 = 2 };
+// Stored values of the blocked kernels: f32, or 16 bits with a per-row f32
+// scale: BF16, FP16, or UF16, an unsigned 16-bit float for nonnegative
+// matrices (no sign bit: 4 exponent bits, 12 mantissa bits; value / row
+// maximum in [2^-15, 1], no zero: code 0 is 2^-15, padding multiplies a zero
+// x entry).
+enum ValType : uint32_t { VAL_F32 = 0, VAL_BF16 = 1, VAL_F16 = 2, VAL_UF16 = 3 };
+constexpr bool is_uf16(uint32_t vt) { return vt == VAL_UF16; }
+constexpr int kUf16ExpBits = 4;
+// The per-row scale of a 16-bit type: this times the row's largest |value|.
+inline float value_scale_mult(uint32_t vt) { return vt == VAL_F16 ? 1.0f / 16384.0f : 1.0f; }
 struct TilePlan {
     uint32_t rows = 0, cols = 0;
     uint64_t nnz = 0;

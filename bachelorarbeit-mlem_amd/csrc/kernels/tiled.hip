@@ -129,6 +129,19 @@ __device__ __forceinline__ Unit make_unit(const uint4 *tl, uint32_t ti, uint32_t
     return u;
 }
 
+// Unsigned 16-bit float with EB exponent bits: the f32 with the same
+// exponent and mantissa bits (the top binade holds the row maximum, 1.0).
+template <int EB>
+__device__ __forceinline__ float uf16_lo(uint32_t w) {  // bits 0-15 of w
+    constexpr int SH = 7 + EB, EMAX = (1 << EB) - 1;
+    return __uint_as_float(((w << 16) >> (16 - SH)) + ((uint32_t)(127 - EMAX) << 23));
+}
+template <int EB>
+__device__ __forceinline__ float uf16_hi(uint32_t w) {  // bits 16-31 of w
+    constexpr int SH = 7 + EB, EMAX = (1 << EB) - 1;
+    return __uint_as_float(((w >> (16 - SH)) & (0xffffu << SH)) + ((uint32_t)(127 - EMAX) << 23));
+}
+
 // Value j (0..7) of a work-item's group from its loaded 16 bytes: f32
 // values come in two halves (entries 0-3, then 4-7), 16-bit values in one
 // load, 8 to a work-item.
@@ -139,6 +152,9 @@ __device__ __forceinline__ float group_val(const f32x4 &h0, const f32x4 &h1, con
     } else if constexpr (VT == VAL_BF16) {
         const uint32_t wv = q[j / 2];
         return __uint_as_float((j & 1) ? (wv & 0xffff0000u) : (wv << 16));
+    } else if constexpr (is_uf16(VT)) {
+        const uint32_t wv = q[j / 2];
+        return (j & 1) ? uf16_hi<kUf16ExpBits>(wv) : uf16_lo<kUf16ExpBits>(wv);
     } else {
         const uint32_t wv = q[j / 2];
         return __half2float(__ushort_as_half((unsigned short)((j & 1) ? (wv >> 16) : (wv & 0xffffu))));
@@ -755,6 +771,22 @@ __device__ __forceinline__ uint16_t to_bf16(float f) {
 
 __device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float((uint32_t)b << 16); }
 
+// t in [0, 1] -> unsigned 16-bit float with EB exponent bits, mantissa
+// rounded to nearest even (every value from 2^-(2^EB - 1) up within 2^-(17 -
+// EB) of itself); below the smallest binade: code 0.
+__device__ __forceinline__ uint16_t to_uf16(float t, int EB) {
+    const int SH = 7 + EB, EMAX = (1 << EB) - 1;
+    const uint32_t u = __float_as_uint(t);
+    const uint32_t ur = u + ((1u << (SH - 1)) - 1u) + ((u >> SH) & 1u);
+    const int32_t c = (int32_t)(ur >> SH) - ((127 - EMAX) << (16 - EB));
+    return (uint16_t)(c < 0 ? 0 : c);
+}
+
+__device__ __forceinline__ float from_uf16(uint16_t c, int EB) {
+    const int SH = 7 + EB, EMAX = (1 << EB) - 1;
+    return __uint_as_float(((uint32_t)c << SH) + ((uint32_t)(127 - EMAX) << 23));
+}
+
 // 16-bit values, each value / scale of its row rounded to nearest (even).
 // Rounding every value on its own keeps each entry within half a unit in the
 // last place; carrying the rounding error along the row (error feedback)
@@ -762,16 +794,25 @@ __device__ __forceinline__ float from_bf16(uint16_t b) { return __uint_as_float(
 // which the back projection and the sensitivity of the voxels behind it do
 // not forgive (measured: rel-L2 up to 0.13 after 100 iterations on the 3D
 // scanner matrix, against at most 3e-3 for FP16 without it).
+// Unsigned 16-bit floats take nonnegative values only: bit 8 of *bad for a
+// negative one.
 __global__ void quantize_rows_kernel(const uint64_t *__restrict__ rowptr, const float *__restrict__ val,
                                      uint32_t rows, const float *__restrict__ scale, uint32_t vt,
-                                     uint16_t *__restrict__ q) {
+                                     uint16_t *__restrict__ q, uint32_t *__restrict__ bad) {
     const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
     if (row >= rows) return;
     const float inv = 1.0f / scale[row];
+    bool neg = false;
     for (uint64_t k = rowptr[row]; k < rowptr[row + 1]; ++k) {
         const float t = val[k] * inv;
-        q[k] = vt == VAL_BF16 ? to_bf16(t) : __half_as_ushort(__float2half_rn(t));
+        if (is_uf16(vt)) {
+            neg = neg || t < 0.0f;
+            q[k] = to_uf16(fminf(fmaxf(t, 0.0f), 1.0f), kUf16ExpBits);
+        } else {
+            q[k] = vt == VAL_BF16 ? to_bf16(t) : __half_as_ushort(__float2half_rn(t));
+        }
     }
+    if (neg && bad) atomicOr(bad, 8u);
 }
 
 // Places every entry: its value straight into the unit it falls in, its
@@ -917,7 +958,9 @@ __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restri
 // codes[k] = the 16-bit value q[k] as f32 (without its row's scale).
 __global__ void decode_codes_kernel(const uint16_t *__restrict__ q, uint64_t n, uint32_t vt, float *__restrict__ codes) {
     for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x)
-        codes[k] = vt == VAL_BF16 ? from_bf16(q[k]) : __half2float(__ushort_as_half(q[k]));
+        codes[k] = vt == VAL_BF16   ? from_bf16(q[k])
+                   : is_uf16(vt) ? from_uf16(q[k], kUf16ExpBits)
+                                 : __half2float(__ushort_as_half(q[k]));
 }
 
 __global__ void fill_u16_kernel(uint16_t *__restrict__ p, uint16_t v, uint64_t n) {
@@ -1260,10 +1303,10 @@ hipError_t launch_tile_fill_into(const TilePlan &p, const uint64_t *rowptr, cons
             if ((e = launch_fill(tm.scale, 1.0f, p.rows, s)) != hipSuccess) return e;
         } else {
             hipLaunchKernelGGL(row_absmax_kernel, dim3(blocks_for(p.rows, kWG / 64)), dim3(kWG), 0, s, rowptr, val,
-                               p.rows, p.vt == VAL_F16 ? 1.0f / 16384.0f : 1.0f, tm.scale);
+                               p.rows, value_scale_mult(p.vt), tm.scale);
         }
         hipLaunchKernelGGL(quantize_rows_kernel, dim3(blocks_for(p.rows, 64)), dim3(64), 0, s, rowptr, val, p.rows,
-                           tm.scale, p.vt, qval);
+                           tm.scale, p.vt, qval, c.bad);
     }
     if ((e = hipMemsetAsync(rtmp, 0xff, 2 * p.cap, s)) != hipSuccess) return e;
     hipLaunchKernelGGL(fill_u16_kernel, dim3(1024), dim3(kWG), 0, s, ctmp, (uint16_t)p.W, p.cap);
@@ -1303,6 +1346,7 @@ hipError_t launch_tiled_partials(const TilePlan &p, const TiledMatrix &tm, const
     switch (p.vt) {
         case VAL_F32: return launch_vt<VAL_F32>(p, tm, wk, x, part, BC, nt, s);
         case VAL_BF16: return launch_vt<VAL_BF16>(p, tm, wk, x, part, BC, nt, s);
+        case VAL_UF16: return launch_vt<VAL_UF16>(p, tm, wk, x, part, BC, nt, s);
         default: return launch_vt<VAL_F16>(p, tm, wk, x, part, BC, nt, s);
     }
 }
@@ -1349,9 +1393,9 @@ hipError_t launch_quantize_codes(const uint64_t *rowptr, const float *val, uint3
                                  float *scale, float *codes, uint16_t *q, hipStream_t s) {
     if (rows == 0) return hipSuccess;
     hipLaunchKernelGGL(row_absmax_kernel, dim3(blocks_for(rows, kWG / 64)), dim3(kWG), 0, s, rowptr, val, rows,
-                       vt == VAL_F16 ? 1.0f / 16384.0f : 1.0f, scale);
+                       value_scale_mult(vt), scale);
     hipLaunchKernelGGL(quantize_rows_kernel, dim3(blocks_for(rows, 64)), dim3(64), 0, s, rowptr, val, rows, scale, vt,
-                       q);
+                       q, nullptr);
     if (nnz) {
         const uint64_t blocks = std::min<uint64_t>((nnz + kWG - 1) / kWG, 65536);
         hipLaunchKernelGGL(decode_codes_kernel, dim3((uint32_t)blocks), dim3(kWG), 0, s, q, nnz, vt, codes);

@@ -97,7 +97,7 @@ def workload(name, threads):
     return geo, A, g, label
 
 
-VALUES = {"f32": 0, "bf16": 1, "fp16": 2}
+VALUES = {"f32": 0, "bf16": 1, "fp16": 2, "uf16": 3}
 
 
 def run_batch(args, M, geo, A, g0, m, s32, threads, label):
@@ -340,8 +340,8 @@ def main():
     ap.add_argument("--batch", type=int, default=0,
                     help="batched mode: B sinograms (seeded phantoms, counts log-uniform 1e6-1e9) over the matrix")
     ap.add_argument("--per-pass", type=int, default=4, help="batched mode: sinograms per pass over the matrix")
-    ap.add_argument("--values", default=None, choices=["f32", "bf16", "fp16"],
-                    help="stored matrix values (default: f32 for one reconstruction, fp16 for --batch and --serve)")
+    ap.add_argument("--values", default=None, choices=["f32", "bf16", "fp16", "uf16"],
+                    help="stored matrix values (default: f32 for one reconstruction, uf16 for --batch and --serve)")
     ap.add_argument("--recon-iters", type=int, default=100, help="iterations per reconstruction (recon/s)")
     ap.add_argument("--batch-check", type=int, default=4,
                     help="batched mode: sinograms whose full reconstructions are compared with the FP32 path")
@@ -358,7 +358,7 @@ def main():
                     help="also run a 1 GiB device-to-device copy (HBM counter calibration)")
     args = ap.parse_args()
     if args.values is None:
-        args.values = "fp16" if (args.batch or args.serve) else "f32"
+        args.values = "uf16" if (args.batch or args.serve) else "f32"
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:

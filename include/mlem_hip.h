@@ -198,10 +198,15 @@ typedef struct mlem_opts {
                              length over each row block's tiles):
                              0 = 24 per CU                                   */
     uint32_t values;      /* stored matrix values of the blocked kernels:
-                             0 = f32, 1 = bf16, 2 = fp16 (16-bit values are
-                             value / scale rounded to nearest, with an f32
-                             scale per row of A (the row's largest value, or
-                             2^-14 of it for fp16); the
+                             0 = f32, 1 = bf16, 2 = fp16, 3 = uf16 (an
+                             unsigned 16-bit float for nonnegative matrices:
+                             4 exponent and 12 mantissa bits, every value
+                             from 2^-15 of its row's largest up within 2^-13
+                             of itself, smaller ones stored as 2^-15 of it;
+                             negative values are refused).  16-bit values
+                             are value / scale rounded to nearest, with an
+                             f32 scale per row of A (the row's largest
+                             |value|, or 2^-14 of it for fp16); the
                              back projection uses the same codes, and the
                              sensitivity is this quantized matrix's column
                              sums in FP64 (a sensitivity passed in is not
@@ -330,7 +335,8 @@ int mlem_recon_destroy(mlem_recon *r);
  * they are.  No counterpart in the reference (one sinogram per process). */
 typedef struct mlem_batch mlem_batch;
 typedef struct mlem_batch_opts {
-    uint32_t values;      /* stored matrix values: 0 f32, 1 bf16, 2 fp16   */
+    uint32_t values;      /* stored matrix values: 0 f32, 1 bf16, 2 fp16,
+                             3 uf16 (as in mlem_opts)                     */
     uint32_t per_pass;    /* slots per pass over the matrix: 1, 4, 8
                              (0 = 4)                                       */
     uint32_t tile_width;  /* 0 = automatic (1024 for 1 per pass, else 512) */
@@ -371,7 +377,7 @@ int mlem_batch_destroy(mlem_batch *b);
 
 /* Times the blocked FP and BP kernels of the matrix over a few tile shapes
  * (width x rows) and workgroup counts for `per_pass` right-hand sides (1, 4
- * or 8) and `values` (0 f32, 1 bf16, 2 fp16), and reports the fastest for
+ * or 8) and `values` (0 f32, 1 bf16, 2 fp16, 3 uf16), and reports the fastest for
  * mlem_opts / mlem_batch_opts (tile_width, tile_rows, tile_groups).  Tiles
  * built for candidates that were not chosen are freed again.  No counterpart
  * in the reference (its section sizes come from argv[5]/[6]). */

@@ -4,8 +4,9 @@ resident matrix) against the FP64 oracle, on the 2D ring matrix.
 Bars: f32 values 1e-5 rel-L2; fp16 values 1e-3; bf16 values 2e-3 (8-bit
 significand: the stored matrix differs from the FP32 one by up to 2^-9 per
 value, each value rounded to nearest after the per-row scale; the products,
-sums, ratio and update stay FP32); a slot's image is bitwise the same alone
-and in a full batch."""
+sums, ratio and update stay FP32); uf16 values 1e-4 (an unsigned 16-bit
+float, 12 mantissa bits: every value within 2^-13 of itself); a slot's image
+is bitwise the same alone and in a full batch."""
 import itertools
 
 import numpy as np
@@ -17,7 +18,7 @@ import oracle as O
 pytestmark = pytest.mark.gpu
 
 ITERS = 30
-BAR = {0: 1e-5, 1: 2e-3, 2: 1e-3}
+BAR = {0: 1e-5, 1: 2e-3, 2: 1e-3, 3: 1e-4}
 
 
 def rel_l2(a, b):
@@ -41,7 +42,7 @@ def setup(c1):
     m.close()
 
 
-@pytest.mark.parametrize("values", [1, 2])
+@pytest.mark.parametrize("values", [1, 2, 3])
 def test_single_reconstruction_with_16bit_values(c1, setup, values):
     bar = BAR[values]
     print(f"single values={values}", flush=True)
@@ -60,7 +61,8 @@ def test_16bit_values_need_the_blocked_path(setup):
 @pytest.mark.parametrize("per_pass,width,values,threads",
                          [(1, 1024, 0, 0), (4, 512, 0, 0), (4, 1024, 0, 0), (8, 512, 0, 0), (4, 512, 1, 0),
                           (4, 512, 2, 0), (8, 512, 2, 0), (4, 1024, 1, 0), (1, 1024, 2, 0), (4, 512, 0, 256),
-                          (4, 512, 0, 512), (8, 512, 2, 256), (4, 1024, 1, 512)])
+                          (4, 512, 0, 512), (8, 512, 2, 256), (4, 1024, 1, 512), (4, 512, 3, 0), (8, 512, 3, 0),
+                          (1, 1024, 3, 0)])
 def test_batch_matches_oracle(setup, per_pass, width, values, threads):
     print(f"batch per_pass={per_pass} W={width} values={values} threads={threads}", flush=True)
     n = len(setup["sinos"])
@@ -111,6 +113,20 @@ def test_deactivated_slot_is_frozen_and_slots_are_reusable(setup):
         assert info["per_pass"] == 4 and info["passes"] == 2 and info["device_bytes"] > 0
 
 
+def test_uf16_values_refuse_negative_matrices():
+    A = M.Csr(3, 4, np.array([0, 2, 3, 5], np.uint64), np.array([0, 2, 1, 0, 3], np.uint32),
+              np.array([1.0, 0.5, -0.25, 2.0, 1.0], np.float32))
+    g = np.array([3, 1, 2], np.int32)
+    f0 = np.ones(4, np.float32)
+    with M.ResidentMatrix(A, M.transpose(A)) as m:
+        with pytest.raises(M.MlemError):
+            M.Batch(m, 4, values=3)
+        with pytest.raises(M.MlemError):
+            M.Recon(m, g, f0, options=M.opts(spmv=0, values=3))
+        with M.Batch(m, 4, values=2) as b:  # fp16 keeps the sign
+            b.set(0, g, f0)
+
+
 def test_batch_argument_errors(setup, c1):
     with pytest.raises(M.MlemError):
         M.Batch(setup["m"], 4, per_pass=3)
@@ -126,7 +142,7 @@ def test_batch_argument_errors(setup, c1):
             b.set(2, setup["sinos"][0], setup["f0"])
 
 
-@pytest.mark.parametrize("per_pass,values", [(4, 0), (4, 2), (1, 0)])
+@pytest.mark.parametrize("per_pass,values", [(4, 0), (4, 3), (1, 0)])
 def test_graph_replays_equal_eager_iterations(setup, per_pass, values):
     """use_graph: every mlem_batch_iterate call replays one captured
     iteration; after each call (eager image reads and host syncs between
