@@ -421,7 +421,7 @@ def main():
     fp_merge = args.fp == "merge"
     o = M.opts(spmv=0 if fp_merge else 1, using_trans=0, secsize_fp=args.secsize, secsize_bp=args.secsize,
                profile=0 if args.graph else 1, use_graph=args.graph, tile_width=args.tile,
-               tile_rows=args.tile_rows, tile_groups=args.tile_groups)
+               tile_rows=args.tile_rows, tile_groups=args.tile_groups, values=VALUES[args.values])
     o.bp_spmv = 1 if args.bp == "merge" else 2
     t = time.time()
     with M.Recon(m, g, f0, sens=s32, options=o) as r:
@@ -444,9 +444,20 @@ def main():
                 f"{ll1:.9e} ({'increased' if ll1 >= ll0 else 'DECREASED'})")
     elapsed = t1 - t0
     its = args.steps / elapsed
+    rel_vs_f32 = None
+    if args.values != "f32":  # the same iterations with f32 values: the image error of the 16-bit matrix
+        o32 = M.opts(spmv=0, using_trans=0, profile=0, tile_width=args.tile, tile_rows=args.tile_rows,
+                     tile_groups=args.tile_groups)
+        with M.Recon(m, g, f0, sens=s32, options=o32) as r32:
+            r32.iterate(args.warmup + args.steps)
+            ref32 = r32.image()
+        rel_vs_f32 = float(np.linalg.norm(img.astype(np.float64) - ref32) / np.linalg.norm(ref32.astype(np.float64)))
+        log(f"{args.values} values: rel-L2 vs the f32-value image after {args.warmup + args.steps} iterations "
+            f"{rel_vs_f32:.2e}")
     rows, cols, nnz = A.rows, A.cols, A.nnz
-    b_fp = 8 * nnz + 8 * (rows + 1) + 4 * cols + 8 * rows          # A, rowptr, x, g, ratio write
-    b_bp = 8 * nnz + 8 * (cols + 1) + 4 * rows + 12 * cols          # A^T, rowptr, ratio, f r/w, sens
+    vb = 4 if args.values == "f32" else 2  # bytes per stored value (16-bit values add a 4-byte scale per row of A)
+    b_fp = (vb + 4) * nnz + 8 * (rows + 1) + 4 * cols + 8 * rows + (4 * rows if vb == 2 else 0)
+    b_bp = (vb + 4) * nnz + 8 * (cols + 1) + 4 * rows + 12 * cols  # A^T, rowptr, ratio, f r/w, sens
     b_iter = b_fp + b_bp
     log(f"{args.steps} iterations in {elapsed * 1e3:.1f} ms: {its:.1f} it/s, {b_iter * its / 1e9:.0f} GB/s; "
         f"sum f = {float(np.sum(img, dtype=np.float32)):.6g}")
@@ -455,8 +466,8 @@ def main():
     blocked = args.tile != 1
     if n_timed:
         ph = [v / n_timed for v in phase]  # FP kernel, FP epilogue, BP kernel, BP epilogue (ms)
-        k_fp = 8 * nnz + 8 * (rows + 1) + 4 * cols
-        k_bp = 8 * nnz + 8 * (cols + 1) + 4 * rows
+        k_fp = (vb + 4) * nnz + 8 * (rows + 1) + 4 * cols
+        k_bp = (vb + 4) * nnz + 8 * (cols + 1) + 4 * rows
         gbs = lambda b, ms: b / (ms * 1e-3) / 1e9 if ms > 0 else 0.0
         log(f"FP kernel {ph[0]:.3f} ms ({gbs(k_fp, ph[0]):.0f} GB/s) + epilogue {ph[1]:.3f} ms; "
             f"BP kernel {ph[2]:.3f} ms ({gbs(k_bp, ph[2]):.0f} GB/s) + epilogue {ph[3]:.3f} ms")
@@ -467,7 +478,7 @@ def main():
         ach = gbs(kb, ms)
         algo = args.fp if dom_fp else args.bp
         if algo == "merge" and blocked:
-            kname = "blocked_spmv_kernel (LDS-staged x segments, 6-byte entries)"
+            kname = f"blocked_spmv_kernel (LDS-staged x segments, {vb + 2}-byte entries)"
         elif algo == "merge":
             kname = "merge_spmv_kernel"
         else:
@@ -479,7 +490,7 @@ def main():
                     "phases_ms": {"fp_kernel": round(ph[0], 4), "fp_epilogue": round(ph[1], 4),
                                   "bp_kernel": round(ph[2], 4), "bp_epilogue": round(ph[3], 4)}}
         tr = (measured_traffic(args.workload, side, args.tile, args.tile_rows)
-              if (algo == "merge" and blocked and not args.tile_groups) else None)
+              if (algo == "merge" and blocked and not args.tile_groups and args.values == "f32") else None)
         if tr:
             roofline["traffic"] = tr["bytes"]
             roofline["traffic_source"] = tr["source"]
@@ -544,7 +555,8 @@ def main():
     out = {
         "metric": METRIC, "value": round(its, 3), "unit": "iters/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 4), "higher_is_better": True,
-        "scaling": "weak", "vs_baseline": None, "dtype": "f32", "data": "synthetic",
+        "scaling": "weak", "vs_baseline": None,
+        "dtype": "f32" if args.values == "f32" else f"{args.values} values, f32 arithmetic", "data": "synthetic",
         "hbm_gbs": round(b_iter * its / 1e9, 1),
         "config": {"workload": label, "rows": rows, "cols": cols, "nnz": nnz,
                    "matrix_gb": round(8 * nnz / 1e9, 2), "bytes_per_iteration": b_iter,
@@ -556,6 +568,8 @@ def main():
                                                     else "matrix and transpose resident")},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if rel_vs_f32 is not None:
+        out["rel_l2_vs_f32_values"] = {"iterations": args.warmup + args.steps, "value": rel_vs_f32}
     if load:
         out["load_csr4"] = load
     print(json.dumps(out), flush=True)
