@@ -191,3 +191,60 @@ def mlem_f32_omp(A, AT, g, sens, f0, iters, threads=0):
 
 def max_threads() -> int:
     return lib().oracle_max_threads()
+
+
+# ------------------------------------------------------------ 16-bit values
+# Restatement of this engine's 16-bit matrix values (the reference stores f32
+# only; its precision extension is the per-row scale of the engine's
+# documentation): scale_r = max_k |a_rk| (1 for an all-zero row) times 2^-14
+# for fp16, 1 otherwise (f32 product); t = a * (1 / scale_r) in f32; the code
+# is t rounded to nearest even in the type; the value the kernels use is
+# scale_r * decode(code).  Pinned by known codes and ties
+# (tests/test_oracle.py::test_quantize_values_known_codes).
+
+def bf16_code(t):
+    u = np.asarray(t, np.float32).view(np.uint32).astype(np.uint64)
+    return ((u + 0x7FFF + ((u >> 16) & 1)) >> 16).astype(np.uint16)
+
+
+def bf16_value(c):
+    return (np.asarray(c, np.uint32) << 16).astype(np.uint32).view(np.float32)
+
+
+def uf16_code(t, exp_bits=4):
+    """Unsigned 16-bit float: exp_bits exponent and 16 - exp_bits mantissa bits, t in [0, 1] (clamped), the top
+    binade holds 1.0, code 0 is the smallest value 2^-(2^exp_bits - 1) (no zero)."""
+    sh, emax = 7 + exp_bits, (1 << exp_bits) - 1
+    t = np.clip(np.asarray(t, np.float32), 0.0, 1.0).astype(np.float32)
+    u = t.view(np.uint32).astype(np.int64)
+    ur = u + ((1 << (sh - 1)) - 1) + ((u >> sh) & 1)
+    c = (ur >> sh) - ((127 - emax) << (16 - exp_bits))
+    return np.where(c < 0, 0, c).astype(np.uint16)
+
+
+def uf16_value(c, exp_bits=4):
+    sh, emax = 7 + exp_bits, (1 << exp_bits) - 1
+    return ((np.asarray(c, np.uint32) << sh) + ((127 - emax) << 23)).astype(np.uint32).view(np.float32)
+
+
+def quantize_values(rowptr, val, rows, kind):
+    """The values the 16-bit kernels multiply with, as f64 (scale_r * decode(code)): kind 1 bf16, 2 fp16, 3 uf16."""
+    rowptr = _c(rowptr, np.uint64).astype(np.int64)
+    val = _c(val, np.float32)
+    lens = np.diff(rowptr)
+    row_of = np.repeat(np.arange(rows), lens)
+    amax = np.zeros(rows, np.float32)
+    np.maximum.at(amax, row_of, np.abs(val))
+    mult = np.float32(2.0 ** -14) if kind == 2 else np.float32(1.0)
+    scale = (np.where(amax > 0, amax, np.float32(1.0)).astype(np.float32) * mult).astype(np.float32)
+    inv = (np.float32(1.0) / scale).astype(np.float32)
+    t = (val * inv[row_of]).astype(np.float32)
+    if kind == 1:
+        dec = bf16_value(bf16_code(t))
+    elif kind == 2:
+        dec = t.astype(np.float16).astype(np.float32)
+    elif kind == 3:
+        dec = uf16_value(uf16_code(t))
+    else:
+        raise ValueError(kind)
+    return scale[row_of].astype(np.float64) * dec.astype(np.float64)

@@ -349,3 +349,31 @@ def test_loglik_hand_worked():
               np.array([1, 2, 3], np.float32))
     ll = O.loglik_f64(A.rowptr, A.col, A.val, 3, np.array([4, 5, 6], np.int32), np.array([1.0, 2.0]))
     assert abs(ll - (4 * np.log(5) - 5 + 6 * np.log(6) - 6)) < 1e-12
+
+
+# ------------------------------------------------------------ 16-bit values
+def test_quantize_values_known_codes():
+    """The 16-bit restatement against known codes, round-to-nearest-even ties
+    and the closed-form error bounds."""
+    f = np.float32
+    # bf16: 1.0, a tie to even below, a tie to even above
+    assert list(O.bf16_code(np.array([1.0, 1 + 2 ** -8, 1 + 3 * 2 ** -8], f))) == [0x3F80, 0x3F80, 0x3F82]
+    assert O.bf16_value(np.array([0x3F81]))[0] == f(1 + 2 ** -7)
+    # uf16 (4 exponent bits, 12 mantissa bits, 1.0 in the top binade)
+    assert list(O.uf16_code(np.array([1.0, 0.75, 2 ** -15, 0.0, 2.0], f))) == [0xF000, 0xE800, 0, 0, 0xF000]
+    t_tie_even, t_tie_up = f(0.5 * (1 + 2 ** -13)), f(0.5 * (1 + 3 * 2 ** -13))
+    assert list(O.uf16_code(np.array([t_tie_even, t_tie_up]))) == [0xE000, 0xE002]
+    assert O.uf16_value(np.array([0xE001]))[0] == f(0.5 * (1 + 2 ** -12))
+    rng = np.random.default_rng(5)
+    t = np.concatenate([rng.random(20000), 2.0 ** rng.uniform(-15, 0, 20000)]).astype(f)
+    t = t[t >= 2 ** -15]
+    assert np.max(np.abs(O.uf16_value(O.uf16_code(t)) - t) / t) <= 2 ** -13
+    assert np.max(np.abs(O.bf16_value(O.bf16_code(t)) - t) / t) <= 2 ** -8
+    # whole rows: exact where the ratios are binary fractions; scale 1 for an empty row
+    A = M.Csr(3, 4, np.array([0, 3, 3, 5], np.uint64), np.array([0, 1, 3, 0, 2], np.uint32),
+              np.array([4.0, 1.0, 0.5, 3.0, 1.0], np.float32))
+    for kind in (1, 2, 3):
+        q = O.quantize_values(A.rowptr, A.val, A.rows, kind)
+        assert list(q[:3]) == [4.0, 1.0, 0.5], kind
+        rel = abs(q[4] - 1.0)
+        assert 0 < rel <= {1: 2 ** -8, 2: 2 ** -11, 3: 2 ** -13}[kind], (kind, rel)
