@@ -5,6 +5,7 @@
 //        <trans: 0 = BP on A^T, 1 = scatter> <spmv: 0 = merge, 1 = CSR-vector>
 //        [--out=image.f32] [--sens=gpu64|host32] [--transpose=host|gpu]
 //        [--x0=ref|one] [--graph=0|1] [--threads=N] [--loglik] [--verbose]
+//        [--values=f32|bf16|fp16|uf16]
 //
 // Same checks as main.cu:43-62 (exactly 8 positional arguments, iters, nGPU
 // and both section sizes parsed with strtol and != 0, trans and spmv in
@@ -40,7 +41,8 @@ static void help() {
     printf("argv[7]: BP uses transposed matrix?      0: yes                1: no\n");
     printf("argv[8]: which SpMV algorithm to use?    0: merge-based SpMV   1: csr-vector SpMV\n");
     printf("options after the 8 arguments: --out=PATH --sens=gpu64|host32 --transpose=host|gpu\n");
-    printf("                               --x0=ref|one --graph=0|1 --threads=N --loglik --verbose\n\n");
+    printf("                               --x0=ref|one --graph=0|1 --threads=N --loglik --verbose\n");
+    printf("                               --values=f32|bf16|fp16|uf16 (stored matrix values, merge-based only)\n\n");
 }
 
 struct Timer {
@@ -78,7 +80,7 @@ int main(int argc, char **argv) {
     unsigned int matrix_vector_mul = (unsigned int)strtol(argv[8], nullptr, 10);
     if (matrix_vector_mul != 0 && matrix_vector_mul != 1) { help(); return EXIT_FAILURE; }
 
-    std::string out_path, sens_mode = "gpu64", trans_mode = "host", x0_mode = "ref";
+    std::string out_path, sens_mode = "gpu64", trans_mode = "host", x0_mode = "ref", values_mode = "f32";
     int graph = 0, threads = 0;
     bool verbose = false, want_ll = false;
     for (int a = 9; a < argc; ++a) {
@@ -93,12 +95,17 @@ int main(int argc, char **argv) {
         else if (const char *v = val("--x0=")) x0_mode = v;
         else if (const char *v = val("--graph=")) graph = std::atoi(v);
         else if (const char *v = val("--threads=")) threads = std::atoi(v);
+        else if (const char *v = val("--values=")) values_mode = v;
         else if (s == "--verbose") verbose = true;
         else if (s == "--loglik") want_ll = true;
         else { help(); return EXIT_FAILURE; }
     }
+    const char *value_names[] = {"f32", "bf16", "fp16", "uf16"};
+    int values = -1;
+    for (int k = 0; k < 4; ++k)
+        if (values_mode == value_names[k]) values = k;
     if ((sens_mode != "gpu64" && sens_mode != "host32") || (trans_mode != "host" && trans_mode != "gpu") ||
-        (x0_mode != "ref" && x0_mode != "one")) {
+        (x0_mode != "ref" && x0_mode != "one") || values < 0) {
         help();
         return EXIT_FAILURE;
     }
@@ -245,6 +252,7 @@ int main(int argc, char **argv) {
     o.secsize_fp = secsize_fp;
     o.secsize_bp = secsize_bp;
     o.use_graph = graph ? 1 : 0;
+    o.values = (uint32_t)values;  // 16-bit values: the quantized matrix's own sensitivity is used
     mlem_recon *R = nullptr;
     st = mlem_recon_create(M, g, norm.data(), f0.data(), &o, &R);
     if (st == MLEM_OK) {

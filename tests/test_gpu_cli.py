@@ -52,6 +52,22 @@ def test_cli_reconstruction_matches_oracle(c1, files, trans, spmv, extra):
     assert r.stdout.endswith(f"\nSum f: {s:f}\n")
 
 
+def test_cli_16bit_values(c1, files):
+    """--values=uf16: the blocked kernels on the quantized matrix (its own FP64 sensitivity), within 1e-4 of the
+    FP64 oracle on the f32 matrix after 20 iterations; a bad value name prints the help."""
+    A, g = c1["A"], c1["g"]
+    out = str(files["dir"] / "img_uf16.f32")
+    r = run(files["matrix"], files["sino"], "20", "1", "6", "7", "0", "0", f"--out={out}", "--values=uf16")
+    assert r.returncode == 0, r.stderr
+    img = M.read_raw(out, np.float32)
+    s64 = O.colsum_f64(A.rowptr, A.col, A.val, A.rows, A.cols)
+    f0, _ = M.initial_image(g, A.rows, s64.astype(np.float32))
+    ref = O.mlem_f64(A.rowptr, A.col, A.val, A.rows, A.cols, g, s64.astype(np.float32).astype(np.float64), f0, 20)
+    assert np.linalg.norm(img - ref) / np.linalg.norm(ref) <= 1e-4
+    r = run(files["matrix"], files["sino"], "2", "1", "2", "2", "0", "0", "--values=int8")
+    assert r.returncode == 1 and "--values" in r.stdout
+
+
 def test_cli_streamed_load_equals_host_path(c1, files):
     # --transpose=gpu with the GPU sensitivity streams the csr4 file to the
     # device (no host CSR); the image is bitwise the one of the host path.
