@@ -172,7 +172,7 @@ struct mlem_matrix {
     // tiles of A and A^T (width tile_W, tile_R rows, f32) and the
     // sensitivity taken while they were built.
     bool tiles_only = false;
-    uint32_t tile_W = 0, tile_R = 0;
+    uint32_t tile_W = 0, tile_R = 0, tile_vt = 0;
     DevBuf sens32, sens64;
     std::mutex mu;
     std::map<std::pair<int, uint32_t>, std::unique_ptr<Partition>> parts;  // (0: A / 1: A^T, secsize)
@@ -275,8 +275,8 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bo
         return MLEM_OK;
     }
     if (m->tiles_only)
-        return set_err(MLEM_ERR_UNSUPPORTED, "a tiles-only matrix holds tiles of width %u and %u rows (f32) only",
-                       m->tile_W, m->tile_R);
+        return set_err(MLEM_ERR_UNSUPPORTED, "a tiles-only matrix holds tiles of width %u and %u rows (values %u) only",
+                       m->tile_W, m->tile_R, m->tile_vt);
     auto ref = m->tile_refused.find(key);
     if (ref != m->tile_refused.end()) {
         *flags = ref->second;
@@ -358,8 +358,8 @@ void progress(const char *fmt, ...) {
     va_end(ap);
 }
 
-// Tiles (f32 values, one right-hand side) of a matrix that is not resident
-// as one CSR: given as chunks of whole row blocks, each a CSR of its own
+// Tiles (one right-hand side; f32 or 16-bit values) of a matrix that is not
+// resident as one CSR: given as chunks of whole row blocks, each a CSR of its own
 // (rows r0 .. r0+rows_c, row pointers from 0) on the device.  Two passes
 // over the same row blocks (the chunks may differ between them): the counts
 // give every tile's size and so the offsets of the whole layout; then, the
@@ -376,7 +376,8 @@ struct ChunkedTiles {
     // The chunk's own plan, its count into `scratch` (allocated here).
     int count_chunk(uint32_t r0, uint32_t rows_c, uint64_t nnz_c, const uint64_t *rp, const uint32_t *ci,
                     TilePlan *pc, DevBuf *scratch) {
-        *pc = tile_plan(rows_c, P.cols, nnz_c, P.W, P.R, VAL_F32, false);
+        *pc = tile_plan(rows_c, P.cols, nnz_c, P.W, P.R, P.vt, false);
+        pc->unit_scale = P.unit_scale;
         HIPCHK(scratch->alloc(tile_count_scratch_bytes(*pc)));
         uint32_t fl = 0;
         HIPCHK(launch_tile_count(*pc, rp, ci, scratch->p, &fl, nullptr));
@@ -424,9 +425,10 @@ struct ChunkedTiles {
         v.tile_off = local_off.as<uint64_t>();
         v.tile_nz = t->view.tile_nz + t0;
         v.tdata = t->view.tdata + (uint64_t)(P.vb() + 2) * size[t0];
-        v.scale = nullptr;
+        v.scale = t->view.scale ? t->view.scale + r0 : nullptr;
         uint32_t fl = 0;
         HIPCHK(launch_tile_fill_into(pc, rp, ci, va, v, scratch.p, fill_scratch.p, &fl, nullptr));
+        if (fl & 8u) return set_err(MLEM_ERR_INVALID_ARGUMENT, "uf16 values need a matrix without negative values");
         if (fl) return set_err(MLEM_ERR_FORMAT, "tiling: layout check failed (flags %u, rows from %u)", fl, r0);
         return MLEM_OK;
     }
@@ -997,7 +999,7 @@ int mlem_matrix_create(int device, uint32_t rows, uint32_t cols, uint64_t nnz, c
 
 int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
                              const uint32_t *col, const float *val, uint32_t tile_width, uint32_t tile_rows,
-                             uint64_t chunk_bytes, mlem_matrix **out) {
+                             uint32_t values, uint64_t chunk_bytes, mlem_matrix **out) {
     ARGCHK(out, "mlem_matrix_create_tiled: null out");
     *out = nullptr;
     ARGCHK(rowptr && (nnz == 0 || (col && val)), "mlem_matrix_create_tiled: null matrix array");
@@ -1006,6 +1008,7 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
     const uint32_t W = tile_width ? tile_width : 1024u, R = tile_rows ? tile_rows : 2048u;
     ARGCHK(valid_tile_config(W, 1), "mlem_matrix_create_tiled: tile_width must be 0, 1024 or 2048");
     ARGCHK(valid_tile_rows(R), "mlem_matrix_create_tiled: tile_rows must be 512 ... 16384 (a power of two)");
+    ARGCHK(values <= VAL_UF16, "mlem_matrix_create_tiled: values must be 0 (f32), 1 (bf16), 2 (fp16) or 3 (uf16)");
     HIPCHK(hipSetDevice(device));
     auto m = std::make_unique<mlem_matrix>();
     m->device = device;
@@ -1015,6 +1018,7 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
     m->tiles_only = true;
     m->tile_W = W;
     m->tile_R = R;
+    m->tile_vt = values;
     const size_t margin = (size_t)1 << 30;
     auto avail = [&](size_t *b) -> int {
         size_t free_b = 0, total_b = 0;
@@ -1027,8 +1031,11 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
     HIPCHK(m->sens64.alloc(sizeof(double) * cols));
     auto tT = std::make_unique<Tiled>();
     auto tA = std::make_unique<Tiled>();
-    ChunkedTiles bT(tile_plan(cols, rows, nnz, W, R, VAL_F32, false), tT.get());
-    ChunkedTiles bA(tile_plan(rows, cols, nnz, W, R, VAL_F32, false), tA.get());
+    TilePlan pT = tile_plan(cols, rows, nnz, W, R, values, false);
+    pT.unit_scale = values != VAL_F32;  // A^T's tiles hold A's codes (decoded below, in place)
+    ChunkedTiles bT(pT, tT.get());
+    ChunkedTiles bA(tile_plan(rows, cols, nnz, W, R, values, false), tA.get());
+    DevBuf scaleA;  // 16-bit values: A's row scales (for the sensitivity)
     {
         // A on the device; A^T's tiles from slabs of A^T's rows (column
         // ranges of A, whole row blocks of A^T), the sensitivity from the
@@ -1044,6 +1051,18 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
         }
         HIPCHK(colptrT.alloc(sizeof(uint64_t) * ((size_t)cols + 1)));
         HIPCHK(srp.alloc(sizeof(uint64_t) * ((size_t)cols + 1)));
+        if (values != VAL_F32) {  // A's values become their decoded codes: what A^T's tiles store
+            DevBuf bad;
+            HIPCHK(scaleA.alloc(sizeof(float) * rows));
+            HIPCHK(bad.alloc(sizeof(uint32_t)));
+            HIPCHK(launch_fill(bad.as<float>(), 0.0f, 1, nullptr));
+            HIPCHK(launch_quantize_in_place(rp.as<uint64_t>(), va.as<float>(), rows, values, scaleA.as<float>(),
+                                            bad.as<uint32_t>(), nullptr));
+            uint32_t fl = 0;
+            HIPCHK(hipMemcpy(&fl, bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
+            if (fl & 8u)
+                return set_err(MLEM_ERR_INVALID_ARGUMENT, "uf16 values need a matrix without negative values");
+        }
         for (int pass = 0; pass < 2; ++pass) {
             if (pass == 1)
                 if (int st = bT.allocate()) return st;
@@ -1053,8 +1072,13 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
                 hipError_t e = launch_rebase(colptrT.as<uint64_t>() + c0, off0, c1 - c0, srp.as<uint64_t>(), nullptr);
                 if (e != hipSuccess) return e;
                 if (pass == 0) {
-                    e = launch_colsum_rows_f64(srp.as<uint64_t>(), valT, c1 - c0, m->sens64.as<double>() + c0,
-                                               m->sens32.as<float>() + c0, nullptr);
+                    // FP64 column sums of A (of the quantized matrix: codes times row scales)
+                    e = values == VAL_F32
+                            ? launch_colsum_rows_f64(srp.as<uint64_t>(), valT, c1 - c0, m->sens64.as<double>() + c0,
+                                                     m->sens32.as<float>() + c0, nullptr)
+                            : launch_colsum_weighted_f64(srp.as<uint64_t>(), rowT, valT, scaleA.as<float>(), c1 - c0,
+                                                         m->sens64.as<double>() + c0, m->sens32.as<float>() + c0,
+                                                         nullptr);
                     if (e != hipSuccess) return e;
                     st_sink = bT.count(c0, c1 - c0, n, srp.as<uint64_t>(), rowT);
                 } else {
@@ -1122,8 +1146,12 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
     }
     HIPCHK(hipDeviceSynchronize());
     progress("A tiles %.1f GB", tA->mem.bytes / 1e9);
-    m->tiles.emplace(std::make_tuple(0, W, R, (uint32_t)VAL_F32, false), std::move(tA));
-    m->tiles.emplace(std::make_tuple(1, W, R, (uint32_t)VAL_F32, false), std::move(tT));
+    if (values != VAL_F32) {  // the quantized matrix's own sensitivity, as the resident path keeps it
+        HIPCHK(tT->sens.alloc(sizeof(float) * cols));
+        HIPCHK(hipMemcpy(tT->sens.p, m->sens32.p, sizeof(float) * cols, hipMemcpyDeviceToDevice));
+    }
+    m->tiles.emplace(std::make_tuple(0, W, R, values, false), std::move(tA));
+    m->tiles.emplace(std::make_tuple(1, W, R, values, false), std::move(tT));
     *out = m.release();
     return MLEM_OK;
 }
@@ -1356,12 +1384,15 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
         return set_err(MLEM_ERR_INVALID_ARGUMENT,
                        "mlem_recon_create: using_trans == 0 needs the transpose (upload it or build it)");
     if (m->tiles_only) {
-        ARGCHK(o.spmv == 0 && o.using_trans == 0 && o.bp_spmv <= 1 && o.values == VAL_F32,
-               "mlem_recon_create: a tiles-only matrix takes merge-based FP and BP on A^T with f32 values");
-        ARGCHK((o.tile_width == 0 || o.tile_width == m->tile_W) && (o.tile_rows == 0 || o.tile_rows == m->tile_R),
-               "mlem_recon_create: the matrix holds tiles of width %u and %u rows", m->tile_W, m->tile_R);
+        ARGCHK(o.spmv == 0 && o.using_trans == 0 && o.bp_spmv <= 1,
+               "mlem_recon_create: a tiles-only matrix takes merge-based FP and BP on A^T");
+        ARGCHK((o.tile_width == 0 || o.tile_width == m->tile_W) && (o.tile_rows == 0 || o.tile_rows == m->tile_R) &&
+                   (o.values == 0 || o.values == m->tile_vt),
+               "mlem_recon_create: the matrix holds tiles of width %u and %u rows with values %u", m->tile_W,
+               m->tile_R, m->tile_vt);
         o.tile_width = m->tile_W;
         o.tile_rows = m->tile_R;
+        o.values = m->tile_vt;
     }
     HIPCHK(hipSetDevice(m->device));
     auto r = std::make_unique<mlem_recon>();

@@ -154,6 +154,10 @@ TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint3
 // (q, 2 bytes per nonzero of scratch) and decoded to f32 without the scale.
 hipError_t launch_quantize_codes(const uint64_t *rowptr, const float *val, uint32_t rows, uint64_t nnz, uint32_t vt,
                                  float *scale, float *codes, uint16_t *q, hipStream_t s);
+// The same scales, and each value replaced by its decoded code in place (no
+// code scratch); *bad (device, may be null) bit 8: a negative value for uf16.
+hipError_t launch_quantize_in_place(const uint64_t *rowptr, float *val, uint32_t rows, uint32_t vt, float *scale,
+                                    uint32_t *bad, hipStream_t s);
 bool valid_tile_width(uint32_t W);
 // Kernel shapes built: (W, right-hand sides per pass) in {1024, 2048} x 1,
 // {512, 1024} x 4, 512 x 8.

@@ -955,6 +955,26 @@ __global__ __launch_bounds__(kWG) void tile_pack_kernel(const uint16_t *__restri
     if (big) atomicOr(bad, 4u);
 }
 
+// val[k] <- its 16-bit code as f32 (without the row's scale), in place: the
+// values the transposed tiles store (bit 8 of *bad as in quantize_rows_kernel).
+__global__ void quantize_in_place_kernel(const uint64_t *__restrict__ rowptr, float *__restrict__ val, uint32_t rows,
+                                         const float *__restrict__ scale, uint32_t vt, uint32_t *__restrict__ bad) {
+    const uint32_t row = blockIdx.x * blockDim.x + threadIdx.x;
+    if (row >= rows) return;
+    const float inv = 1.0f / scale[row];
+    bool neg = false;
+    for (uint64_t k = rowptr[row]; k < rowptr[row + 1]; ++k) {
+        const float t = val[k] * inv;
+        if (is_uf16(vt)) {
+            neg = neg || t < 0.0f;
+            val[k] = from_uf16(to_uf16(fminf(fmaxf(t, 0.0f), 1.0f), kUf16ExpBits), kUf16ExpBits);
+        } else {
+            val[k] = vt == VAL_BF16 ? from_bf16(to_bf16(t)) : __half2float(__float2half_rn(t));
+        }
+    }
+    if (neg && bad) atomicOr(bad, 8u);
+}
+
 // codes[k] = the 16-bit value q[k] as f32 (without its row's scale).
 __global__ void decode_codes_kernel(const uint16_t *__restrict__ q, uint64_t n, uint32_t vt, float *__restrict__ codes) {
     for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x)
@@ -1387,6 +1407,16 @@ hipError_t launch_tiled_reduce(const TilePlan &p, const TiledMatrix &tm, const T
         case 8: return launch_reduce_bc<8>(p, wk, part, tm.scale, epi, ea, s);
         default: return hipErrorInvalidValue;
     }
+}
+
+hipError_t launch_quantize_in_place(const uint64_t *rowptr, float *val, uint32_t rows, uint32_t vt, float *scale,
+                                    uint32_t *bad, hipStream_t s) {
+    if (rows == 0) return hipSuccess;
+    hipLaunchKernelGGL(row_absmax_kernel, dim3(blocks_for(rows, kWG / 64)), dim3(kWG), 0, s, rowptr, val, rows,
+                       value_scale_mult(vt), scale);
+    hipLaunchKernelGGL(quantize_in_place_kernel, dim3(blocks_for(rows, 64)), dim3(64), 0, s, rowptr, val, rows, scale,
+                       vt, bad);
+    return hipGetLastError();
 }
 
 hipError_t launch_quantize_codes(const uint64_t *rowptr, const float *val, uint32_t rows, uint64_t nnz, uint32_t vt,

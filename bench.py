@@ -378,7 +378,8 @@ def main():
     t = time.time()
     tiles_only = args.workload == "c3" and not args.c3_csr
     if tiles_only:
-        m = M.ResidentMatrix.tiles_only(A, tile_width=args.tile if args.tile > 1 else 0, tile_rows=args.tile_rows)
+        m = M.ResidentMatrix.tiles_only(A, tile_width=args.tile if args.tile > 1 else 0, tile_rows=args.tile_rows,
+                                        values=VALUES[args.values])
         log(f"tiles of A and A^T built without resident CSR copies (A^T's from column ranges of A on the device, "
             f"A's from row chunks of the host CSR) {time.time() - t:.2f} s; "
             f"{m.info()['device_bytes'] / 1e9:.1f} GB resident")
@@ -445,7 +446,7 @@ def main():
     elapsed = t1 - t0
     its = args.steps / elapsed
     rel_vs_f32 = None
-    if args.values != "f32":  # the same iterations with f32 values: the image error of the 16-bit matrix
+    if args.values != "f32" and not tiles_only:  # the same iterations with f32 values: the 16-bit image error
         o32 = M.opts(spmv=0, using_trans=0, profile=0, tile_width=args.tile, tile_rows=args.tile_rows,
                      tile_groups=args.tile_groups)
         with M.Recon(m, g, f0, sens=s32, options=o32) as r32:

@@ -47,6 +47,25 @@ def test_tiles_only_equals_resident_tiles(c1, width, rows, chunk):
     assert rel_l2(img, ref) <= 1e-5
 
 
+@pytest.mark.parametrize("values", [1, 2, 3])
+def test_tiles_only_16bit_values_equal_resident_tiles(c1, values):
+    """16-bit values: A's codes quantized on the device before the column
+    ranges (A^T's tiles hold them, the sensitivity is the quantized matrix's
+    FP64 column sums), A's tiles quantized again from the host chunks: the
+    images are bitwise those of the resident path with the same values."""
+    A, AT, g = c1["A"], c1["AT"], c1["g"]
+    f0 = np.ones(A.cols, np.float32)
+    with M.ResidentMatrix(A, AT) as m:
+        ref_img, _ = recon(m, g, f0, ITERS, tile_width=1024, tile_rows=512, values=values)
+    with M.ResidentMatrix.tiles_only(A, tile_width=1024, tile_rows=512, values=values, chunk_bytes=48 << 20) as t:
+        img, _ = recon(t, g, f0, ITERS)
+        img2, _ = recon(t, g, f0, ITERS, values=values)
+        with pytest.raises(M.MlemError):
+            recon(t, g, f0, 1, values=(values % 3) + 1)  # not the matrix's values
+    assert np.array_equal(img.view(np.uint32), ref_img.view(np.uint32))
+    assert np.array_equal(img2.view(np.uint32), ref_img.view(np.uint32))
+
+
 def test_tiles_only_edge_matrices():
     """Empty rows and columns, one 60,000-entry row, heavy tails, duplicate
     columns, no entries, one row, row gaps: bitwise the resident tiled path,
