@@ -1115,9 +1115,12 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
         uint32_t chunks = 0;
         for (uint32_t r0 = 0; r0 < rows; ++chunks) {
             uint32_t r1 = std::min<uint64_t>((uint64_t)r0 + R, rows);
+            // Per nonzero: the chunk's CSR (8 bytes), the fill's column and
+            // row scratch (4, with padding), 16-bit codes (2); per row: the
+            // per-segment counts and positions.
             auto need = [&](uint32_t a, uint32_t b) {
                 const uint64_t n = rowptr[b] - rowptr[a];
-                return n * 14 + (uint64_t)(b - a) * (nseg * 8 + 8) + ((size_t)64 << 20);
+                return n * (values == VAL_F32 ? 16 : 18) + (uint64_t)(b - a) * (nseg * 8 + 8) + ((size_t)256 << 20);
             };
             while (r1 < rows) {
                 const uint32_t nx = (uint32_t)std::min<uint64_t>((uint64_t)r1 + R, rows);
