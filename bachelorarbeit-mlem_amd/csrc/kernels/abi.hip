@@ -212,11 +212,9 @@ struct mlem_batch {
     hipGraphExec_t gexec = nullptr;
     std::vector<uint8_t> graph_passes;
     ~mlem_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);  // no replay still running
         if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (stream) {
-            (void)hipStreamSynchronize(stream);
-            (void)hipStreamDestroy(stream);
-        }
+        if (stream) (void)hipStreamDestroy(stream);
     }
 };
 
@@ -1334,6 +1332,7 @@ int mlem_matrix_info(const mlem_matrix *m, uint32_t *rows, uint32_t *cols, uint6
 int mlem_matrix_device_arrays(const mlem_matrix *m, const uint64_t **rowptr, const uint32_t **col, const float **val,
                               const uint64_t **rowptrT, const uint32_t **colT, const float **valT) {
     ARGCHK(m, "mlem_matrix_device_arrays: null matrix");
+    ARGCHK(!m->tiles_only, "mlem_matrix_device_arrays: a tiles-only matrix has no CSR arrays");
     if (rowptr) *rowptr = m->rowptr.as<uint64_t>();
     if (col) *col = m->col.as<uint32_t>();
     if (val) *val = m->val.as<float>();

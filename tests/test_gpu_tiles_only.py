@@ -98,5 +98,7 @@ def test_tiles_only_refuses_other_paths(c1):
             t.build_transpose()
         with pytest.raises(M.MlemError):
             t.autotune()
+        with pytest.raises(M.MlemError):
+            t.download()  # no CSR arrays on the device
     with pytest.raises(M.MlemError):
         M.ResidentMatrix.tiles_only(A, tile_width=512)
