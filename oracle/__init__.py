@@ -2,7 +2,7 @@
 reference's own sptrans compiled into oracle/_ref/.
 
 TEST INFRASTRUCTURE ONLY: imported by tests/, __graft_entry__.smoke() and the
-cpu_baseline leg of bench.py, as the checker.  The engine never imports it.
+cpu_baseline leg of bench.py, to check results against.  The engine never imports it.
 """
 from __future__ import annotations
 

@@ -1,7 +1,7 @@
 /*
  * mlem_oracle.c -- CPU restatement of the reference MLEM path.
  *
- * TEST INFRASTRUCTURE ONLY.  This file is the checker: only tests/,
+ * TEST INFRASTRUCTURE ONLY.  Results are checked against it: only tests/,
  * __graft_entry__.smoke() and bench.py's cpu_baseline leg may load
  * liboracle.so, and never as the thing measured or shipped.  The engine's
  * product path (libmlem_host.so, libmlem_hip.so) does not link or call it.

@@ -1,6 +1,6 @@
 #!/usr/bin/env python3
 """Writes the committed fixtures of tests/golden/ (run in a checkout that has
-the reference at /root/reference; the GPU box and the tests only read the
+the reference at /root/reference; GPU runs and the tests only read the
 outputs).
 
 * sptrans_golden.npz: small seeded CSR matrices (edge cases included) and
