@@ -172,7 +172,7 @@ struct mlem_matrix {
     // tiles of A and A^T (width tile_W, tile_R rows, f32) and the
     // sensitivity taken while they were built.
     bool tiles_only = false;
-    uint32_t tile_W = 0, tile_R = 0, tile_vt = 0;
+    uint32_t tile_W = 0, tile_R = 0, tile_vt = 0, tile_bc = 1;  // tile_bc: right-hand sides per pass
     DevBuf sens32, sens64;
     std::mutex mu;
     std::map<std::pair<int, uint32_t>, std::unique_ptr<Partition>> parts;  // (0: A / 1: A^T, secsize)
@@ -374,7 +374,7 @@ struct ChunkedTiles {
     // The chunk's own plan, its count into `scratch` (allocated here).
     int count_chunk(uint32_t r0, uint32_t rows_c, uint64_t nnz_c, const uint64_t *rp, const uint32_t *ci,
                     TilePlan *pc, DevBuf *scratch) {
-        *pc = tile_plan(rows_c, P.cols, nnz_c, P.W, P.R, P.vt, false);
+        *pc = tile_plan(rows_c, P.cols, nnz_c, P.W, P.R, P.vt, P.quad);
         pc->unit_scale = P.unit_scale;
         HIPCHK(scratch->alloc(tile_count_scratch_bytes(*pc)));
         uint32_t fl = 0;
@@ -997,14 +997,18 @@ int mlem_matrix_create(int device, uint32_t rows, uint32_t cols, uint64_t nnz, c
 
 int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
                              const uint32_t *col, const float *val, uint32_t tile_width, uint32_t tile_rows,
-                             uint32_t values, uint64_t chunk_bytes, mlem_matrix **out) {
+                             uint32_t values, uint32_t per_pass, uint64_t chunk_bytes, mlem_matrix **out) {
     ARGCHK(out, "mlem_matrix_create_tiled: null out");
     *out = nullptr;
     ARGCHK(rowptr && (nnz == 0 || (col && val)), "mlem_matrix_create_tiled: null matrix array");
     ARGCHK(rows > 0 && cols > 0, "mlem_matrix_create_tiled: rows and cols must be > 0");
     if (int st = check_csr_host("mlem_matrix_create_tiled", rows, cols, nnz, rowptr, col)) return st;
-    const uint32_t W = tile_width ? tile_width : 1024u, R = tile_rows ? tile_rows : 2048u;
-    ARGCHK(valid_tile_config(W, 1), "mlem_matrix_create_tiled: tile_width must be 0, 1024 or 2048");
+    const uint32_t BC = per_pass ? per_pass : 1u;
+    ARGCHK(BC == 1 || BC == 4 || BC == 8, "mlem_matrix_create_tiled: per_pass must be 0, 1, 4 or 8");
+    const bool quad = BC > 1;  // the batches' layout
+    const uint32_t W = tile_width ? tile_width : (quad ? 512u : 1024u);
+    const uint32_t R = tile_rows ? tile_rows : (BC == 8 ? 1024u : 2048u);
+    ARGCHK(valid_tile_config(W, BC), "mlem_matrix_create_tiled: no blocked kernel for width %u with %u per pass", W, BC);
     ARGCHK(valid_tile_rows(R), "mlem_matrix_create_tiled: tile_rows must be 512 ... 16384 (a power of two)");
     ARGCHK(values <= VAL_UF16, "mlem_matrix_create_tiled: values must be 0 (f32), 1 (bf16), 2 (fp16) or 3 (uf16)");
     HIPCHK(hipSetDevice(device));
@@ -1017,6 +1021,7 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
     m->tile_W = W;
     m->tile_R = R;
     m->tile_vt = values;
+    m->tile_bc = BC;
     const size_t margin = (size_t)1 << 30;
     auto avail = [&](size_t *b) -> int {
         size_t free_b = 0, total_b = 0;
@@ -1029,10 +1034,10 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
     HIPCHK(m->sens64.alloc(sizeof(double) * cols));
     auto tT = std::make_unique<Tiled>();
     auto tA = std::make_unique<Tiled>();
-    TilePlan pT = tile_plan(cols, rows, nnz, W, R, values, false);
+    TilePlan pT = tile_plan(cols, rows, nnz, W, R, values, quad);
     pT.unit_scale = values != VAL_F32;  // A^T's tiles hold A's codes (decoded below, in place)
     ChunkedTiles bT(pT, tT.get());
-    ChunkedTiles bA(tile_plan(rows, cols, nnz, W, R, values, false), tA.get());
+    ChunkedTiles bA(tile_plan(rows, cols, nnz, W, R, values, quad), tA.get());
     DevBuf scaleA;  // 16-bit values: A's row scales (for the sensitivity)
     {
         // A on the device; A^T's tiles from slabs of A^T's rows (column
@@ -1151,8 +1156,8 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
         HIPCHK(tT->sens.alloc(sizeof(float) * cols));
         HIPCHK(hipMemcpy(tT->sens.p, m->sens32.p, sizeof(float) * cols, hipMemcpyDeviceToDevice));
     }
-    m->tiles.emplace(std::make_tuple(0, W, R, values, false), std::move(tA));
-    m->tiles.emplace(std::make_tuple(1, W, R, values, false), std::move(tT));
+    m->tiles.emplace(std::make_tuple(0, W, R, values, quad), std::move(tA));
+    m->tiles.emplace(std::make_tuple(1, W, R, values, quad), std::move(tT));
     *out = m.release();
     return MLEM_OK;
 }
@@ -1388,6 +1393,7 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
     if (m->tiles_only) {
         ARGCHK(o.spmv == 0 && o.using_trans == 0 && o.bp_spmv <= 1,
                "mlem_recon_create: a tiles-only matrix takes merge-based FP and BP on A^T");
+        ARGCHK(m->tile_bc == 1, "mlem_recon_create: the matrix holds the tiles of batches of %u per pass", m->tile_bc);
         ARGCHK((o.tile_width == 0 || o.tile_width == m->tile_W) && (o.tile_rows == 0 || o.tile_rows == m->tile_R) &&
                    (o.values == 0 || o.values == m->tile_vt),
                "mlem_recon_create: the matrix holds tiles of width %u and %u rows with values %u", m->tile_W,
@@ -1740,7 +1746,16 @@ int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opt
     const uint32_t BC = o.per_pass ? o.per_pass : 4u;
     ARGCHK(BC == 1 || BC == 4 || BC == 8, "mlem_batch_create: per_pass must be 1, 4 or 8");
     ARGCHK(o.values <= VAL_UF16, "mlem_batch_create: values must be 0 (f32), 1 (bf16), 2 (fp16) or 3 (uf16)");
-    ARGCHK(m->has_T, "mlem_batch_create: the transpose must be resident (upload it or build it)");
+    ARGCHK(m->has_T || m->tiles_only, "mlem_batch_create: the transpose must be resident (upload it or build it)");
+    if (m->tiles_only) {
+        ARGCHK(BC == m->tile_bc && (o.tile_width == 0 || o.tile_width == m->tile_W) &&
+                   (o.tile_rows == 0 || o.tile_rows == m->tile_R) && (o.values == 0 || o.values == m->tile_vt),
+               "mlem_batch_create: the matrix holds tiles of width %u and %u rows with values %u for %u per pass",
+               m->tile_W, m->tile_R, m->tile_vt, m->tile_bc);
+        o.tile_width = m->tile_W;
+        o.tile_rows = m->tile_R;
+        o.values = m->tile_vt;
+    }
     const uint32_t W = o.tile_width ? o.tile_width : (BC == 1 ? 1024u : 512u);
     const uint32_t R = o.tile_rows ? o.tile_rows : (BC == 8 ? 1024u : 2048u);
     ARGCHK(valid_tile_config(W, BC), "mlem_batch_create: no blocked kernel for width %u with %u per pass", W, BC);

@@ -68,7 +68,7 @@ def hip_lib() -> C.CDLL:
             "mlem_matrix_create": (cint, [cint, u32, u32, u64, vp, vp, vp, vp, vp, vp, P(vp)]),
             "mlem_matrix_build_transpose": (cint, [vp]),
             "mlem_matrix_build_transpose_limited": (cint, [vp, u64, P(u32)]),
-            "mlem_matrix_create_tiled": (cint, [cint, u32, u32, u64, vp, vp, vp, u32, u32, u32, u64, P(vp)]),
+            "mlem_matrix_create_tiled": (cint, [cint, u32, u32, u64, vp, vp, vp, u32, u32, u32, u32, u64, P(vp)]),
             "mlem_autotune": (cint, [vp, u32, u32, vp]),
             "mlem_matrix_load_csr4": (cint, [cint, C.c_char_p, cint, P(vp), vp]),
             "mlem_batch_opts_default": (None, [vp]),
@@ -493,16 +493,17 @@ class ResidentMatrix:
             _ptr(AT.val) if AT is not None else None, C.byref(self.h)))
 
     @classmethod
-    def tiles_only(cls, A: Csr, tile_width: int = 0, tile_rows: int = 0, values: int = 0, chunk_bytes: int = 0,
-                   device: int = 0):
+    def tiles_only(cls, A: Csr, tile_width: int = 0, tile_rows: int = 0, values: int = 0, per_pass: int = 1,
+                   chunk_bytes: int = 0, device: int = 0):
         """A resident as the tiles of A and A^T only (mlem_matrix_create_tiled), for matrices whose CSR copies do
-        not fit beside their tiles; reconstructions take opts(spmv=0) (the matrix's values)."""
+        not fit beside their tiles: per_pass 1 for reconstructions (opts(spmv=0)), 4 or 8 for Batch with that
+        many per pass (the matrix's values)."""
         self = cls.__new__(cls)
         self.h = vp()
         self.rows, self.cols, self.nnz = A.rows, A.cols, A.nnz
         _check(hip_lib().mlem_matrix_create_tiled(device, A.rows, A.cols, A.nnz, _ptr(A.rowptr), _ptr(A.col),
-                                                  _ptr(A.val), tile_width, tile_rows, values, chunk_bytes,
-                                                  C.byref(self.h)))
+                                                  _ptr(A.val), tile_width, tile_rows, values, per_pass,
+                                                  chunk_bytes, C.byref(self.h)))
         return self
 
     @classmethod

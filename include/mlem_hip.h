@@ -238,9 +238,11 @@ int mlem_matrix_build_transpose(mlem_matrix *m);
  * be NULL) = column ranges used, 0 for the single sort.  MLEM_ERR_OUT_OF_MEMORY
  * when one column's nonzeros do not fit. */
 int mlem_matrix_build_transpose_limited(mlem_matrix *m, uint64_t scratch_bytes, uint32_t *ranges);
-/* A matrix resident as tiles only: the blocked layout of A and of A^T (tile
- * width 0 = 1024 or 2048, tile_rows 0 = 2048; `values` as in mlem_opts, one
- * right-hand side) and the FP64 sensitivity (of the quantized matrix for
+/* A matrix resident as tiles only: the blocked layout of A and of A^T
+ * (`values` as in mlem_opts; per_pass 0 or 1: the tiles of single
+ * reconstructions, tile width 0 = 1024 or 2048, tile_rows 0 = 2048;
+ * per_pass 4 or 8: those of batches with that many per pass, widths and
+ * rows as in mlem_batch_opts) and the FP64 sensitivity (of the quantized matrix for
  * 16-bit values), without the CSR copies, for matrices whose
  * two CSR copies leave no room for the tiles (the 100 GB matrix: 265 of
  * 288 GB).  Replaces the upload of main.cu:236-299 plus the transpose of
@@ -249,8 +251,9 @@ int mlem_matrix_build_transpose_limited(mlem_matrix *m, uint64_t scratch_bytes, 
  * row sums; A's copy is freed; A's tiles are built from row chunks of the
  * host CSR (read a second time).  The tiles are byte for byte those of the
  * resident path.  chunk_bytes bounds the scratch of each range / chunk
- * (0 = what is free).  A reconstruction on it takes merge-based FP and BP on
- * A^T (spmv 0, using_trans 0) with the matrix's tile shape and values;
+ * (0 = what is free).  A reconstruction on it (per_pass 1) takes merge-based
+ * FP and BP on A^T (spmv 0, using_trans 0) with the matrix's tile shape and
+ * values, a batch (per_pass 4 or 8) the matrix's per_pass, shape and values;
  * the plain kernels, batches, the autotuner and the transpose calls refuse
  * it.  MLEM_ERR_OUT_OF_MEMORY when the tiles or one row block's scratch do
  * not fit. */
@@ -258,7 +261,8 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
                              const uint64_t *rowptr, const uint32_t *col,
                              const float *val, uint32_t tile_width,
                              uint32_t tile_rows, uint32_t values,
-                             uint64_t chunk_bytes, mlem_matrix **out);
+                             uint32_t per_pass, uint64_t chunk_bytes,
+                             mlem_matrix **out);
 
 /* Load A straight from a csr4 file into a resident matrix (Csr4Matrix +
  * mapRows + csr_format_for_cuda + the H2D copies of main.cu:71-79,263-272,

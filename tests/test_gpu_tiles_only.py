@@ -66,6 +66,38 @@ def test_tiles_only_16bit_values_equal_resident_tiles(c1, values):
     assert np.array_equal(img2.view(np.uint32), ref_img.view(np.uint32))
 
 
+@pytest.mark.parametrize("per_pass,values,rows", [(4, 3, 2048), (4, 0, 512), (8, 2, 1024)])
+def test_tiles_only_batches_equal_resident_batches(c1, per_pass, values, rows):
+    """Tiles-only matrices in the batches' layout (quad groups, block
+    streams) built from column ranges and row chunks: every slot of a batch
+    bitwise the same as on the resident matrix; single reconstructions and
+    other batch shapes are refused."""
+    A, AT = c1["A"], c1["AT"]
+    rng = np.random.default_rng(3)
+    sinos = [rng.integers(0, 40, A.rows).astype(np.int32) for _ in range(per_pass + 2)]
+    f0 = np.ones(A.cols, np.float32)
+
+    def run(m, **kw):
+        with M.Batch(m, len(sinos), values=values, per_pass=per_pass, tile_rows=rows, **kw) as b:
+            for k, g in enumerate(sinos):
+                b.set(k, g, f0)
+            b.iterate(ITERS)
+            return [b.image(k) for k in range(len(sinos))]
+
+    with M.ResidentMatrix(A, AT) as m:
+        ref = run(m)
+    # scratch for a few column ranges (a range holds whole row blocks of A^T)
+    with M.ResidentMatrix.tiles_only(A, tile_rows=rows, values=values, per_pass=per_pass,
+                                     chunk_bytes=(48 << 20) * max(1, rows // 512)) as t:
+        got = run(t)
+        with pytest.raises(M.MlemError):
+            recon(t, sinos[0], f0, 1)
+        with pytest.raises(M.MlemError):
+            M.Batch(t, 4, values=values, per_pass=1 if per_pass > 1 else 4)
+    for k in range(len(sinos)):
+        assert np.array_equal(got[k].view(np.uint32), ref[k].view(np.uint32)), k
+
+
 def test_tiles_only_edge_matrices():
     """Empty rows and columns, one 60,000-entry row, heavy tails, duplicate
     columns, no entries, one row, row gaps: bitwise the resident tiled path,
