@@ -1026,7 +1026,7 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
     auto avail = [&](size_t *b) -> int {
         size_t free_b = 0, total_b = 0;
         HIPCHK(hipMemGetInfo(&free_b, &total_b));
-        *b = free_b > margin ? free_b - margin : 0;
+        *b = free_b > margin ? (free_b - margin) / 20 * 17 : 0;  // 85%: the free memory is not one piece
         if (chunk_bytes) *b = std::min<size_t>(*b, chunk_bytes);
         return MLEM_OK;
     };
@@ -1115,21 +1115,16 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
             if (int st = bA.allocate()) return st;
         size_t budget = 0;
         if (int st = avail(&budget)) return st;
-        uint32_t chunks = 0;
-        for (uint32_t r0 = 0; r0 < rows; ++chunks) {
-            uint32_t r1 = std::min<uint64_t>((uint64_t)r0 + R, rows);
-            // Per nonzero: the chunk's CSR (8 bytes), the fill's column and
-            // row scratch (4, with padding), 16-bit codes (2); per row: the
-            // per-segment counts and positions.
-            auto need = [&](uint32_t a, uint32_t b) {
-                const uint64_t n = rowptr[b] - rowptr[a];
-                return n * (values == VAL_F32 ? 16 : 18) + (uint64_t)(b - a) * (nseg * 8 + 8) + ((size_t)256 << 20);
-            };
-            while (r1 < rows) {
-                const uint32_t nx = (uint32_t)std::min<uint64_t>((uint64_t)r1 + R, rows);
-                if (need(r0, nx) > budget) break;
-                r1 = nx;
-            }
+        uint32_t chunks = 0, retries = 0;
+        // Per nonzero: the chunk's CSR (8 bytes), the fill's column and row
+        // scratch (4, with padding), 16-bit codes (2); per row: the
+        // per-segment counts and positions.
+        auto need = [&](uint32_t a, uint32_t b) {
+            const uint64_t n = rowptr[b] - rowptr[a];
+            return n * (values == VAL_F32 ? 16 : 18) + (uint64_t)(b - a) * (nseg * 8 + 8) + ((size_t)256 << 20);
+        };
+        // One chunk of whole row blocks [r0, r1): uploaded, counted or filled.
+        auto chunk = [&](uint32_t r0, uint32_t r1) -> int {
             const uint32_t rc = r1 - r0;
             const uint64_t n = rowptr[r1] - rowptr[r0];
             std::vector<uint64_t> hrp(rc + 1);
@@ -1139,16 +1134,33 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
             HIPCHK(ci.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, n)));
             HIPCHK(hipMemcpy(rp.p, hrp.data(), rp.bytes, hipMemcpyHostToDevice));
             if (n) HIPCHK(hipMemcpy(ci.p, col + rowptr[r0], sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-            if (pass == 0) {
-                if (int st = bA.count(r0, rc, n, rp.as<uint64_t>(), ci.as<uint32_t>())) return st;
-            } else {
-                HIPCHK(va.alloc(sizeof(float) * std::max<uint64_t>(1, n)));
-                if (n) HIPCHK(hipMemcpy(va.p, val + rowptr[r0], sizeof(float) * n, hipMemcpyHostToDevice));
-                if (int st = bA.fill(r0, rc, n, rp.as<uint64_t>(), ci.as<uint32_t>(), va.as<float>())) return st;
+            if (pass == 0) return bA.count(r0, rc, n, rp.as<uint64_t>(), ci.as<uint32_t>());
+            HIPCHK(va.alloc(sizeof(float) * std::max<uint64_t>(1, n)));
+            if (n) HIPCHK(hipMemcpy(va.p, val + rowptr[r0], sizeof(float) * n, hipMemcpyHostToDevice));
+            return bA.fill(r0, rc, n, rp.as<uint64_t>(), ci.as<uint32_t>(), va.as<float>());
+        };
+        for (uint32_t r0 = 0; r0 < rows; ++chunks) {
+            uint32_t r1 = std::min<uint64_t>((uint64_t)r0 + R, rows);
+            while (r1 < rows) {
+                const uint32_t nx = (uint32_t)std::min<uint64_t>((uint64_t)r1 + R, rows);
+                if (need(r0, nx) > budget) break;
+                r1 = nx;
+            }
+            // An allocation that fails although the estimate fit (the free
+            // memory is not one piece): the same start with half the blocks.
+            for (;;) {
+                const int st = chunk(r0, r1);
+                if (st == MLEM_OK) break;
+                const uint32_t blocks = (r1 - r0 + R - 1) / R;
+                if (st != MLEM_ERR_OUT_OF_MEMORY || blocks < 2) return st;
+                (void)hipGetLastError();
+                r1 = r0 + (blocks / 2) * R;
+                ++retries;
             }
             r0 = r1;
         }
-        progress("A tiles %s over %u row chunks", pass ? "filled" : "counted", chunks);
+        progress("A tiles %s over %u row chunks (%u smaller after a failed allocation; %.1f GB of chunk scratch)",
+                 pass ? "filled" : "counted", chunks, retries, budget / 1e9);
     }
     HIPCHK(hipDeviceSynchronize());
     progress("A tiles %.1f GB", tA->mem.bytes / 1e9);
