@@ -120,16 +120,24 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label, sinos=None, single=Tr
     if sinos is None:
         sinos = batch_sinograms(M, geo, A, B, threads)
     f0 = np.ones(A.cols, np.float32)
-    t_single = None
-    if single:  # one-at-a-time FP32 reference path (a captured graph too when --graph)
-        o = M.opts(spmv=0, using_trans=0, profile=0, use_graph=args.graph)
+    t_single = t_plain = None
+
+    def time_single(o):
         with M.Recon(m, g0, f0, sens=s32, options=o) as r:
             r.iterate(3)
             r.sync()
             t0 = time.perf_counter()
             r.iterate(20)
             r.sync()
-            t_single = (time.perf_counter() - t0) / 20
+            return (time.perf_counter() - t0) / 20
+
+    if single:
+        # one-at-a-time FP32: the fastest path (blocked kernels; a captured graph too when --graph), and the
+        # reference's kernels (merge-path FP, CSR-vector BP on A^T)
+        t_single = time_single(M.opts(spmv=0, using_trans=0, profile=0, use_graph=args.graph))
+        o_plain = M.opts(spmv=0, using_trans=0, profile=0, use_graph=args.graph, tile_width=1)
+        o_plain.bp_spmv = 2
+        t_plain = time_single(o_plain)
     t = time.time()
     with M.Batch(m, B, values=VALUES[args.values], per_pass=args.per_pass,
                  tile_width=args.tile if args.tile > 1 else 0, tile_rows=args.tile_rows,
@@ -173,7 +181,8 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label, sinos=None, single=Tr
     log(f"batched: {args.steps} iterations of {B} sinograms in {elapsed * 1e3:.1f} ms = {t_iter * 1e3:.2f} ms per "
         f"iteration ({t_iter / B * 1e3:.3f} ms per sinogram-iteration); {recon_s:.2f} recon/s of "
         f"{args.recon_iters} iterations"
-        + (f"; one-at-a-time FP32 {t_single * 1e3:.2f} ms, {recon_s / single_recon_s:.2f}x" if t_single else ""))
+        + (f"; one-at-a-time FP32 {t_single * 1e3:.2f} ms, {recon_s / single_recon_s:.2f}x; with the reference's "
+           f"kernels {t_plain * 1e3:.2f} ms, {t_plain / t_iter * B:.2f}x" if t_single else ""))
     out = {
         "metric": METRIC, "value": round(recon_s, 3), "unit": "recon/s", "n_gpus": 1, "steps": args.steps,
         "warmup": args.warmup, "ms_per_step": round(t_iter * 1e3, 4), "higher_is_better": True, "scaling": "weak",
@@ -185,6 +194,9 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label, sinos=None, single=Tr
         "one_at_a_time_f32": ({"ms_per_iteration": round(t_single * 1e3, 4), "recon_s": round(single_recon_s, 3)}
                               if t_single else None),
         "speedup_vs_one_at_a_time_f32": round(recon_s / single_recon_s, 3) if t_single else None,
+        "one_at_a_time_f32_reference_kernels": (
+            {"path": "merge-path FP + CSR-vector BP on A^T", "ms_per_iteration": round(t_plain * 1e3, 4),
+             "speedup": round(t_plain / t_iter * B, 3)} if t_plain else None),
     }
     if errs:
         out["rel_l2_vs_one_at_a_time_f32"] = {"iterations": args.recon_iters, "per_sinogram": [round(e, 8) for e in errs],
