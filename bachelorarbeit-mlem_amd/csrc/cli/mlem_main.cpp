@@ -5,7 +5,11 @@
 //        <trans: 0 = BP on A^T, 1 = scatter> <spmv: 0 = merge, 1 = CSR-vector>
 //        [--out=image.f32] [--sens=gpu64|host32] [--transpose=host|gpu]
 //        [--x0=ref|one] [--graph=0|1] [--threads=N] [--loglik] [--verbose]
-//        [--values=f32|bf16|fp16|uf16]
+//        [--values=f32|bf16|fp16|uf16] [--resident=csr|tiles]
+//
+// --resident=tiles keeps only the tiles of A and A^T on the GPU (no CSR
+// copies; for matrices whose two CSR copies leave no room for the tiles,
+// such as the 100 GB class); it needs trans = 0 and spmv = 0.
 //
 // Same checks as main.cu:43-62 (exactly 8 positional arguments, iters, nGPU
 // and both section sizes parsed with strtol and != 0, trans and spmv in
@@ -42,7 +46,8 @@ static void help() {
     printf("argv[8]: which SpMV algorithm to use?    0: merge-based SpMV   1: csr-vector SpMV\n");
     printf("options after the 8 arguments: --out=PATH --sens=gpu64|host32 --transpose=host|gpu\n");
     printf("                               --x0=ref|one --graph=0|1 --threads=N --loglik --verbose\n");
-    printf("                               --values=f32|bf16|fp16|uf16 (stored matrix values, merge-based only)\n\n");
+    printf("                               --values=f32|bf16|fp16|uf16 (stored matrix values, merge-based only)\n");
+    printf("                               --resident=csr|tiles (tiles: no CSR copies on the GPU; trans 0, spmv 0)\n\n");
 }
 
 struct Timer {
@@ -81,6 +86,7 @@ int main(int argc, char **argv) {
     if (matrix_vector_mul != 0 && matrix_vector_mul != 1) { help(); return EXIT_FAILURE; }
 
     std::string out_path, sens_mode = "gpu64", trans_mode = "host", x0_mode = "ref", values_mode = "f32";
+    std::string resident = "csr";
     int graph = 0, threads = 0;
     bool verbose = false, want_ll = false;
     for (int a = 9; a < argc; ++a) {
@@ -96,6 +102,7 @@ int main(int argc, char **argv) {
         else if (const char *v = val("--graph=")) graph = std::atoi(v);
         else if (const char *v = val("--threads=")) threads = std::atoi(v);
         else if (const char *v = val("--values=")) values_mode = v;
+        else if (const char *v = val("--resident=")) resident = v;
         else if (s == "--verbose") verbose = true;
         else if (s == "--loglik") want_ll = true;
         else { help(); return EXIT_FAILURE; }
@@ -105,7 +112,7 @@ int main(int argc, char **argv) {
     for (int k = 0; k < 4; ++k)
         if (values_mode == value_names[k]) values = k;
     if ((sens_mode != "gpu64" && sens_mode != "host32") || (trans_mode != "host" && trans_mode != "gpu") ||
-        (x0_mode != "ref" && x0_mode != "one") || values < 0) {
+        (x0_mode != "ref" && x0_mode != "one") || values < 0 || (resident != "csr" && resident != "tiles")) {
         help();
         return EXIT_FAILURE;
     }
@@ -121,7 +128,12 @@ int main(int argc, char **argv) {
     Timer T;
     // With the transpose and the sensitivity on the GPU no host CSR is needed:
     // the csr4 data section streams straight to the device.
-    const bool stream_load = trans_mode == "gpu" && sens_mode == "gpu64";
+    const bool tiles_only = resident == "tiles";
+    if (tiles_only && (using_trans != 0 || matrix_vector_mul != 0)) {
+        fprintf(stderr, "error: --resident=tiles needs the transposed, merge-based path (argv[7] = 0, argv[8] = 0)\n");
+        return EXIT_FAILURE;
+    }
+    const bool stream_load = trans_mode == "gpu" && sens_mode == "gpu64" && !tiles_only;
     uint32_t rows = 0, cols = 0;
     uint64_t nnz = 0;
     std::vector<uint64_t> rowptr;
@@ -183,7 +195,7 @@ int main(int argc, char **argv) {
         std::vector<uint64_t> rowptrT;
         std::vector<uint32_t> colT;
         std::vector<float> valT;
-        if (using_trans == 0 && trans_mode == "host") {
+        if (using_trans == 0 && trans_mode == "host" && !tiles_only) {
             rowptrT.resize((size_t)cols + 1);
             colT.resize(nnz);
             valT.resize(nnz);
@@ -194,16 +206,20 @@ int main(int argc, char **argv) {
             }
             t_trans = T.lap();
         }
-        st = mlem_matrix_create(0, rows, cols, nnz, rowptr.data(), col.data(), val.data(),
-                                rowptrT.empty() ? nullptr : rowptrT.data(), colT.empty() ? nullptr : colT.data(),
-                                valT.empty() ? nullptr : valT.data(), &M);
+        if (tiles_only)  // A^T's tiles from column ranges of A on the GPU, A's from row chunks of these arrays
+            st = mlem_matrix_create_tiled(0, rows, cols, nnz, rowptr.data(), col.data(), val.data(), 0, 0,
+                                          (uint32_t)values, 1, 0, &M);
+        else
+            st = mlem_matrix_create(0, rows, cols, nnz, rowptr.data(), col.data(), val.data(),
+                                    rowptrT.empty() ? nullptr : rowptrT.data(), colT.empty() ? nullptr : colT.data(),
+                                    valT.empty() ? nullptr : valT.data(), &M);
         if (st != MLEM_OK) {
             fprintf(stderr, "error: %s\n", mlem_last_error());
             return 4;
         }
         t_upload = T.lap();
     }
-    if (using_trans == 0 && trans_mode == "gpu") {
+    if (using_trans == 0 && trans_mode == "gpu" && !tiles_only) {
         if (mlem_matrix_build_transpose(M) != MLEM_OK) {
             fprintf(stderr, "error: %s\n", mlem_last_error());
             return 4;

@@ -68,6 +68,23 @@ def test_cli_16bit_values(c1, files):
     assert r.returncode == 1 and "--values" in r.stdout
 
 
+@pytest.mark.parametrize("values", ["f32", "uf16"])
+def test_cli_resident_tiles_equals_resident_csr(c1, files, values):
+    """--resident=tiles (no CSR copies on the GPU): the image is bitwise the one of the resident-CSR blocked path
+    (the same tiles; x0 = 1, since a tiles-only matrix with 16-bit values reports the quantized matrix's
+    sensitivity, from which the reference's initial image is scaled); argv[7] = 1 or argv[8] = 1 is refused."""
+    outs = []
+    for res in ("csr", "tiles"):
+        out = str(files["dir"] / f"img_res_{res}_{values}.f32")
+        r = run(files["matrix"], files["sino"], "15", "1", "8", "8", "0", "0", f"--out={out}", "--transpose=gpu",
+                f"--resident={res}", f"--values={values}", "--x0=one")
+        assert r.returncode == 0, r.stderr
+        outs.append(M.read_raw(out, np.float32))
+    assert np.array_equal(outs[0].view(np.uint32), outs[1].view(np.uint32))
+    r = run(files["matrix"], files["sino"], "2", "1", "2", "2", "0", "1", "--resident=tiles")
+    assert r.returncode == 1
+
+
 def test_cli_streamed_load_equals_host_path(c1, files):
     # --transpose=gpu with the GPU sensitivity streams the csr4 file to the
     # device (no host CSR); the image is bitwise the one of the host path.
