@@ -262,6 +262,8 @@ int get_partition(mlem_matrix *m, int which, uint32_t secsize, const Partition *
 // Tiles of A (which 0) or A^T (which 1) with segment width W and R rows per
 // block, built on first use.  MLEM_ERR_UNSUPPORTED (and *flags) when the
 // matrix does not allow them (a row not sorted by column).
+void progress(const char *fmt, ...);
+
 int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bool quad, const Tiled **out,
               uint32_t *flags) {
     std::lock_guard<std::mutex> lk(m->mu);
@@ -335,6 +337,10 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bo
     }
     if (fl & 8u) return set_err(MLEM_ERR_INVALID_ARGUMENT, "uf16 values need a matrix without negative values");
     if (fl) return set_err(MLEM_ERR_FORMAT, "tiling (width %u, rows %u): layout check failed (flags %u)", W, R, fl);
+    progress("tiles of %s: %u columns x %u rows, %s, values %u: %llu entries stored for %llu nonzeros (+%.1f%%)",
+             which ? "A^T" : "A", W, R, quad ? "quad groups" : "row steps", vt,
+             (unsigned long long)(t->plan.cap - 8), (unsigned long long)m->nnz,
+             m->nnz ? 100.0 * ((double)(t->plan.cap - 8) / (double)m->nnz - 1.0) : 0.0);
     *out = t.get();
     m->tiles.emplace(key, std::move(t));
     return MLEM_OK;
