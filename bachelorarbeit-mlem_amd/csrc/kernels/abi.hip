@@ -8,35 +8,13 @@
 // and so do the two memsets (main.cu:436-448) because the kernels overwrite.
 // With using_trans == 1 BP is the atomic scatter over A followed by the
 // update, which also re-zeroes bp for the next iteration.
-#include <hip/hip_runtime.h>
+#include "abi_internal.hpp"
 
-#include <algorithm>
-#include <cstdlib>
-#include <array>
-#include <chrono>
-#include <cstdarg>
-#include <cstdio>
-#include <cstring>
-#include <map>
-#include <set>
-#include <tuple>
-#include <memory>
-#include <mutex>
-#include <string>
-#include <thread>
-#include <vector>
-
-#include "launch.hpp"
-#include "mlem_hip.h"
-#include "mlem_host.h"
-
-using namespace mlem;
-
-namespace {
+namespace mlem {
+namespace abi {
 
 thread_local std::string g_err;
 
-int set_err(int st, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
 int set_err(int st, const char *fmt, ...) {
     char buf[1024];
     va_list ap;
@@ -47,58 +25,12 @@ int set_err(int st, const char *fmt, ...) {
     return st;
 }
 
-int hip_err(hipError_t e, const char *what, int line) {
+int hip_err(hipError_t e, const char *what, const char *file, int line) {
     int st = (e == hipErrorOutOfMemory || e == hipErrorMemoryAllocation) ? MLEM_ERR_OUT_OF_MEMORY
              : (e == hipErrorNoDevice || e == hipErrorInvalidDevice) ? MLEM_ERR_NO_DEVICE
                                                                        : MLEM_ERR_HIP;
-    return set_err(st, "%s failed: %s (abi.hip:%d)", what, hipGetErrorString(e), line);
+    return set_err(st, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
 }
-
-#define HIPCHK(expr)                                                  \
-    do {                                                              \
-        hipError_t e_ = (expr);                                       \
-        if (e_ != hipSuccess) return hip_err(e_, #expr, __LINE__);    \
-    } while (0)
-
-#define ARGCHK(cond, ...)                                                                \
-    do {                                                                                 \
-        if (!(cond)) return set_err(MLEM_ERR_INVALID_ARGUMENT, __VA_ARGS__);             \
-    } while (0)
-
-inline hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
-
-}  // namespace
-
-// The message mlem_last_error() returns, for the other translation units of
-// the library (server.hip).
-void mlem::set_last_error(const char *msg) { g_err = msg; }
-
-namespace {
-
-// Device memory owned by a handle.
-struct DevBuf {
-    void *p = nullptr;
-    size_t bytes = 0;
-    DevBuf() = default;
-    DevBuf(const DevBuf &) = delete;
-    DevBuf &operator=(const DevBuf &) = delete;
-    ~DevBuf() { reset(); }
-    void reset() {
-        if (p) (void)hipFree(p);
-        p = nullptr;
-        bytes = 0;
-    }
-    hipError_t alloc(size_t n) {
-        reset();
-        bytes = n;
-        if (n == 0) return hipSuccess;
-        return hipMalloc(&p, n);
-    }
-    template <typename T>
-    T *as() const {
-        return static_cast<T *>(p);
-    }
-};
 
 // Host-side checks of a CSR before it reaches the GPU: a bad row pointer or
 // column index would make the kernels read outside their arrays.
@@ -131,113 +63,15 @@ int check_csr_host(const char *what, uint32_t rows, uint32_t cols, uint64_t nnz,
     return MLEM_OK;
 }
 
-uint32_t clamp_secsize(uint32_t s) { return std::min(std::max(s, 1u), kMaxSecsize); }
+}  // namespace abi
+}  // namespace mlem
 
-// The SpMV kernels read col/val with 16-byte vector loads.
-bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+// The message mlem_last_error() returns, for the other translation units of
+// the library (server.hip).
+void mlem::set_last_error(const char *msg) { g_err = msg; }
 
-}  // namespace
-
-// ---------------------------------------------------------------------------
-// handles
-// ---------------------------------------------------------------------------
-struct Partition {
-    MergePlan plan;
-    DevBuf buf;
-};
-
-// A work list of a tiled matrix on the device (one per unit size and
-// target workgroup count).
-struct TileWorkDev {
-    TileWork view;
-    DevBuf mem;
-};
-
-struct Tiled {
-    TilePlan plan;
-    DevBuf mem;
-    TiledMatrix view;
-    DevBuf sens;  // A^T tiles of 16-bit values: the quantized matrix's column sums (f32)
-    std::map<uint32_t, std::unique_ptr<TileWorkDev>> works;  // by target workgroup count
-};
-
-struct mlem_matrix {
-    int device = 0;
-    uint32_t rows = 0, cols = 0;
-    uint64_t nnz = 0;
-    DevBuf rowptr, col, val;
-    DevBuf rowptrT, colT, valT;
-    bool has_T = false;
-    // Tiles only (mlem_matrix_create_tiled): no CSR copies on the device, the
-    // tiles of A and A^T (width tile_W, tile_R rows, f32) and the
-    // sensitivity taken while they were built.
-    bool tiles_only = false;
-    uint32_t tile_W = 0, tile_R = 0, tile_vt = 0, tile_bc = 1;  // tile_bc: right-hand sides per pass
-    DevBuf sens32, sens64;
-    std::mutex mu;
-    std::map<std::pair<int, uint32_t>, std::unique_ptr<Partition>> parts;  // (0: A / 1: A^T, secsize)
-    // Tiles keyed by (0: A / 1: A^T, width W, rows R, value type, quad groups).
-    using TileKey = std::tuple<int, uint32_t, uint32_t, uint32_t, bool>;
-    std::map<TileKey, std::unique_ptr<Tiled>> tiles;
-    std::map<TileKey, uint32_t> tile_refused;  // build flags of refused tilings
-
-    uint64_t device_bytes() const {
-        uint64_t b = rowptr.bytes + col.bytes + val.bytes + rowptrT.bytes + colT.bytes + valT.bytes + sens32.bytes +
-                     sens64.bytes;
-        for (const auto &t : tiles) b += t.second->mem.bytes;
-        return b;
-    }
-};
-
-// Per profiled iteration: FP start, FP SpMV kernel done, FP done, BP SpMV
-// kernel done, BP done.
-struct IterEvents {
-    hipEvent_t e[5];
-};
-
-struct mlem_batch {
-    mlem_matrix *m = nullptr;
-    uint32_t slots = 0, BC = 4, passes = 0;
-    uint32_t nt = 0;                // work-items per workgroup of the stream kernel (0: its default)
-    size_t rows_x = 0, cols_x = 0;  // padded vector lengths
-    hipStream_t stream = nullptr;
-    // [passes][rows_x][BC] counts and ratios, [passes][cols_x][BC] images
-    DevBuf g, ratio, f, sens, active, stage, part_fp, part_bp;
-    const Tiled *tfp = nullptr, *tbp = nullptr;
-    const TileWorkDev *wfp = nullptr, *wbp = nullptr;
-    std::vector<uint32_t> active_host;
-    // One iteration captured as a graph (use_graph), for the passes that
-    // held an active slot when it was captured.
-    bool use_graph = false;
-    hipGraphExec_t gexec = nullptr;
-    std::vector<uint8_t> graph_passes;
-    ~mlem_batch() {
-        if (stream) (void)hipStreamSynchronize(stream);  // no replay still running
-        if (gexec) (void)hipGraphExecDestroy(gexec);
-        if (stream) (void)hipStreamDestroy(stream);
-    }
-};
-
-struct mlem_recon {
-    const mlem_matrix *m = nullptr;
-    mlem_opts opts{};
-    hipStream_t stream = nullptr;
-    DevBuf g, sens, f, ratio, bp, scratch_fp, scratch_bp, llpart;
-    const Partition *pfp = nullptr, *pbp = nullptr;
-    const Tiled *tfp = nullptr, *tbp = nullptr;        // blocked LDS-tiled path, when used
-    const TileWorkDev *wfp = nullptr, *wbp = nullptr;  // and its work lists
-    DevBuf part_fp, part_bp;
-    uint32_t lanes_fp = 64, lanes_bp = 64;
-    bool bp_merge = true;  // BP on A^T: merge-based or CSR-vector
-    std::vector<IterEvents> events;  // pool, one set per profiled iteration
-    size_t events_used = 0;
-    double phase_ms[4] = {0, 0, 0, 0};  // FP SpMV, FP epilogue, BP SpMV, BP epilogue
-    uint32_t timed = 0;
-    hipGraph_t graph = nullptr;
-    hipGraphExec_t graph_exec = nullptr;
-};
-
-namespace {
+namespace mlem {
+namespace abi {
 
 int get_partition(mlem_matrix *m, int which, uint32_t secsize, const Partition **out) {
     std::lock_guard<std::mutex> lk(m->mu);
@@ -262,8 +96,6 @@ int get_partition(mlem_matrix *m, int which, uint32_t secsize, const Partition *
 // Tiles of A (which 0) or A^T (which 1) with segment width W and R rows per
 // block, built on first use.  MLEM_ERR_UNSUPPORTED (and *flags) when the
 // matrix does not allow them (a row not sorted by column).
-void progress(const char *fmt, ...);
-
 int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bool quad, const Tiled **out,
               uint32_t *flags) {
     std::lock_guard<std::mutex> lk(m->mu);
@@ -361,82 +193,6 @@ void progress(const char *fmt, ...) {
     std::fflush(stderr);
     va_end(ap);
 }
-
-// Tiles (one right-hand side; f32 or 16-bit values) of a matrix that is not
-// resident as one CSR: given as chunks of whole row blocks, each a CSR of its own
-// (rows r0 .. r0+rows_c, row pointers from 0) on the device.  Two passes
-// over the same row blocks (the chunks may differ between them): the counts
-// give every tile's size and so the offsets of the whole layout; then, the
-// tiles allocated, each chunk fills its part.  The bytes are those that one
-// build over the whole CSR writes (every tile is laid out on its own).
-struct ChunkedTiles {
-    TilePlan P;
-    std::vector<uint64_t> size;  // padded entries per tile, then (allocate) offsets
-    std::vector<uint32_t> nz;    // entries per tile
-    Tiled *t = nullptr;
-
-    ChunkedTiles(const TilePlan &p, Tiled *tiled) : P(p), size(p.ntiles + 1, 0), nz(p.ntiles, 0), t(tiled) {}
-
-    // The chunk's own plan, its count into `scratch` (allocated here).
-    int count_chunk(uint32_t r0, uint32_t rows_c, uint64_t nnz_c, const uint64_t *rp, const uint32_t *ci,
-                    TilePlan *pc, DevBuf *scratch) {
-        *pc = tile_plan(rows_c, P.cols, nnz_c, P.W, P.R, P.vt, P.quad);
-        pc->unit_scale = P.unit_scale;
-        HIPCHK(scratch->alloc(tile_count_scratch_bytes(*pc)));
-        uint32_t fl = 0;
-        HIPCHK(launch_tile_count(*pc, rp, ci, scratch->p, &fl, nullptr));
-        if (fl) return set_err(MLEM_ERR_UNSUPPORTED, "tiling refused: a row is not sorted by column (rows from %u)", r0);
-        return MLEM_OK;
-    }
-
-    int count(uint32_t r0, uint32_t rows_c, uint64_t nnz_c, const uint64_t *rp, const uint32_t *ci) {
-        TilePlan pc;
-        DevBuf scratch;
-        if (int st = count_chunk(r0, rows_c, nnz_c, rp, ci, &pc, &scratch)) return st;
-        const uint64_t t0 = (uint64_t)(r0 / P.R) * P.nseg;
-        std::vector<uint64_t> off(pc.ntiles + 1);
-        HIPCHK(hipMemcpy(off.data(), tile_count_offsets(pc, scratch.p), 8 * (pc.ntiles + 1), hipMemcpyDeviceToHost));
-        if (pc.ntiles)
-            HIPCHK(hipMemcpy(nz.data() + t0, tile_count_sizes(pc, scratch.p), 4 * pc.ntiles, hipMemcpyDeviceToHost));
-        for (uint64_t i = 0; i < pc.ntiles; ++i) size[t0 + i] = off[i + 1] - off[i];
-        return MLEM_OK;
-    }
-
-    int allocate() {
-        uint64_t run = 0;
-        for (uint64_t i = 0; i <= P.ntiles; ++i) {
-            const uint64_t n = i < P.ntiles ? size[i] : 0;
-            size[i] = run;
-            run += n;
-        }
-        P.cap = run + 8;  // as one build: a multiple of 8, at least one group
-        t->plan = P;
-        HIPCHK(t->mem.alloc(tiled_matrix_bytes(P)));
-        t->view = tiled_matrix_view(P, t->mem.p);
-        HIPCHK(hipMemcpy(t->view.tile_off, size.data(), 8 * (P.ntiles + 1), hipMemcpyHostToDevice));
-        if (P.ntiles) HIPCHK(hipMemcpy(t->view.tile_nz, nz.data(), 4 * P.ntiles, hipMemcpyHostToDevice));
-        return MLEM_OK;
-    }
-
-    int fill(uint32_t r0, uint32_t rows_c, uint64_t nnz_c, const uint64_t *rp, const uint32_t *ci, const float *va) {
-        TilePlan pc;
-        DevBuf scratch, local_off, fill_scratch;
-        if (int st = count_chunk(r0, rows_c, nnz_c, rp, ci, &pc, &scratch)) return st;
-        const uint64_t t0 = (uint64_t)(r0 / P.R) * P.nseg;
-        HIPCHK(local_off.alloc(8 * (pc.ntiles + 1)));
-        HIPCHK(fill_scratch.alloc(tile_fill_scratch_bytes(pc)));
-        TiledMatrix v;
-        v.tile_off = local_off.as<uint64_t>();
-        v.tile_nz = t->view.tile_nz + t0;
-        v.tdata = t->view.tdata + (uint64_t)(P.vb() + 2) * size[t0];
-        v.scale = t->view.scale ? t->view.scale + r0 : nullptr;
-        uint32_t fl = 0;
-        HIPCHK(launch_tile_fill_into(pc, rp, ci, va, v, scratch.p, fill_scratch.p, &fl, nullptr));
-        if (fl & 8u) return set_err(MLEM_ERR_INVALID_ARGUMENT, "uf16 values need a matrix without negative values");
-        if (fl) return set_err(MLEM_ERR_FORMAT, "tiling: layout check failed (flags %u, rows from %u)", fl, r0);
-        return MLEM_OK;
-    }
-};
 
 // Work list of tiled matrix t for about `target` workgroups, built on first
 // use from the tile sizes.
@@ -638,7 +394,8 @@ int sens_on_device(const mlem_matrix *m, float *s32, double *s64_or_null, hipStr
     return MLEM_OK;
 }
 
-}  // namespace
+}  // namespace abi
+}  // namespace mlem
 
 extern "C" {
 
@@ -1001,185 +758,6 @@ int mlem_matrix_create(int device, uint32_t rows, uint32_t cols, uint64_t nnz, c
     return MLEM_OK;
 }
 
-int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
-                             const uint32_t *col, const float *val, uint32_t tile_width, uint32_t tile_rows,
-                             uint32_t values, uint32_t per_pass, uint64_t chunk_bytes, mlem_matrix **out) {
-    ARGCHK(out, "mlem_matrix_create_tiled: null out");
-    *out = nullptr;
-    ARGCHK(rowptr && (nnz == 0 || (col && val)), "mlem_matrix_create_tiled: null matrix array");
-    ARGCHK(rows > 0 && cols > 0, "mlem_matrix_create_tiled: rows and cols must be > 0");
-    if (int st = check_csr_host("mlem_matrix_create_tiled", rows, cols, nnz, rowptr, col)) return st;
-    const uint32_t BC = per_pass ? per_pass : 1u;
-    ARGCHK(BC == 1 || BC == 4 || BC == 8, "mlem_matrix_create_tiled: per_pass must be 0, 1, 4 or 8");
-    const bool quad = BC > 1;  // the batches' layout
-    const uint32_t W = tile_width ? tile_width : (BC == 8 ? 512u : 1024u);  // as mlem_batch_create
-    const uint32_t R = tile_rows ? tile_rows : (BC == 1 ? 2048u : 1024u);
-    ARGCHK(valid_tile_config(W, BC), "mlem_matrix_create_tiled: no blocked kernel for width %u with %u per pass", W, BC);
-    ARGCHK(valid_tile_rows(R), "mlem_matrix_create_tiled: tile_rows must be 512 ... 16384 (a power of two)");
-    ARGCHK(values <= VAL_UF16, "mlem_matrix_create_tiled: values must be 0 (f32), 1 (bf16), 2 (fp16) or 3 (uf16)");
-    HIPCHK(hipSetDevice(device));
-    auto m = std::make_unique<mlem_matrix>();
-    m->device = device;
-    m->rows = rows;
-    m->cols = cols;
-    m->nnz = nnz;
-    m->tiles_only = true;
-    m->tile_W = W;
-    m->tile_R = R;
-    m->tile_vt = values;
-    m->tile_bc = BC;
-    const size_t margin = (size_t)1 << 30;
-    auto avail = [&](size_t *b) -> int {
-        size_t free_b = 0, total_b = 0;
-        HIPCHK(hipMemGetInfo(&free_b, &total_b));
-        *b = free_b > margin ? (free_b - margin) / 20 * 17 : 0;  // 85%: the free memory is not one piece
-        if (chunk_bytes) *b = std::min<size_t>(*b, chunk_bytes);
-        return MLEM_OK;
-    };
-    HIPCHK(m->sens32.alloc(sizeof(float) * cols));
-    HIPCHK(m->sens64.alloc(sizeof(double) * cols));
-    auto tT = std::make_unique<Tiled>();
-    auto tA = std::make_unique<Tiled>();
-    TilePlan pT = tile_plan(cols, rows, nnz, W, R, values, quad);
-    pT.unit_scale = values != VAL_F32;  // A^T's tiles hold A's codes (decoded below, in place)
-    ChunkedTiles bT(pT, tT.get());
-    ChunkedTiles bA(tile_plan(rows, cols, nnz, W, R, values, quad), tA.get());
-    DevBuf scaleA;  // 16-bit values: A's row scales (for the sensitivity)
-    {
-        // A on the device; A^T's tiles from slabs of A^T's rows (column
-        // ranges of A, whole row blocks of A^T), the sensitivity from the
-        // slabs' row sums (FP64, as for a resident transpose).
-        DevBuf rp, ci, va, colptrT, srp;
-        HIPCHK(rp.alloc(sizeof(uint64_t) * ((size_t)rows + 1)));
-        HIPCHK(ci.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, nnz)));
-        HIPCHK(va.alloc(sizeof(float) * std::max<uint64_t>(1, nnz)));
-        HIPCHK(hipMemcpy(rp.p, rowptr, rp.bytes, hipMemcpyHostToDevice));
-        if (nnz) {
-            HIPCHK(hipMemcpy(ci.p, col, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(va.p, val, sizeof(float) * nnz, hipMemcpyHostToDevice));
-        }
-        HIPCHK(colptrT.alloc(sizeof(uint64_t) * ((size_t)cols + 1)));
-        HIPCHK(srp.alloc(sizeof(uint64_t) * ((size_t)cols + 1)));
-        if (values != VAL_F32) {  // A's values become their decoded codes: what A^T's tiles store
-            DevBuf bad;
-            HIPCHK(scaleA.alloc(sizeof(float) * rows));
-            HIPCHK(bad.alloc(sizeof(uint32_t)));
-            HIPCHK(launch_fill(bad.as<float>(), 0.0f, 1, nullptr));
-            HIPCHK(launch_quantize_in_place(rp.as<uint64_t>(), va.as<float>(), rows, values, scaleA.as<float>(),
-                                            bad.as<uint32_t>(), nullptr));
-            uint32_t fl = 0;
-            HIPCHK(hipMemcpy(&fl, bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost));
-            if (fl & 8u)
-                return set_err(MLEM_ERR_INVALID_ARGUMENT, "uf16 values need a matrix without negative values");
-        }
-        for (int pass = 0; pass < 2; ++pass) {
-            if (pass == 1)
-                if (int st = bT.allocate()) return st;
-            int st_sink = MLEM_OK;
-            RangeSink sink = [&](uint32_t c0, uint32_t c1, uint64_t off0, uint64_t n, const uint32_t *rowT,
-                                 const float *valT) -> hipError_t {
-                hipError_t e = launch_rebase(colptrT.as<uint64_t>() + c0, off0, c1 - c0, srp.as<uint64_t>(), nullptr);
-                if (e != hipSuccess) return e;
-                if (pass == 0) {
-                    // FP64 column sums of A (of the quantized matrix: codes times row scales)
-                    e = values == VAL_F32
-                            ? launch_colsum_rows_f64(srp.as<uint64_t>(), valT, c1 - c0, m->sens64.as<double>() + c0,
-                                                     m->sens32.as<float>() + c0, nullptr)
-                            : launch_colsum_weighted_f64(srp.as<uint64_t>(), rowT, valT, scaleA.as<float>(), c1 - c0,
-                                                         m->sens64.as<double>() + c0, m->sens32.as<float>() + c0,
-                                                         nullptr);
-                    if (e != hipSuccess) return e;
-                    st_sink = bT.count(c0, c1 - c0, n, srp.as<uint64_t>(), rowT);
-                } else {
-                    st_sink = bT.fill(c0, c1 - c0, n, srp.as<uint64_t>(), rowT, valT);
-                }
-                return st_sink == MLEM_OK ? hipSuccess : hipErrorUnknown;
-            };
-            // Half of what is free for the ranges' sorts, the rest for the
-            // slab's tiling scratch.
-            size_t budget = 0;
-            if (int st = avail(&budget)) return st;
-            uint32_t nr = 0;
-            const hipError_t e = transpose_ranges_to(rows, cols, nnz, rp.as<uint64_t>(), ci.as<uint32_t>(),
-                                                     va.as<float>(), colptrT.as<uint64_t>(), nullptr, nullptr,
-                                                     budget / 2, R, sink, &nr, nullptr);
-            if (st_sink != MLEM_OK) return st_sink;
-            if (e == hipErrorOutOfMemory)
-                return set_err(MLEM_ERR_OUT_OF_MEMORY, "mlem_matrix_create_tiled: %u rows of A^T do not fit", R);
-            HIPCHK(e);
-            progress("A^T tiles %s over %u column ranges (%.1f GB of range scratch)", pass ? "filled" : "counted", nr,
-                     budget / 2 / 1e9);
-        }
-        HIPCHK(hipDeviceSynchronize());
-    }
-    // A's tiles from row chunks of the host CSR (A's device copy is gone),
-    // whole row blocks whose CSR and tiling scratch fit.
-    const uint64_t nseg = bA.P.nseg;
-    progress("A^T tiles %.1f GB; A's device copy freed", tT->mem.bytes / 1e9);
-    for (int pass = 0; pass < 2; ++pass) {
-        if (pass == 1)
-            if (int st = bA.allocate()) return st;
-        size_t budget = 0;
-        if (int st = avail(&budget)) return st;
-        uint32_t chunks = 0, retries = 0;
-        // Per nonzero: the chunk's CSR (8 bytes), the fill's column and row
-        // scratch (4, with padding), 16-bit codes (2); per row: the
-        // per-segment counts and positions.
-        auto need = [&](uint32_t a, uint32_t b) {
-            const uint64_t n = rowptr[b] - rowptr[a];
-            return n * (values == VAL_F32 ? 16 : 18) + (uint64_t)(b - a) * (nseg * 8 + 8) + ((size_t)256 << 20);
-        };
-        // One chunk of whole row blocks [r0, r1): uploaded, counted or filled.
-        auto chunk = [&](uint32_t r0, uint32_t r1) -> int {
-            const uint32_t rc = r1 - r0;
-            const uint64_t n = rowptr[r1] - rowptr[r0];
-            std::vector<uint64_t> hrp(rc + 1);
-            for (uint32_t i = 0; i <= rc; ++i) hrp[i] = rowptr[r0 + i] - rowptr[r0];
-            DevBuf rp, ci, va;
-            HIPCHK(rp.alloc(sizeof(uint64_t) * (rc + 1)));
-            HIPCHK(ci.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, n)));
-            HIPCHK(hipMemcpy(rp.p, hrp.data(), rp.bytes, hipMemcpyHostToDevice));
-            if (n) HIPCHK(hipMemcpy(ci.p, col + rowptr[r0], sizeof(uint32_t) * n, hipMemcpyHostToDevice));
-            if (pass == 0) return bA.count(r0, rc, n, rp.as<uint64_t>(), ci.as<uint32_t>());
-            HIPCHK(va.alloc(sizeof(float) * std::max<uint64_t>(1, n)));
-            if (n) HIPCHK(hipMemcpy(va.p, val + rowptr[r0], sizeof(float) * n, hipMemcpyHostToDevice));
-            return bA.fill(r0, rc, n, rp.as<uint64_t>(), ci.as<uint32_t>(), va.as<float>());
-        };
-        for (uint32_t r0 = 0; r0 < rows; ++chunks) {
-            uint32_t r1 = std::min<uint64_t>((uint64_t)r0 + R, rows);
-            while (r1 < rows) {
-                const uint32_t nx = (uint32_t)std::min<uint64_t>((uint64_t)r1 + R, rows);
-                if (need(r0, nx) > budget) break;
-                r1 = nx;
-            }
-            // An allocation that fails although the estimate fit (the free
-            // memory is not one piece): the same start with half the blocks.
-            for (;;) {
-                const int st = chunk(r0, r1);
-                if (st == MLEM_OK) break;
-                const uint32_t blocks = (r1 - r0 + R - 1) / R;
-                if (st != MLEM_ERR_OUT_OF_MEMORY || blocks < 2) return st;
-                (void)hipGetLastError();
-                r1 = r0 + (blocks / 2) * R;
-                ++retries;
-            }
-            r0 = r1;
-        }
-        progress("A tiles %s over %u row chunks (%u smaller after a failed allocation; %.1f GB of chunk scratch)",
-                 pass ? "filled" : "counted", chunks, retries, budget / 1e9);
-    }
-    HIPCHK(hipDeviceSynchronize());
-    progress("A tiles %.1f GB", tA->mem.bytes / 1e9);
-    if (values != VAL_F32) {  // the quantized matrix's own sensitivity, as the resident path keeps it
-        HIPCHK(tT->sens.alloc(sizeof(float) * cols));
-        HIPCHK(hipMemcpy(tT->sens.p, m->sens32.p, sizeof(float) * cols, hipMemcpyDeviceToDevice));
-    }
-    m->tiles.emplace(std::make_tuple(0, W, R, values, quad), std::move(tA));
-    m->tiles.emplace(std::make_tuple(1, W, R, values, quad), std::move(tT));
-    *out = m.release();
-    return MLEM_OK;
-}
-
 int mlem_matrix_load_csr4(int device, const char *path, int threads, mlem_matrix **out, mlem_load_stats *stats) {
     ARGCHK(path && out, "mlem_matrix_load_csr4: null argument");
     *out = nullptr;
@@ -1493,7 +1071,7 @@ int mlem_recon_iterate(mlem_recon *r, uint32_t iters) {
             hipError_t e2 = hipStreamEndCapture(r->stream, &gr);
             if (e != hipSuccess) {
                 if (gr) (void)hipGraphDestroy(gr);
-                return hip_err(e, "enqueue_iteration (capture)", __LINE__);
+                return hip_err(e, "enqueue_iteration (capture)", __FILE_NAME__, __LINE__);
             }
             HIPCHK(e2);
             r->graph = gr;
@@ -1513,7 +1091,7 @@ int mlem_recon_iterate(mlem_recon *r, uint32_t iters) {
             ev = &r->events[r->events_used++];
         }
         hipError_t e = enqueue_iteration(r, ev);
-        if (e != hipSuccess) return hip_err(e, "enqueue_iteration", __LINE__);
+        if (e != hipSuccess) return hip_err(e, "enqueue_iteration", __FILE_NAME__, __LINE__);
     }
     return MLEM_OK;
 }
@@ -1606,357 +1184,6 @@ int mlem_recon_destroy(mlem_recon *r) {
         for (auto &e : x.e) (void)hipEventDestroy(e);
     if (r->stream) (void)hipStreamDestroy(r->stream);
     delete r;
-    return MLEM_OK;
-}
-
-// -------------------------------------------------------------- autotuner
-// Times the blocked kernels of both sides over a few tile shapes and
-// workgroup counts and reports the fastest.  Candidate tiles that did not
-// exist before and were not chosen are freed again.
-int mlem_autotune(mlem_matrix *m, uint32_t per_pass, uint32_t values, mlem_tune_result *out) {
-    ARGCHK(m && out, "mlem_autotune: null argument");
-    std::memset(out, 0, sizeof(*out));
-    const uint32_t BC = per_pass ? per_pass : 1u;
-    ARGCHK(BC == 1 || BC == 4 || BC == 8, "mlem_autotune: per_pass must be 1, 4 or 8");
-    ARGCHK(values <= VAL_UF16, "mlem_autotune: values must be 0 (f32), 1 (bf16), 2 (fp16) or 3 (uf16)");
-    ARGCHK(m->has_T, "mlem_autotune: the transpose must be resident");
-    HIPCHK(hipSetDevice(m->device));
-    int dev = 0, cus = 256;
-    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    std::vector<std::pair<uint32_t, uint32_t>> shapes;  // (W, R)
-    std::vector<uint32_t> groups;                       // per CU
-    std::vector<uint32_t> threads = {0};                // work-items per workgroup (several right-hand sides)
-    if (BC > 1) threads = {256, 512};  // 128 was slower at every shape timed on the 3D scanner matrix
-    if (BC == 1) {
-        shapes = {{1024, 1024}, {1024, 2048}, {1024, 4096}, {2048, 1024}, {2048, 2048}};
-        groups = {16, 24, 32};
-    } else if (BC == 4) {
-        shapes = {{512, 1024}, {512, 2048}, {1024, 1024}};
-        groups = {8, 16};
-    } else {
-        shapes = {{512, 512}, {512, 1024}};
-        groups = {8, 16};
-    }
-    std::set<mlem_matrix::TileKey> before;
-    {
-        std::lock_guard<std::mutex> lk(m->mu);
-        for (const auto &t : m->tiles) before.insert(t.first);
-    }
-    const size_t rows_x = ((size_t)m->rows + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
-    const size_t cols_x = ((size_t)m->cols + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
-    DevBuf xf, xb, yf, yb;
-    HIPCHK(xf.alloc(sizeof(float) * cols_x * BC));
-    HIPCHK(xb.alloc(sizeof(float) * rows_x * BC));
-    HIPCHK(yf.alloc(sizeof(float) * rows_x * BC));
-    HIPCHK(yb.alloc(sizeof(float) * cols_x * BC));
-    hipStream_t s = nullptr;
-    HIPCHK(hipStreamCreateWithFlags(&s, hipStreamNonBlocking));
-    hipEvent_t e0 = nullptr, e1 = nullptr;
-    auto cleanup = [&]() {
-        if (e0) (void)hipEventDestroy(e0);
-        if (e1) (void)hipEventDestroy(e1);
-        (void)hipStreamSynchronize(s);
-        (void)hipStreamDestroy(s);
-    };
-    if (hipEventCreate(&e0) != hipSuccess || hipEventCreate(&e1) != hipSuccess) {
-        cleanup();
-        return set_err(MLEM_ERR_HIP, "mlem_autotune: event creation failed");
-    }
-    if (launch_fill(xf.as<float>(), 1.0f, cols_x * BC, s) != hipSuccess ||
-        launch_fill(xb.as<float>(), 1.0f, rows_x * BC, s) != hipSuccess) {
-        cleanup();
-        return set_err(MLEM_ERR_HIP, "mlem_autotune: fill failed");
-    }
-    double best = 1e300;
-    uint32_t bestW = 0, bestR = 0, bestG = 0, bestT = 0, tried = 0;
-    double best_fp = 0, best_bp = 0;
-    // One side's time per launch (partials + reduce), mean of three after a warm-up.
-    auto time_side = [&](const Tiled *t, const TileWorkDev *w, DevBuf &part, const float *x, float *y, uint32_t nt,
-                         double *ms) {
-        EpiArgs ea{};
-        ea.out = y;
-        for (int rep = 0; rep < 4; ++rep) {
-            if (rep == 1 && hipEventRecord(e0, s) != hipSuccess) return hipErrorUnknown;
-            hipError_t e = launch_tiled_partials(t->plan, t->view, w->view, x, part.as<float>(), BC, nt, s);
-            if (e == hipSuccess) e = launch_tiled_reduce(t->plan, t->view, w->view, part.as<float>(), BC, EPI_PLAIN, ea, s);
-            if (e != hipSuccess) return e;
-        }
-        hipError_t e = hipEventRecord(e1, s);
-        if (e == hipSuccess) e = hipEventSynchronize(e1);
-        float f = 0.0f;
-        if (e == hipSuccess) e = hipEventElapsedTime(&f, e0, e1);
-        *ms = f / 3.0;
-        return e;
-    };
-    for (const auto &sh : shapes) {
-        for (uint32_t g : groups) {
-            const Tiled *tf = nullptr, *tb = nullptr;
-            const TileWorkDev *wf = nullptr, *wb = nullptr;
-            DevBuf pf, pb;
-            int st = setup_tiled(m, 0, sh.first, sh.second, values, BC, g * (uint32_t)cus, &tf, &wf, &pf);
-            if (st == MLEM_OK) st = setup_tiled(m, 1, sh.first, sh.second, values, BC, g * (uint32_t)cus, &tb, &wb, &pb);
-            if (st == MLEM_ERR_UNSUPPORTED || st == MLEM_ERR_OUT_OF_MEMORY) {
-                (void)hipGetLastError();
-                continue;  // this shape does not suit or fit the matrix
-            }
-            if (st != MLEM_OK) {
-                cleanup();
-                return st;
-            }
-            for (uint32_t nt : threads) {
-                double fp = 0, bp = 0;
-                hipError_t e = time_side(tf, wf, pf, xf.as<float>(), yf.as<float>(), nt, &fp);
-                if (e == hipSuccess) e = time_side(tb, wb, pb, xb.as<float>(), yb.as<float>(), nt, &bp);
-                if (e != hipSuccess) {
-                    cleanup();
-                    return hip_err(e, "mlem_autotune: timing", __LINE__);
-                }
-                ++tried;
-                if (fp + bp < best) {
-                    best = fp + bp;
-                    bestW = sh.first;
-                    bestR = sh.second;
-                    bestG = g;
-                    bestT = nt;
-                    best_fp = fp;
-                    best_bp = bp;
-                }
-            }
-        }
-    }
-    cleanup();
-    {  // free the candidate tiles that were built here and not chosen
-        std::lock_guard<std::mutex> lk(m->mu);
-        for (auto it = m->tiles.begin(); it != m->tiles.end();) {
-            const auto &k = it->first;
-            const bool chosen = std::get<1>(k) == bestW && std::get<2>(k) == bestR && std::get<3>(k) == values &&
-                                std::get<4>(k) == (BC > 1);
-            it = (!before.count(k) && !chosen) ? m->tiles.erase(it) : std::next(it);
-        }
-    }
-    if (!tried) return set_err(MLEM_ERR_UNSUPPORTED, "mlem_autotune: no tile shape suits or fits this matrix");
-    out->tile_width = bestW;
-    out->tile_rows = bestR;
-    out->tile_groups = bestG * (uint32_t)cus;
-    out->tile_threads = bestT;
-    out->candidates = tried;
-    out->fp_ms = best_fp;
-    out->bp_ms = best_bp;
-    return MLEM_OK;
-}
-
-// ---------------------------------------------------------------- batches
-void mlem_batch_opts_default(mlem_batch_opts *o) {
-    if (!o) return;
-    std::memset(o, 0, sizeof(*o));
-    o->per_pass = 4;
-}
-
-int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opts, mlem_batch **out) {
-    ARGCHK(out, "mlem_batch_create: null out");
-    *out = nullptr;
-    ARGCHK(m && slots > 0, "mlem_batch_create: null matrix or no slots");
-    mlem_batch_opts o;
-    if (opts)
-        o = *opts;
-    else
-        mlem_batch_opts_default(&o);
-    const uint32_t BC = o.per_pass ? o.per_pass : 4u;
-    ARGCHK(BC == 1 || BC == 4 || BC == 8, "mlem_batch_create: per_pass must be 1, 4 or 8");
-    ARGCHK(o.values <= VAL_UF16, "mlem_batch_create: values must be 0 (f32), 1 (bf16), 2 (fp16) or 3 (uf16)");
-    ARGCHK(m->has_T || m->tiles_only, "mlem_batch_create: the transpose must be resident (upload it or build it)");
-    if (m->tiles_only) {
-        ARGCHK(BC == m->tile_bc && (o.tile_width == 0 || o.tile_width == m->tile_W) &&
-                   (o.tile_rows == 0 || o.tile_rows == m->tile_R) && (o.values == 0 || o.values == m->tile_vt),
-               "mlem_batch_create: the matrix holds tiles of width %u and %u rows with values %u for %u per pass",
-               m->tile_W, m->tile_R, m->tile_vt, m->tile_bc);
-        o.tile_width = m->tile_W;
-        o.tile_rows = m->tile_R;
-        o.values = m->tile_vt;
-    }
-    // Defaults: 1024 x 2048 for one per pass, 1024 x 1024 for 4 (3.28x the
-    // single FP32 path at C2 against 3.17x for 512 x 2048), 512 x 1024 for 8.
-    const uint32_t W = o.tile_width ? o.tile_width : (BC == 8 ? 512u : 1024u);
-    const uint32_t R = o.tile_rows ? o.tile_rows : (BC == 1 ? 2048u : 1024u);
-    ARGCHK(valid_tile_config(W, BC), "mlem_batch_create: no blocked kernel for width %u with %u per pass", W, BC);
-    ARGCHK(valid_tile_rows(R), "mlem_batch_create: tile_rows must be 512 ... 16384 (a power of two)");
-    ARGCHK(valid_stream_threads(o.tile_threads), "mlem_batch_create: tile_threads must be 0, 128, 256 or 512");
-    HIPCHK(hipSetDevice(m->device));
-    auto b = std::make_unique<mlem_batch>();
-    b->m = m;
-    b->slots = slots;
-    b->BC = BC;
-    b->nt = o.tile_threads;
-    b->use_graph = o.use_graph != 0;
-    b->passes = (slots + BC - 1) / BC;
-    b->rows_x = (m->rows + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
-    b->cols_x = (m->cols + kTileXPad - 1) / kTileXPad * kTileXPad + kTileXPad;
-    HIPCHK(hipStreamCreateWithFlags(&b->stream, hipStreamNonBlocking));
-    int cus = 256, dev = 0;
-    if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const uint32_t groups = o.tile_groups ? o.tile_groups : (BC == 1 ? 24u : 8u) * (uint32_t)std::max(1, cus);
-    if (int st = setup_tiled(m, 0, W, R, o.values, BC, groups, &b->tfp, &b->wfp, &b->part_fp)) return st;
-    if (int st = setup_tiled(m, 1, W, R, o.values, BC, groups, &b->tbp, &b->wbp, &b->part_bp)) return st;
-    const size_t per = (size_t)BC * b->passes;
-    HIPCHK(b->g.alloc(sizeof(int32_t) * b->rows_x * per));
-    HIPCHK(b->ratio.alloc(sizeof(float) * b->rows_x * per));
-    HIPCHK(b->f.alloc(sizeof(float) * b->cols_x * per));
-    HIPCHK(b->sens.alloc(sizeof(float) * m->cols));
-    HIPCHK(b->active.alloc(sizeof(uint32_t) * per));
-    HIPCHK(b->stage.alloc(sizeof(float) * std::max<size_t>(m->rows, m->cols)));
-    HIPCHK(launch_fill(b->g.as<float>(), 0.0f, b->rows_x * per, b->stream));  // 0.0f has the bits of int 0
-    HIPCHK(launch_fill(b->ratio.as<float>(), 0.0f, b->rows_x * per, b->stream));
-    HIPCHK(launch_fill(b->f.as<float>(), 0.0f, b->cols_x * per, b->stream));
-    HIPCHK(launch_fill(b->active.as<float>(), 0.0f, per, b->stream));
-    b->active_host.assign(per, 0);
-    if (o.values != VAL_F32)  // the quantized matrix's own sensitivity
-        HIPCHK(hipMemcpyAsync(b->sens.p, b->tbp->sens.p, sizeof(float) * m->cols, hipMemcpyDeviceToDevice,
-                              b->stream));
-    else if (int st = sens_on_device(m, b->sens.as<float>(), nullptr, b->stream))
-        return st;
-    HIPCHK(hipStreamSynchronize(b->stream));
-    *out = b.release();
-    return MLEM_OK;
-}
-
-int mlem_batch_set(mlem_batch *b, uint32_t slot, const int32_t *g, const float *f0) {
-    ARGCHK(b && g && f0, "mlem_batch_set: null argument");
-    ARGCHK(slot < b->slots, "mlem_batch_set: slot %u >= %u", slot, b->slots);
-    HIPCHK(hipSetDevice(b->m->device));
-    const uint32_t p = slot / b->BC, q = slot % b->BC;
-    const mlem_matrix *m = b->m;
-    hipStream_t s = b->stream;
-    HIPCHK(hipMemcpyAsync(b->stage.p, g, sizeof(int32_t) * m->rows, hipMemcpyHostToDevice, s));
-    HIPCHK(launch_strided_put(b->g.as<int32_t>() + ((size_t)p * b->rows_x) * b->BC + q, b->BC, b->stage.p, m->rows, s));
-    HIPCHK(hipStreamSynchronize(s));  // the stage is reused below
-    HIPCHK(hipMemcpyAsync(b->stage.p, f0, sizeof(float) * m->cols, hipMemcpyHostToDevice, s));
-    HIPCHK(launch_strided_put(b->f.as<float>() + ((size_t)p * b->cols_x) * b->BC + q, b->BC, b->stage.p, m->cols, s));
-    b->active_host[slot] = 1;
-    HIPCHK(hipMemcpyAsync(b->active.p, b->active_host.data(), sizeof(uint32_t) * b->active_host.size(),
-                          hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
-    return MLEM_OK;
-}
-
-int mlem_batch_deactivate(mlem_batch *b, uint32_t slot) {
-    ARGCHK(b, "mlem_batch_deactivate: null handle");
-    ARGCHK(slot < b->slots, "mlem_batch_deactivate: slot %u >= %u", slot, b->slots);
-    HIPCHK(hipSetDevice(b->m->device));
-    b->active_host[slot] = 0;
-    HIPCHK(hipMemcpyAsync(b->active.p, b->active_host.data(), sizeof(uint32_t) * b->active_host.size(),
-                          hipMemcpyHostToDevice, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    return MLEM_OK;
-}
-
-namespace {
-// One iteration of the passes marked in `passes` on the batch's stream.
-int enqueue_batch_iteration(mlem_batch *b, const std::vector<uint8_t> &passes) {
-    hipStream_t s = b->stream;
-    const uint32_t BC = b->BC;
-    {
-        for (uint32_t p = 0; p < b->passes; ++p) {
-            if (!passes[p]) continue;
-            float *f = b->f.as<float>() + (size_t)p * b->cols_x * BC;
-            float *ratio = b->ratio.as<float>() + (size_t)p * b->rows_x * BC;
-            EpiArgs fa{};
-            fa.out = ratio;
-            fa.g = b->g.as<int32_t>() + (size_t)p * b->rows_x * BC;
-            HIPCHK(launch_tiled_partials(b->tfp->plan, b->tfp->view, b->wfp->view, f, b->part_fp.as<float>(), BC,
-                                         b->nt, s));
-            HIPCHK(launch_tiled_reduce(b->tfp->plan, b->tfp->view, b->wfp->view, b->part_fp.as<float>(), BC, EPI_RATIO,
-                                       fa, s));
-            EpiArgs ba{};
-            ba.norm = b->sens.as<float>();
-            ba.f = f;
-            ba.active = b->active.as<uint32_t>() + (size_t)p * BC;
-            HIPCHK(launch_tiled_partials(b->tbp->plan, b->tbp->view, b->wbp->view, ratio, b->part_bp.as<float>(), BC,
-                                         b->nt, s));
-            HIPCHK(launch_tiled_reduce(b->tbp->plan, b->tbp->view, b->wbp->view, b->part_bp.as<float>(), BC,
-                                       EPI_UPDATE, ba, s));
-        }
-    }
-    return MLEM_OK;
-}
-}  // namespace
-
-int mlem_batch_iterate(mlem_batch *b, uint32_t iters) {
-    ARGCHK(b, "mlem_batch_iterate: null handle");
-    HIPCHK(hipSetDevice(b->m->device));
-    const uint32_t BC = b->BC;
-    std::vector<uint8_t> passes(b->passes, 0);
-    bool any = false;
-    for (uint32_t p = 0; p < b->passes; ++p) {
-        for (uint32_t q = 0; q < BC; ++q) passes[p] = passes[p] || b->active_host[(size_t)p * BC + q];
-        any = any || passes[p];
-    }
-    if (!any || iters == 0) return MLEM_OK;
-    if (!b->use_graph) {
-        for (uint32_t it = 0; it < iters; ++it)
-            if (int st = enqueue_batch_iteration(b, passes)) return st;
-        return MLEM_OK;
-    }
-    if (!b->gexec || passes != b->graph_passes) {  // (re)capture for this set of passes
-        if (b->gexec) {
-            HIPCHK(hipStreamSynchronize(b->stream));
-            HIPCHK(hipGraphExecDestroy(b->gexec));
-            b->gexec = nullptr;
-        }
-        HIPCHK(hipStreamBeginCapture(b->stream, hipStreamCaptureModeThreadLocal));
-        const int st = enqueue_batch_iteration(b, passes);
-        hipGraph_t gr = nullptr;
-        const hipError_t e2 = hipStreamEndCapture(b->stream, &gr);
-        if (st != MLEM_OK) {
-            if (gr) (void)hipGraphDestroy(gr);
-            return st;
-        }
-        HIPCHK(e2);
-        const hipError_t e3 = hipGraphInstantiate(&b->gexec, gr, nullptr, nullptr, 0);
-        (void)hipGraphDestroy(gr);
-        HIPCHK(e3);
-        b->graph_passes = passes;
-    }
-    for (uint32_t it = 0; it < iters; ++it) HIPCHK(hipGraphLaunch(b->gexec, b->stream));
-    return MLEM_OK;
-}
-
-int mlem_batch_sync(mlem_batch *b) {
-    ARGCHK(b, "mlem_batch_sync: null handle");
-    HIPCHK(hipSetDevice(b->m->device));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    return MLEM_OK;
-}
-
-int mlem_batch_get_image(mlem_batch *b, uint32_t slot, float *f_out) {
-    ARGCHK(b && f_out, "mlem_batch_get_image: null argument");
-    ARGCHK(slot < b->slots, "mlem_batch_get_image: slot %u >= %u", slot, b->slots);
-    HIPCHK(hipSetDevice(b->m->device));
-    const uint32_t p = slot / b->BC, q = slot % b->BC;
-    HIPCHK(launch_strided_get(b->stage.p, b->f.as<float>() + ((size_t)p * b->cols_x) * b->BC + q, b->BC, b->m->cols,
-                              b->stream));
-    HIPCHK(hipMemcpyAsync(f_out, b->stage.p, sizeof(float) * b->m->cols, hipMemcpyDeviceToHost, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    return MLEM_OK;
-}
-
-int mlem_batch_info(const mlem_batch *b, uint32_t *slots, uint32_t *per_pass, uint32_t *passes,
-                    uint64_t *device_bytes) {
-    ARGCHK(b, "mlem_batch_info: null handle");
-    if (slots) *slots = b->slots;
-    if (per_pass) *per_pass = b->BC;
-    if (passes) *passes = b->passes;
-    if (device_bytes)
-        *device_bytes = b->g.bytes + b->ratio.bytes + b->f.bytes + b->sens.bytes + b->active.bytes + b->stage.bytes +
-                        b->part_fp.bytes + b->part_bp.bytes;
-    return MLEM_OK;
-}
-
-void *mlem_batch_stream(mlem_batch *b) { return b ? b->stream : nullptr; }
-
-int mlem_batch_destroy(mlem_batch *b) {
-    if (!b) return MLEM_OK;
-    (void)hipSetDevice(b->m->device);
-    delete b;
     return MLEM_OK;
 }
 

@@ -1,0 +1,222 @@
+// Internal declarations shared by the C ABI's translation units (abi.hip,
+// tiles_only.hip, batch.hip): error reporting, device buffers, the handles
+// behind the opaque types of mlem_hip.h, and the helpers that build
+// partitions, tiles and work lists on first use.  Not installed.
+#pragma once
+
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cstdlib>
+#include <array>
+#include <chrono>
+#include <cstdarg>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <set>
+#include <tuple>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <thread>
+#include <vector>
+
+#include "launch.hpp"
+#include "mlem_hip.h"
+#include "mlem_host.h"
+
+namespace mlem {
+namespace abi {
+
+// The calling thread's last error message (mlem_last_error()).
+extern thread_local std::string g_err;
+// Sets the message, returns st.
+int set_err(int st, const char *fmt, ...) __attribute__((format(printf, 2, 3)));
+// A HIP error as a status (out of memory, no device, else a HIP error) with
+// the failing call and its place.
+int hip_err(hipError_t e, const char *what, const char *file, int line);
+
+inline hipStream_t S(void *s) { return static_cast<hipStream_t>(s); }
+
+}  // namespace abi
+}  // namespace mlem
+
+#define HIPCHK(expr)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (expr);                                                              \
+        if (e_ != hipSuccess) return mlem::abi::hip_err(e_, #expr, __FILE_NAME__, __LINE__); \
+    } while (0)
+
+#define ARGCHK(cond, ...)                                                                \
+    do {                                                                                 \
+        if (!(cond)) return mlem::abi::set_err(MLEM_ERR_INVALID_ARGUMENT, __VA_ARGS__);   \
+    } while (0)
+
+namespace mlem {
+namespace abi {
+
+// Device memory owned by a handle.
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { reset(); }
+    void reset() {
+        if (p) (void)hipFree(p);
+        p = nullptr;
+        bytes = 0;
+    }
+    hipError_t alloc(size_t n) {
+        reset();
+        bytes = n;
+        if (n == 0) return hipSuccess;
+        return hipMalloc(&p, n);
+    }
+    template <typename T>
+    T *as() const {
+        return static_cast<T *>(p);
+    }
+};
+
+// Host-side checks of a CSR before it reaches the GPU: a bad row pointer or
+// column index would make the kernels read outside their arrays.
+int check_csr_host(const char *what, uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
+                   const uint32_t *col);
+
+inline uint32_t clamp_secsize(uint32_t s) { return std::min(std::max(s, 1u), kMaxSecsize); }
+
+// The SpMV kernels read col/val with 16-byte vector loads.
+inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 15u) == 0; }
+
+}  // namespace abi
+}  // namespace mlem
+
+using namespace mlem;
+using namespace mlem::abi;
+
+// ---------------------------------------------------------------------------
+// handles
+// ---------------------------------------------------------------------------
+struct Partition {
+    MergePlan plan;
+    DevBuf buf;
+};
+
+// A work list of a tiled matrix on the device (one per unit size and
+// target workgroup count).
+struct TileWorkDev {
+    TileWork view;
+    DevBuf mem;
+};
+
+struct Tiled {
+    TilePlan plan;
+    DevBuf mem;
+    TiledMatrix view;
+    DevBuf sens;  // A^T tiles of 16-bit values: the quantized matrix's column sums (f32)
+    std::map<uint32_t, std::unique_ptr<TileWorkDev>> works;  // by target workgroup count
+};
+
+struct mlem_matrix {
+    int device = 0;
+    uint32_t rows = 0, cols = 0;
+    uint64_t nnz = 0;
+    DevBuf rowptr, col, val;
+    DevBuf rowptrT, colT, valT;
+    bool has_T = false;
+    // Tiles only (mlem_matrix_create_tiled): no CSR copies on the device, the
+    // tiles of A and A^T (width tile_W, tile_R rows, f32) and the
+    // sensitivity taken while they were built.
+    bool tiles_only = false;
+    uint32_t tile_W = 0, tile_R = 0, tile_vt = 0, tile_bc = 1;  // tile_bc: right-hand sides per pass
+    DevBuf sens32, sens64;
+    std::mutex mu;
+    std::map<std::pair<int, uint32_t>, std::unique_ptr<Partition>> parts;  // (0: A / 1: A^T, secsize)
+    // Tiles keyed by (0: A / 1: A^T, width W, rows R, value type, quad groups).
+    using TileKey = std::tuple<int, uint32_t, uint32_t, uint32_t, bool>;
+    std::map<TileKey, std::unique_ptr<Tiled>> tiles;
+    std::map<TileKey, uint32_t> tile_refused;  // build flags of refused tilings
+
+    uint64_t device_bytes() const {
+        uint64_t b = rowptr.bytes + col.bytes + val.bytes + rowptrT.bytes + colT.bytes + valT.bytes + sens32.bytes +
+                     sens64.bytes;
+        for (const auto &t : tiles) b += t.second->mem.bytes;
+        return b;
+    }
+};
+
+// Per profiled iteration: FP start, FP SpMV kernel done, FP done, BP SpMV
+// kernel done, BP done.
+struct IterEvents {
+    hipEvent_t e[5];
+};
+
+struct mlem_batch {
+    mlem_matrix *m = nullptr;
+    uint32_t slots = 0, BC = 4, passes = 0;
+    uint32_t nt = 0;                // work-items per workgroup of the stream kernel (0: its default)
+    size_t rows_x = 0, cols_x = 0;  // padded vector lengths
+    hipStream_t stream = nullptr;
+    // [passes][rows_x][BC] counts and ratios, [passes][cols_x][BC] images
+    DevBuf g, ratio, f, sens, active, stage, part_fp, part_bp;
+    const Tiled *tfp = nullptr, *tbp = nullptr;
+    const TileWorkDev *wfp = nullptr, *wbp = nullptr;
+    std::vector<uint32_t> active_host;
+    // One iteration captured as a graph (use_graph), for the passes that
+    // held an active slot when it was captured.
+    bool use_graph = false;
+    hipGraphExec_t gexec = nullptr;
+    std::vector<uint8_t> graph_passes;
+    ~mlem_batch() {
+        if (stream) (void)hipStreamSynchronize(stream);  // no replay still running
+        if (gexec) (void)hipGraphExecDestroy(gexec);
+        if (stream) (void)hipStreamDestroy(stream);
+    }
+};
+
+struct mlem_recon {
+    const mlem_matrix *m = nullptr;
+    mlem_opts opts{};
+    hipStream_t stream = nullptr;
+    DevBuf g, sens, f, ratio, bp, scratch_fp, scratch_bp, llpart;
+    const Partition *pfp = nullptr, *pbp = nullptr;
+    const Tiled *tfp = nullptr, *tbp = nullptr;        // blocked LDS-tiled path, when used
+    const TileWorkDev *wfp = nullptr, *wbp = nullptr;  // and its work lists
+    DevBuf part_fp, part_bp;
+    uint32_t lanes_fp = 64, lanes_bp = 64;
+    bool bp_merge = true;  // BP on A^T: merge-based or CSR-vector
+    std::vector<IterEvents> events;  // pool, one set per profiled iteration
+    size_t events_used = 0;
+    double phase_ms[4] = {0, 0, 0, 0};  // FP SpMV, FP epilogue, BP SpMV, BP epilogue
+    uint32_t timed = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+};
+
+namespace mlem {
+namespace abi {
+
+// Merge-path partition of A (which 0) or A^T (1) for secsize, built on first use.
+int get_partition(mlem_matrix *m, int which, uint32_t secsize, const Partition **out);
+// Tiles of A or A^T, built on first use from the resident CSR (a
+// tiles-only matrix has only the ones it was built with).
+int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bool quad, const Tiled **out,
+              uint32_t *flags);
+// Work list of t for about `target` workgroups, built on first use.
+int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t target, const TileWorkDev **out);
+// Tiles, work list and partial-sum buffer of one side for BC right-hand sides.
+int setup_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, uint32_t BC, uint32_t groups,
+                const Tiled **tiled, const TileWorkDev **work, DevBuf *part);
+// The blocked or the plain merge path for one side of a reconstruction.
+int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t secsize, const Tiled **tiled,
+                     const TileWorkDev **work, DevBuf *part, const Partition **partn, DevBuf *scratch);
+// FP64 column sums of A (s32 and s64 may be null).
+int sens_on_device(const mlem_matrix *m, float *s32, double *s64_or_null, hipStream_t s);
+// Progress of long builds on stderr when MLEM_VERBOSE is set.
+void progress(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
+
+}  // namespace abi
+}  // namespace mlem

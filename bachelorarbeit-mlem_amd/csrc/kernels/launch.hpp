@@ -1,4 +1,4 @@
-// Internal launchers shared by the C ABI (abi.hip) and the kernels
+// Internal launchers shared by the C ABI (abi.hip, batch.hip, tiles_only.hip) and the kernels
 // (spmv.hip, transpose.hip).  All launchers are asynchronous on `s` and
 // return hipSuccess or the launch error.
 #pragma once
