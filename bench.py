@@ -296,8 +296,15 @@ def run_serve(args, M, threads):
     log(f"server set-up {time.time() - t:.2f} s: {srv.info()}")
     rng = np.random.default_rng(2027)
     n = args.serve_requests
-    choices = [int(c) for c in args.serve_iters.split(",")]
-    reqs = [(int(rng.integers(len(mats))), int(rng.integers(4)), int(rng.choice(choices))) for _ in range(n)]
+    if "-" in args.serve_iters:  # uniform integers lo..hi
+        lo, hi = (int(c) for c in args.serve_iters.split("-"))
+        draw_iters = lambda: int(rng.integers(lo, hi + 1))
+        iters_desc = f"uniform {lo}..{hi}"
+    else:
+        choices = [int(c) for c in args.serve_iters.split(",")]
+        draw_iters = lambda: int(rng.choice(choices))
+        iters_desc = choices
+    reqs = [(int(rng.integers(len(mats))), int(rng.integers(4)), draw_iters()) for _ in range(n)]
     # warm-up: one request per matrix
     serve_drive(srv, [(i, 0, 2) for i in range(len(mats))], np.zeros(len(mats)), mats)
     lat1, span1 = serve_drive(srv, reqs, np.zeros(n), mats)
@@ -306,13 +313,20 @@ def run_serve(args, M, threads):
     serial_s = n / sspan1
     log(f"backlog of {n} requests served in {span1:.3f} s: {recon_s:.2f} recon/s (one at a time: {sspan1:.3f} s, "
         f"{serial_s:.2f} recon/s; {recon_s / serial_s:.2f}x)")
-    rate = args.serve_load * recon_s
-    arr = np.cumsum(rng.exponential(1.0 / rate, n))
-    lat2, span2 = serve_drive(srv, reqs, arr, mats)
-    ser2, _ = serial_fifo(M, reqs, arr, mats)
     pct = lambda a, q: round(float(np.percentile(a, q)) * 1e3, 3)
-    log(f"Poisson arrivals at {rate:.2f} req/s: p50 {pct(lat2, 50)} ms, p99 {pct(lat2, 99)} ms "
-        f"(one at a time: p50 {pct(ser2, 50)} ms, p99 {pct(ser2, 99)} ms)")
+    # Poisson arrivals over a sweep of rates (fractions of the backlog throughput); the same arrival times are
+    # replayed through the one-at-a-time path.  The first load is the headline latency.
+    sweep = []
+    for load in [float(c) for c in args.serve_load.split(",")]:
+        rate = load * recon_s
+        arr = np.cumsum(np.random.default_rng(2028).exponential(1.0 / rate, n))
+        lat, _ = serve_drive(srv, reqs, arr, mats)
+        ser, _ = serial_fifo(M, reqs, arr, mats)
+        log(f"Poisson arrivals at {rate:.2f} req/s ({load} x): p50 {pct(lat, 50)} ms, p99 {pct(lat, 99)} ms "
+            f"(one at a time: p50 {pct(ser, 50)} ms, p99 {pct(ser, 99)} ms)")
+        sweep.append(dict(load=load, rate=round(rate, 3), p50=pct(lat, 50), p99=pct(lat, 99), max=pct(lat, 100),
+                          one_at_a_time_p50=pct(ser, 50), one_at_a_time_p99=pct(ser, 99)))
+    head = sweep[0]
     info = srv.info()
     srv.close()
     for d in mats:
@@ -322,12 +336,14 @@ def run_serve(args, M, threads):
         "warmup": len(mats), "ms_per_step": round(span1 / n * 1e3, 4), "higher_is_better": True, "scaling": "weak",
         "vs_baseline": None, "dtype": f"{args.values} values, f32 arithmetic", "data": "synthetic",
         "config": {"workload": "served mix: " + " + ".join(d["label"] for d in mats), "matrices": names,
-                   "requests": n, "iterations_per_request": choices, "slots_per_matrix": args.serve_slots,
+                   "requests": n, "iterations_per_request": iters_desc, "slots_per_matrix": args.serve_slots,
                    "per_pass": args.per_pass, "values": args.values, "hbm_used_bytes": info["used"]},
-        "latency_ms": {"arrivals": f"Poisson, {rate:.3f} req/s ({args.serve_load} x backlog throughput)",
-                       "p50": pct(lat2, 50), "p99": pct(lat2, 99), "max": pct(lat2, 100),
+        "latency_ms": {"arrivals": f"Poisson, {head['rate']:.3f} req/s ({head['load']} x backlog throughput)",
+                       "p50": head["p50"], "p99": head["p99"], "max": head["max"],
                        "backlog_p50": pct(lat1, 50), "backlog_p99": pct(lat1, 99)},
-        "one_at_a_time_f32": {"recon_s": round(serial_s, 3), "p50_ms": pct(ser2, 50), "p99_ms": pct(ser2, 99),
+        "latency_sweep_ms": sweep,
+        "one_at_a_time_f32": {"recon_s": round(serial_s, 3), "p50_ms": head["one_at_a_time_p50"],
+                              "p99_ms": head["one_at_a_time_p99"],
                               "ms_per_iteration": {d["name"]: round(d["t_single"] * 1e3, 4) for d in mats},
                               "note": "the same requests and arrivals served first come first served by the "
                                       "single-reconstruction FP32 path, one handle per request, measured"},
@@ -372,9 +388,11 @@ def main():
                     help="serving mode over these workloads (comma list of c1, mid, c2): continuous batching")
     ap.add_argument("--serve-requests", type=int, default=48, help="serving mode: number of requests")
     ap.add_argument("--serve-slots", type=int, default=16, help="serving mode: slots per matrix")
-    ap.add_argument("--serve-iters", default="20,50,100", help="serving mode: iteration counts drawn per request")
-    ap.add_argument("--serve-load", type=float, default=0.7,
-                    help="serving mode: Poisson arrival rate as a fraction of the backlog throughput")
+    ap.add_argument("--serve-iters", default="10-100",
+                    help="serving mode: iterations per request, uniform over lo-hi or drawn from a comma list")
+    ap.add_argument("--serve-load", default="0.7,0.3,0.5,0.9",
+                    help="serving mode: Poisson arrival rates as fractions of the backlog throughput (the first is "
+                         "the headline latency)")
     ap.add_argument("--load-test", action="store_true",
                     help="also write the matrix as a csr4 file and time the streamed loader on it")
     ap.add_argument("--calibrate-copy", action="store_true",
