@@ -10,6 +10,9 @@
 // update, which also re-zeroes bp for the next iteration.
 #include "abi_internal.hpp"
 
+using namespace mlem;
+using namespace mlem::abi;
+
 namespace mlem {
 namespace abi {
 

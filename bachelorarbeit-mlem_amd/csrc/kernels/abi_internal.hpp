@@ -94,29 +94,26 @@ inline bool aligned16(const void *p) { return (reinterpret_cast<uintptr_t>(p) & 
 }  // namespace abi
 }  // namespace mlem
 
-using namespace mlem;
-using namespace mlem::abi;
-
 // ---------------------------------------------------------------------------
 // handles
 // ---------------------------------------------------------------------------
 struct Partition {
-    MergePlan plan;
-    DevBuf buf;
+    mlem::MergePlan plan;
+    mlem::abi::DevBuf buf;
 };
 
 // A work list of a tiled matrix on the device (one per unit size and
 // target workgroup count).
 struct TileWorkDev {
-    TileWork view;
-    DevBuf mem;
+    mlem::TileWork view;
+    mlem::abi::DevBuf mem;
 };
 
 struct Tiled {
-    TilePlan plan;
-    DevBuf mem;
-    TiledMatrix view;
-    DevBuf sens;  // A^T tiles of 16-bit values: the quantized matrix's column sums (f32)
+    mlem::TilePlan plan;
+    mlem::abi::DevBuf mem;
+    mlem::TiledMatrix view;
+    mlem::abi::DevBuf sens;  // A^T tiles of 16-bit values: the quantized matrix's column sums (f32)
     std::map<uint32_t, std::unique_ptr<TileWorkDev>> works;  // by target workgroup count
 };
 
@@ -124,15 +121,15 @@ struct mlem_matrix {
     int device = 0;
     uint32_t rows = 0, cols = 0;
     uint64_t nnz = 0;
-    DevBuf rowptr, col, val;
-    DevBuf rowptrT, colT, valT;
+    mlem::abi::DevBuf rowptr, col, val;
+    mlem::abi::DevBuf rowptrT, colT, valT;
     bool has_T = false;
     // Tiles only (mlem_matrix_create_tiled): no CSR copies on the device, the
     // tiles of A and A^T (width tile_W, tile_R rows, f32) and the
     // sensitivity taken while they were built.
     bool tiles_only = false;
     uint32_t tile_W = 0, tile_R = 0, tile_vt = 0, tile_bc = 1;  // tile_bc: right-hand sides per pass
-    DevBuf sens32, sens64;
+    mlem::abi::DevBuf sens32, sens64;
     std::mutex mu;
     std::map<std::pair<int, uint32_t>, std::unique_ptr<Partition>> parts;  // (0: A / 1: A^T, secsize)
     // Tiles keyed by (0: A / 1: A^T, width W, rows R, value type, quad groups).
@@ -161,7 +158,7 @@ struct mlem_batch {
     size_t rows_x = 0, cols_x = 0;  // padded vector lengths
     hipStream_t stream = nullptr;
     // [passes][rows_x][BC] counts and ratios, [passes][cols_x][BC] images
-    DevBuf g, ratio, f, sens, active, stage, part_fp, part_bp;
+    mlem::abi::DevBuf g, ratio, f, sens, active, stage, part_fp, part_bp;
     const Tiled *tfp = nullptr, *tbp = nullptr;
     const TileWorkDev *wfp = nullptr, *wbp = nullptr;
     std::vector<uint32_t> active_host;
@@ -181,11 +178,11 @@ struct mlem_recon {
     const mlem_matrix *m = nullptr;
     mlem_opts opts{};
     hipStream_t stream = nullptr;
-    DevBuf g, sens, f, ratio, bp, scratch_fp, scratch_bp, llpart;
+    mlem::abi::DevBuf g, sens, f, ratio, bp, scratch_fp, scratch_bp, llpart;
     const Partition *pfp = nullptr, *pbp = nullptr;
     const Tiled *tfp = nullptr, *tbp = nullptr;        // blocked LDS-tiled path, when used
     const TileWorkDev *wfp = nullptr, *wbp = nullptr;  // and its work lists
-    DevBuf part_fp, part_bp;
+    mlem::abi::DevBuf part_fp, part_bp;
     uint32_t lanes_fp = 64, lanes_bp = 64;
     bool bp_merge = true;  // BP on A^T: merge-based or CSR-vector
     std::vector<IterEvents> events;  // pool, one set per profiled iteration

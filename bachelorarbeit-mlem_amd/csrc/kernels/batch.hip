@@ -2,6 +2,9 @@
 // tile-shape autotuner (mlem_autotune).
 #include "abi_internal.hpp"
 
+using namespace mlem;
+using namespace mlem::abi;
+
 namespace {
 
 // One iteration of the passes marked in `passes` on the batch's stream.

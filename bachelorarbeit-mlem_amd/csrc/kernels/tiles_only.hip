@@ -3,6 +3,9 @@
 // to stay on the device beside them.
 #include "abi_internal.hpp"
 
+using namespace mlem;
+using namespace mlem::abi;
+
 namespace {
 
 // Tiles (one right-hand side; f32 or 16-bit values) of a matrix that is not
