@@ -650,7 +650,26 @@ __global__ void blocked_reduce_kernel(const float *__restrict__ part, const uint
     float y[BC];
 #pragma unroll
     for (int q = 0; q < BC; ++q) y[q] = 0.0f;
-    for (uint32_t w = wl_first[b]; w < wl_first[b + 1]; ++w) {
+    // The partials of U workgroups are loaded together, then added in
+    // workgroup order: one memory latency per U workgroups instead of one
+    // each (a block of a small matrix can have hundreds), the same sums.
+    constexpr uint32_t U = BC == 1 ? 16 : (BC == 4 ? 8 : 4);
+    uint32_t w = wl_first[b];
+    const uint32_t we = wl_first[b + 1];
+    for (; w + U <= we; w += U) {
+        float t[U][BC];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const float *pr = part + ((((uint64_t)(w + u)) << rshift) + i) * BC;
+#pragma unroll
+            for (int q = 0; q < BC; ++q) t[u][q] = pr[q];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < BC; ++q) y[q] += t[u][q];
+    }
+    for (; w < we; ++w) {
         const float *pr = part + (((uint64_t)w << rshift) + i) * BC;
 #pragma unroll
         for (int q = 0; q < BC; ++q) y[q] += pr[q];
