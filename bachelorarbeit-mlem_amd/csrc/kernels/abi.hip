@@ -227,6 +227,8 @@ int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t target, const TileWorkDev *
     w->view.span = L.span;
     w->view.nwg = L.nwg;
     w->view.ntl = L.ntl;
+    for (size_t b = 0; b + 1 < L.wl_first.size(); ++b)
+        w->view.max_runs = std::max(w->view.max_runs, L.wl_first[b + 1] - L.wl_first[b]);
     w->view.wl = reinterpret_cast<const uint4 *>(base);
     w->view.tiles = reinterpret_cast<const uint4 *>(base + o_tl);
     w->view.wl_first = reinterpret_cast<const uint32_t *>(base + o_first);

@@ -144,6 +144,7 @@ struct TileWorkList {
 // The same on the device.
 struct TileWork {
     uint32_t span = 1, nwg = 0, ntl = 0;
+    uint32_t max_runs = 0;  // the most runs of one row block
     const uint4 *wl = nullptr;
     const uint4 *tiles = nullptr;
     const uint32_t *wl_first = nullptr;

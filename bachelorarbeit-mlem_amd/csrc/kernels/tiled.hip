@@ -637,43 +637,12 @@ __global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t ns
     for (uint32_t i = tid; i < R * BC; i += NT) pw[i] = acc[i];
 }
 
-// Row r of block b = r / R sums the partials of the block's workgroups
-// wl_first[b] .. wl_first[b+1]-1 in order (for each of the BC right-hand
-// sides), multiplies by the row's value scale (16-bit values) and applies
-// the epilogue.  Vectors are [rows][BC].
+// The epilogue of row r from its sum y over the block's workgroups: the
+// row's value scale (16-bit values), then the ratio, the update or the plain
+// sum.  Vectors are [rows][BC].
 template <int EPI, int BC>
-__global__ void blocked_reduce_kernel(const float *__restrict__ part, const uint32_t *__restrict__ wl_first,
-                                      uint32_t rshift, uint32_t rows, const float *__restrict__ scale, EpiArgs ea) {
-    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
-    if (r >= rows) return;
-    const uint32_t b = r >> rshift, i = r & ((1u << rshift) - 1u);
-    float y[BC];
-#pragma unroll
-    for (int q = 0; q < BC; ++q) y[q] = 0.0f;
-    // The partials of U workgroups are loaded together, then added in
-    // workgroup order: one memory latency per U workgroups instead of one
-    // each (a block of a small matrix can have hundreds), the same sums.
-    constexpr uint32_t U = BC == 1 ? 16 : (BC == 4 ? 8 : 4);
-    uint32_t w = wl_first[b];
-    const uint32_t we = wl_first[b + 1];
-    for (; w + U <= we; w += U) {
-        float t[U][BC];
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u) {
-            const float *pr = part + ((((uint64_t)(w + u)) << rshift) + i) * BC;
-#pragma unroll
-            for (int q = 0; q < BC; ++q) t[u][q] = pr[q];
-        }
-#pragma unroll
-        for (uint32_t u = 0; u < U; ++u)
-#pragma unroll
-            for (int q = 0; q < BC; ++q) y[q] += t[u][q];
-    }
-    for (; w < we; ++w) {
-        const float *pr = part + (((uint64_t)w << rshift) + i) * BC;
-#pragma unroll
-        for (int q = 0; q < BC; ++q) y[q] += pr[q];
-    }
+__device__ __forceinline__ void reduce_epilogue(uint32_t r, const float (&y)[BC], const float *__restrict__ scale,
+                                                const EpiArgs &ea) {
     const float sc = scale ? scale[r] : 1.0f;
 #pragma unroll
     for (int q = 0; q < BC; ++q) {
@@ -692,6 +661,82 @@ __global__ void blocked_reduce_kernel(const float *__restrict__ part, const uint
             if (!ea.active || ea.active[q]) ea.f[k] = (n == 0.0f) ? v : v / n;
         }
     }
+}
+
+// y += the partials of row i (of R = 2^rshift) in workgroups w0, w0 + S,
+// ... below we, in that order.  The partials of U workgroups are loaded
+// together, then added in order: one memory latency per U workgroups
+// instead of one each.
+template <int BC, uint32_t S>
+__device__ __forceinline__ void sum_partials(float (&y)[BC], const float *__restrict__ part, uint32_t w0,
+                                             uint32_t we, uint32_t rshift, uint32_t i) {
+    constexpr uint32_t U = BC == 1 ? 16 : (BC == 4 ? 8 : 4);
+    uint32_t w = w0;
+    for (; w + (U - 1) * S < we; w += U * S) {
+        float t[U][BC];
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u) {
+            const float *pr = part + ((((uint64_t)(w + u * S)) << rshift) + i) * BC;
+#pragma unroll
+            for (int q = 0; q < BC; ++q) t[u][q] = pr[q];
+        }
+#pragma unroll
+        for (uint32_t u = 0; u < U; ++u)
+#pragma unroll
+            for (int q = 0; q < BC; ++q) y[q] += t[u][q];
+    }
+    for (; w < we; w += S) {
+        const float *pr = part + (((uint64_t)w << rshift) + i) * BC;
+#pragma unroll
+        for (int q = 0; q < BC; ++q) y[q] += pr[q];
+    }
+}
+
+// Row r of block b = r / R sums the partials of the block's workgroups
+// wl_first[b] .. wl_first[b+1]-1 in order (for each of the BC right-hand
+// sides) and applies the epilogue.
+template <int EPI, int BC>
+__global__ void blocked_reduce_kernel(const float *__restrict__ part, const uint32_t *__restrict__ wl_first,
+                                      uint32_t rshift, uint32_t rows, const float *__restrict__ scale, EpiArgs ea) {
+    const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
+    if (r >= rows) return;
+    const uint32_t b = r >> rshift, i = r & ((1u << rshift) - 1u);
+    float y[BC];
+#pragma unroll
+    for (int q = 0; q < BC; ++q) y[q] = 0.0f;
+    sum_partials<BC, 1>(y, part, wl_first[b], wl_first[b + 1], rshift, i);
+    reduce_epilogue<EPI, BC>(r, y, scale, ea);
+}
+
+// The same for blocks with many workgroups (a small matrix spread over the
+// whole chip has hundreds per block): wave j of a (64, kSlices) workgroup
+// sums workgroups j, j + kSlices, ... of each of 64 consecutive rows, then
+// wave 0 adds the kSlices sums in slice order.
+constexpr uint32_t kSlices = 8;
+constexpr uint32_t kSlicedRuns = 64;  // runs of a row block from which the sliced sum is used
+template <int EPI, int BC>
+__global__ __launch_bounds__(64 * kSlices) void blocked_reduce_sliced_kernel(
+    const float *__restrict__ part, const uint32_t *__restrict__ wl_first, uint32_t rshift, uint32_t rows,
+    const float *__restrict__ scale, EpiArgs ea) {
+    __shared__ float s_y[kSlices][64][BC];
+    const uint32_t x = threadIdx.x, j = threadIdx.y;
+    const uint32_t r = blockIdx.x * 64 + x;
+    float y[BC];
+#pragma unroll
+    for (int q = 0; q < BC; ++q) y[q] = 0.0f;
+    if (r < rows) {
+        const uint32_t b = r >> rshift, i = r & ((1u << rshift) - 1u);
+        sum_partials<BC, kSlices>(y, part, wl_first[b] + j, wl_first[b + 1], rshift, i);
+    }
+#pragma unroll
+    for (int q = 0; q < BC; ++q) s_y[j][x][q] = y[q];
+    __syncthreads();
+    if (j != 0 || r >= rows) return;
+#pragma unroll
+    for (uint32_t jj = 1; jj < kSlices; ++jj)
+#pragma unroll
+        for (int q = 0; q < BC; ++q) y[q] += s_y[jj][x][q];
+    reduce_epilogue<EPI, BC>(r, y, scale, ea);
 }
 
 // ----------------------------------------------------------------- builder
@@ -1393,25 +1438,26 @@ hipError_t launch_tiled_partials(const TilePlan &p, const TiledMatrix &tm, const
 bool valid_stream_threads(uint32_t nt) { return nt == 0 || nt == 128 || nt == 256 || nt == 512; }
 
 namespace {
+template <int EPI, int BC>
+void launch_reduce_epi(const TilePlan &p, const TileWork &wk, const float *part, const float *scale, uint32_t rshift,
+                       const EpiArgs &ea, hipStream_t s) {
+    if (wk.max_runs >= kSlicedRuns)
+        hipLaunchKernelGGL((blocked_reduce_sliced_kernel<EPI, BC>), dim3(blocks_for(p.rows, 64)), dim3(64, kSlices), 0,
+                           s, part, wk.wl_first, rshift, p.rows, scale, ea);
+    else
+        hipLaunchKernelGGL((blocked_reduce_kernel<EPI, BC>), dim3(blocks_for(p.rows, kWG)), dim3(kWG), 0, s, part,
+                           wk.wl_first, rshift, p.rows, scale, ea);
+}
+
 template <int BC>
 hipError_t launch_reduce_bc(const TilePlan &p, const TileWork &wk, const float *part, const float *scale, Epi epi,
                             const EpiArgs &ea, hipStream_t s) {
     uint32_t rshift = 0;
     while ((1u << rshift) < p.R) ++rshift;
-    const dim3 grid(blocks_for(p.rows, kWG)), block(kWG);
     switch (epi) {
-        case EPI_PLAIN:
-            hipLaunchKernelGGL((blocked_reduce_kernel<EPI_PLAIN, BC>), grid, block, 0, s, part, wk.wl_first, rshift,
-                               p.rows, scale, ea);
-            break;
-        case EPI_RATIO:
-            hipLaunchKernelGGL((blocked_reduce_kernel<EPI_RATIO, BC>), grid, block, 0, s, part, wk.wl_first, rshift,
-                               p.rows, scale, ea);
-            break;
-        default:
-            hipLaunchKernelGGL((blocked_reduce_kernel<EPI_UPDATE, BC>), grid, block, 0, s, part, wk.wl_first, rshift,
-                               p.rows, scale, ea);
-            break;
+        case EPI_PLAIN: launch_reduce_epi<EPI_PLAIN, BC>(p, wk, part, scale, rshift, ea, s); break;
+        case EPI_RATIO: launch_reduce_epi<EPI_RATIO, BC>(p, wk, part, scale, rshift, ea, s); break;
+        default: launch_reduce_epi<EPI_UPDATE, BC>(p, wk, part, scale, rshift, ea, s); break;
     }
     return hipGetLastError();
 }
