@@ -100,11 +100,13 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
     ARGCHK(BC == 1 || BC == 4 || BC == 8, "mlem_matrix_create_tiled: per_pass must be 0, 1, 4 or 8");
     const bool quad = BC > 1;  // the batches' layout
     const uint32_t W = tile_width ? tile_width : (BC == 8 ? 512u : 1024u);  // as mlem_batch_create
-    const uint32_t R = tile_rows ? tile_rows : (BC == 1 ? 2048u : 1024u);
     ARGCHK(valid_tile_config(W, BC), "mlem_matrix_create_tiled: no blocked kernel for width %u with %u per pass", W, BC);
-    ARGCHK(valid_tile_rows(R), "mlem_matrix_create_tiled: tile_rows must be 512 ... 16384 (a power of two)");
+    ARGCHK(tile_rows == 0 || valid_tile_rows(tile_rows),
+           "mlem_matrix_create_tiled: tile_rows must be 512 ... 16384 (a power of two)");
     ARGCHK(values <= VAL_UF16, "mlem_matrix_create_tiled: values must be 0 (f32), 1 (bf16), 2 (fp16) or 3 (uf16)");
     HIPCHK(hipSetDevice(device));
+    // as a reconstruction over the resident CSR would choose
+    const uint32_t R = tile_rows ? tile_rows : (BC == 1 ? default_tile_rows(nnz, 0) : 1024u);
     auto m = std::make_unique<mlem_matrix>();
     m->device = device;
     m->rows = rows;

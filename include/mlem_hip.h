@@ -192,8 +192,11 @@ typedef struct mlem_opts {
                              1 = plain merge path; 1024 or 2048 = blocked
                              with that segment width (an error if the
                              matrix does not allow it)                       */
-    uint32_t tile_rows;   /* rows per block of the blocked kernels: 0 = 2048,
-                             or 512, 1024, 2048, 4096, 8192, 16384           */
+    uint32_t tile_rows;   /* rows per block of the blocked kernels: 0 = 2048
+                             (512 when nnz < 2 x 2048 x the workgroup
+                             count: a run's partial sums would cost as much
+                             as its entries), or 512, 1024, 2048, 4096,
+                             8192, 16384                                     */
     uint32_t tile_groups; /* workgroups per blocked SpMV (runs of equal
                              length over each row block's tiles):
                              0 = 24 per CU                                   */
@@ -240,7 +243,7 @@ int mlem_matrix_build_transpose(mlem_matrix *m);
 int mlem_matrix_build_transpose_limited(mlem_matrix *m, uint64_t scratch_bytes, uint32_t *ranges);
 /* A matrix resident as tiles only: the blocked layout of A and of A^T
  * (`values` as in mlem_opts; per_pass 0 or 1: the tiles of single
- * reconstructions, tile width 0 = 1024 or 2048, tile_rows 0 = 2048;
+ * reconstructions, tile width 0 = 1024 or 2048, tile_rows 0 as in mlem_opts;
  * per_pass 4 or 8: those of batches with that many per pass, widths and
  * rows as in mlem_batch_opts) and the FP64 sensitivity (of the quantized matrix for
  * 16-bit values), without the CSR copies, for matrices whose
