@@ -267,11 +267,12 @@ int setup_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, 
     return MLEM_OK;
 }
 
-uint32_t default_tile_rows(uint64_t nnz, uint32_t groups) {
+uint32_t default_tile_rows(uint64_t nnz, uint32_t groups, uint32_t BC) {
     int dev = 0, cus = 256;
     if (hipGetDevice(&dev) == hipSuccess) (void)hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev);
-    const uint64_t target = groups ? groups : 24u * (uint64_t)std::max(1, cus);
-    return nnz < target * 2 * kTileUnit ? 512u : 2048u;
+    const uint64_t target = groups ? groups : (BC == 1 ? 24u : 8u) * (uint64_t)std::max(1, cus);
+    if (nnz < target * 2 * kTileUnit) return 512u;
+    return BC == 1 ? 2048u : 1024u;
 }
 
 // Chooses the blocked (LDS-tiled) or the plain merge path for one side of
@@ -285,7 +286,7 @@ int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t sec
     *work = nullptr;
     if (o.tile_width != 1) {
         const uint32_t W = o.tile_width == 0 ? 1024u : o.tile_width;
-        const uint32_t R = o.tile_rows == 0 ? default_tile_rows(m->nnz, o.tile_groups) : o.tile_rows;
+        const uint32_t R = o.tile_rows == 0 ? default_tile_rows(m->nnz, o.tile_groups, 1) : o.tile_rows;
         const int st = setup_tiled(m, which, W, R, o.values, 1, o.tile_groups, tiled, work, part);
         if (st == MLEM_OK || m->tiles_only) return st;
         // Automatic width with f32 values: a matrix the tiles do not suit, or

@@ -212,12 +212,13 @@ int setup_merge_side(mlem_matrix *m, int which, const mlem_opts &o, uint32_t sec
                      const TileWorkDev **work, DevBuf *part, const Partition **partn, DevBuf *scratch);
 // FP64 column sums of A (s32 and s64 may be null).
 int sens_on_device(const mlem_matrix *m, float *s32, double *s64_or_null, hipStream_t s);
-// Rows per tile block of the one-right-hand-side blocked kernel when none is
-// asked for, on the current device: 512 when the matrix's runs (about
-// `groups` of them, 0: 24 per CU) would hold under two units of entries
-// each -- a run's partial sums (4 bytes per row of its block) would then
-// cost about as much as its entries -- else 2048.
-uint32_t default_tile_rows(uint64_t nnz, uint32_t groups);
+// Rows per tile block of the blocked kernels for BC right-hand sides when
+// none is asked for, on the current device: 512 when the matrix's runs
+// (about `groups` of them; 0: 24 per CU for one right-hand side, 8 for
+// several) would hold under two units of entries each -- a run's partial
+// sums (4 bytes per row of its block and right-hand side) would then cost
+// about as much as its entries -- else 2048 (one) or 1024 (several).
+uint32_t default_tile_rows(uint64_t nnz, uint32_t groups, uint32_t BC);
 // Progress of long builds on stderr when MLEM_VERBOSE is set.
 void progress(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 

@@ -204,13 +204,15 @@ int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opt
         o.values = m->tile_vt;
     }
     // Defaults: 1024 x 2048 for one per pass, 1024 x 1024 for 4 (3.28x the
-    // single FP32 path at C2 against 3.17x for 512 x 2048), 512 x 1024 for 8.
+    // single FP32 path at C2 against 3.17x for 512 x 2048), 512 x 1024 for 8;
+    // 512 rows for small matrices (default_tile_rows).
     const uint32_t W = o.tile_width ? o.tile_width : (BC == 8 ? 512u : 1024u);
-    const uint32_t R = o.tile_rows ? o.tile_rows : (BC == 1 ? 2048u : 1024u);
     ARGCHK(valid_tile_config(W, BC), "mlem_batch_create: no blocked kernel for width %u with %u per pass", W, BC);
-    ARGCHK(valid_tile_rows(R), "mlem_batch_create: tile_rows must be 512 ... 16384 (a power of two)");
+    ARGCHK(o.tile_rows == 0 || valid_tile_rows(o.tile_rows),
+           "mlem_batch_create: tile_rows must be 512 ... 16384 (a power of two)");
     ARGCHK(valid_stream_threads(o.tile_threads), "mlem_batch_create: tile_threads must be 0, 128, 256 or 512");
     HIPCHK(hipSetDevice(m->device));
+    const uint32_t R = o.tile_rows ? o.tile_rows : default_tile_rows(m->nnz, o.tile_groups, BC);
     auto b = std::make_unique<mlem_batch>();
     b->m = m;
     b->slots = slots;

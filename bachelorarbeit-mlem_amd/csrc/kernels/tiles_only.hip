@@ -106,7 +106,7 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
     ARGCHK(values <= VAL_UF16, "mlem_matrix_create_tiled: values must be 0 (f32), 1 (bf16), 2 (fp16) or 3 (uf16)");
     HIPCHK(hipSetDevice(device));
     // as a reconstruction over the resident CSR would choose
-    const uint32_t R = tile_rows ? tile_rows : (BC == 1 ? default_tile_rows(nnz, 0) : 1024u);
+    const uint32_t R = tile_rows ? tile_rows : default_tile_rows(nnz, 0, BC);
     auto m = std::make_unique<mlem_matrix>();
     m->device = device;
     m->rows = rows;

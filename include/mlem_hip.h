@@ -348,7 +348,8 @@ typedef struct mlem_batch_opts {
     uint32_t per_pass;    /* slots per pass over the matrix: 1, 4, 8
                              (0 = 4)                                       */
     uint32_t tile_width;  /* 0 = automatic (512 for 8 per pass, else 1024) */
-    uint32_t tile_rows;   /* 0 = automatic (2048 for 1 per pass, else 1024) */
+    uint32_t tile_rows;   /* 0 = automatic (2048 for 1 per pass, else 1024;
+                             512 for small matrices, as in mlem_opts)        */
     uint32_t tile_groups; /* workgroups per blocked SpMV, 0 = automatic    */
     uint32_t tile_threads; /* work-items per workgroup of the kernel for
                               4 or 8 per pass: 128, 256 or 512 (each takes
