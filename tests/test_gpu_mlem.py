@@ -190,6 +190,32 @@ def test_blocked_work_split(c1, resident, ref50, groups):
     assert rel_l2(img, ref50["ref"]) <= REL_L2
 
 
+def test_default_row_blocks_follow_matrix_size(c1, resident, ref50):
+    """The 2D ring has under two units of entries per run at the default workgroup counts, so its default row
+    blocks are 512 rows: the default run is bitwise the 512-row run (one reconstruction and a batch of 4 per pass),
+    not the 2048-row one, and both are within the oracle bar.  A small workgroup target (runs of about four units)
+    gives the same matrix 2048-row blocks again."""
+    g, f0, s32 = c1["g"], ref50["f0"], ref50["s32"]
+    run = lambda **kw: M.mlem_run(resident, g, f0, 50, sens=s32, options=M.opts(spmv=0, using_trans=0, **kw))
+    auto, r512, r2048 = run(), run(tile_rows=512), run(tile_rows=2048)
+    assert np.array_equal(auto.view(np.uint32), r512.view(np.uint32))
+    assert not np.array_equal(auto.view(np.uint32), r2048.view(np.uint32))
+    for img in (auto, r2048):
+        assert rel_l2(img, ref50["ref"]) <= REL_L2
+    # runs of about four units of 2048 entries each
+    groups = int(c1["A"].nnz // 4096 // 2)
+    few_auto, few_2048 = run(tile_groups=groups), run(tile_groups=groups, tile_rows=2048)
+    assert np.array_equal(few_auto.view(np.uint32), few_2048.view(np.uint32))
+    imgs = []
+    for rows in (0, 512):
+        with M.Batch(resident, 4, per_pass=4, tile_rows=rows) as b:
+            for k in range(4):
+                b.set(k, g, f0)
+            b.iterate(10)
+            imgs.append(b.image(2))
+    assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
+
+
 def test_blocked_largest_tiles(c1, resident, ref50):
     # 16384 rows x 2048 columns per tile: 72 KiB of LDS per workgroup.
     o = M.opts(spmv=0, tile_width=2048, tile_rows=16384, secsize_fp=16, secsize_bp=16)
