@@ -217,13 +217,16 @@ int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t target, const TileWorkDev *
     const TileWorkList L = tile_work_list(p, nz.data(), off.data(), target);
     if (L.bad) return set_err(MLEM_ERR_FORMAT, "tiles (width %u, rows %u) are not aligned for the stream kernel", p.W, p.R);
     auto w = std::make_unique<TileWorkDev>();
-    const size_t b_wl = 4 * L.wl.size(), b_tl = 4 * L.tiles.size(), b_first = 4 * L.wl_first.size();
-    const size_t o_tl = (b_wl + 255) & ~size_t(255), o_first = o_tl + ((b_tl + 255) & ~size_t(255));
-    HIPCHK(w->mem.alloc(std::max<size_t>(256, o_first + b_first)));
+    const size_t b_wl = 4 * L.wl.size(), b_tl = 4 * L.tiles.size(), b_first = 4 * L.wl_first.size(),
+                 b_order = 4 * L.order.size();
+    const size_t o_tl = (b_wl + 255) & ~size_t(255), o_first = o_tl + ((b_tl + 255) & ~size_t(255)),
+                 o_order = o_first + ((b_first + 255) & ~size_t(255));
+    HIPCHK(w->mem.alloc(std::max<size_t>(256, o_order + b_order)));
     char *base = static_cast<char *>(w->mem.p);
     if (b_wl) HIPCHK(hipMemcpy(base, L.wl.data(), b_wl, hipMemcpyHostToDevice));
     if (b_tl) HIPCHK(hipMemcpy(base + o_tl, L.tiles.data(), b_tl, hipMemcpyHostToDevice));
     HIPCHK(hipMemcpy(base + o_first, L.wl_first.data(), b_first, hipMemcpyHostToDevice));
+    if (b_order) HIPCHK(hipMemcpy(base + o_order, L.order.data(), b_order, hipMemcpyHostToDevice));
     w->view.span = L.span;
     w->view.nwg = L.nwg;
     w->view.ntl = L.ntl;
@@ -232,6 +235,7 @@ int get_tile_work(mlem_matrix *m, Tiled *t, uint32_t target, const TileWorkDev *
     w->view.wl = reinterpret_cast<const uint4 *>(base);
     w->view.tiles = reinterpret_cast<const uint4 *>(base + o_tl);
     w->view.wl_first = reinterpret_cast<const uint32_t *>(base + o_first);
+    w->view.order = b_order ? reinterpret_cast<const uint32_t *>(base + o_order) : nullptr;
     *out = w.get();
     t->works.emplace(key, std::move(w));
     return MLEM_OK;

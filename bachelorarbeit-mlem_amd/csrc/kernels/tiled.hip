@@ -401,6 +401,7 @@ template <int W, int VT, int BC, int NT>
 __global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t nseg,
                                                          const uint64_t *__restrict__ tile_off,
                                                          const uint4 *__restrict__ wl,
+                                                         const uint32_t *__restrict__ order,
                                                          const uint4 *__restrict__ win,
                                                          const uint8_t *__restrict__ tdata,
                                                          const float *__restrict__ x, float *__restrict__ part) {
@@ -423,7 +424,8 @@ __global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t ns
 
     const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
     const uint32_t g0 = tid * GPW;  // the work-item's first group in a window
-    const uint4 w = wl[blockIdx.x];
+    const uint32_t run = order[blockIdx.x];  // workgroups start in column order, runs keep their place
+    const uint4 w = wl[run];
     const uint64_t e0 = tile_off[(uint64_t)w.x * nseg], e1 = tile_off[(uint64_t)(w.x + 1) * nseg];
     const uint64_t G = (e1 - e0) / 8;  // groups of the row block
     const uint8_t *vbase = tdata + (uint64_t)(VB + 2) * e0;
@@ -633,7 +635,7 @@ __global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t ns
         }
     }
     __syncthreads();
-    float *pw = part + (uint64_t)blockIdx.x * R * BC;
+    float *pw = part + (uint64_t)run * R * BC;
     for (uint32_t i = tid; i < R * BC; i += NT) pw[i] = acc[i];
 }
 
@@ -1076,8 +1078,9 @@ hipError_t launch_k(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk
 template <int W, int VT, int BC, int NT>
 hipError_t launch_stream_nt(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk, const float *x, float *part,
                             hipStream_t s) {
+    if (!wk.order) return hipErrorInvalidValue;
     hipLaunchKernelGGL((stream_spmv_kernel<W, VT, BC, NT>), dim3(wk.nwg), dim3(NT), stream_lds_bytes(W, p.R, BC), s,
-                       p.R, p.nseg, tm.tile_off, wk.wl, wk.tiles, tm.tdata, x, part);
+                       p.R, p.nseg, tm.tile_off, wk.wl, wk.order, wk.tiles, tm.tdata, x, part);
     return hipGetLastError();
 }
 
@@ -1235,6 +1238,16 @@ TileWorkList stream_work_list(const TilePlan &p, const uint64_t *off, uint32_t t
         }
     }
     L.wl_first[p.nblk] = L.nwg;
+    // Launch order: by the column segment each run starts in, then by row
+    // block, so that the workgroups in flight together stage the x segments
+    // of a band of columns, which then stay in L2 (in row-block order every
+    // row block re-reads all of x from HBM).  A run's place in wl, and so its
+    // partial sums and their order in the reduction, do not change.
+    L.order.resize(L.nwg);
+    for (uint32_t r = 0; r < L.nwg; ++r) L.order[r] = r;
+    auto first_seg = [&](uint32_t r) { return L.tiles[4 * (size_t)L.wl[4 * (size_t)r + 1] + 2]; };
+    std::stable_sort(L.order.begin(), L.order.end(),
+                     [&](uint32_t a, uint32_t b) { return first_seg(a) < first_seg(b); });
     return L;
 }
 }  // namespace
