@@ -882,6 +882,9 @@ int mlem_matrix_build_transpose(mlem_matrix *m) { return mlem_matrix_build_trans
 int mlem_matrix_build_transpose_limited(mlem_matrix *m, uint64_t scratch_bytes, uint32_t *ranges) {
     ARGCHK(m, "mlem_matrix_build_transpose: null matrix");
     ARGCHK(!m->tiles_only, "mlem_matrix_build_transpose: a tiles-only matrix has no CSR");
+    ARGCHK(m->users.load() == 0,
+           "mlem_matrix_build_transpose: %u reconstructions or batches still use the matrix's transpose",
+           m->users.load());
     HIPCHK(hipSetDevice(m->device));
     if (ranges) *ranges = 0;
     DevBuf scratch, rp, ci, va;
@@ -932,6 +935,8 @@ int mlem_matrix_build_transpose_limited(mlem_matrix *m, uint64_t scratch_bytes, 
 
 int mlem_matrix_destroy(mlem_matrix *m) {
     if (!m) return MLEM_OK;
+    ARGCHK(m->users.load() == 0, "mlem_matrix_destroy: %u reconstructions or batches still use the matrix",
+           m->users.load());
     (void)hipSetDevice(m->device);
     delete m;
     return MLEM_OK;
@@ -1074,6 +1079,7 @@ int mlem_recon_create(const mlem_matrix *mc, const int32_t *g, const float *sens
     HIPCHK(r->llpart.alloc(sizeof(double) * std::max<uint32_t>(1, loglik_blocks(m->rows))));
     HIPCHK(hipStreamSynchronize(r->stream));
     guard.armed = false;
+    m->users.fetch_add(1);
     *out = r.release();
     return MLEM_OK;
 }
@@ -1201,6 +1207,7 @@ int mlem_recon_destroy(mlem_recon *r) {
     for (auto &x : r->events)
         for (auto &e : x.e) (void)hipEventDestroy(e);
     if (r->stream) (void)hipStreamDestroy(r->stream);
+    r->m->users.fetch_sub(1);
     delete r;
     return MLEM_OK;
 }

@@ -9,6 +9,7 @@
 #include <algorithm>
 #include <cstdlib>
 #include <array>
+#include <atomic>
 #include <chrono>
 #include <cstdarg>
 #include <cstdio>
@@ -136,6 +137,10 @@ struct mlem_matrix {
     using TileKey = std::tuple<int, uint32_t, uint32_t, uint32_t, bool>;
     std::map<TileKey, std::unique_ptr<Tiled>> tiles;
     std::map<TileKey, uint32_t> tile_refused;  // build flags of refused tilings
+    // Reconstructions and batches alive on this matrix: they hold its arrays,
+    // tiles and work lists by address, so the transpose is not rebuilt and
+    // the matrix is not destroyed while any exists.
+    mutable std::atomic<uint32_t> users{0};
 
     uint64_t device_bytes() const {
         uint64_t b = rowptr.bytes + col.bytes + val.bytes + rowptrT.bytes + colT.bytes + valT.bytes + sens32.bytes +

@@ -246,6 +246,7 @@ int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opt
     else if (int st = sens_on_device(m, b->sens.as<float>(), nullptr, b->stream))
         return st;
     HIPCHK(hipStreamSynchronize(b->stream));
+    m->users.fetch_add(1);
     *out = b.release();
     return MLEM_OK;
 }
@@ -356,6 +357,7 @@ void *mlem_batch_stream(mlem_batch *b) { return b ? b->stream : nullptr; }
 int mlem_batch_destroy(mlem_batch *b) {
     if (!b) return MLEM_OK;
     (void)hipSetDevice(b->m->device);
+    b->m->users.fetch_sub(1);
     delete b;
     return MLEM_OK;
 }

@@ -140,7 +140,7 @@ struct TileWorkList {
     uint32_t span = 1, nwg = 0, ntl = 0;
     uint32_t bad = 0;  // quad tiles not aligned as the stream kernel needs
     std::vector<uint32_t> wl, tiles, wl_first;
-    std::vector<uint32_t> order;  // stream lists: run of each workgroup, in launch order
+    std::vector<uint32_t> order;  // run of each workgroup, in launch order (column-segment order)
 };
 // The same on the device.
 struct TileWork {
@@ -149,7 +149,7 @@ struct TileWork {
     const uint4 *wl = nullptr;
     const uint4 *tiles = nullptr;
     const uint32_t *wl_first = nullptr;
-    const uint32_t *order = nullptr;  // stream lists: run of each workgroup
+    const uint32_t *order = nullptr;  // run of each workgroup (null: workgroup w takes run w)
 };
 TilePlan tile_plan(uint32_t rows, uint32_t cols, uint64_t nnz, uint32_t W, uint32_t R, uint32_t vt, bool quad);
 // The 16-bit quantization of a CSR's values as the tiles store it: per-row

@@ -57,6 +57,8 @@
 #include <rocprim/device/device_scan.hpp>
 
 #include <algorithm>
+#include <cstdlib>
+#include <cstring>
 
 #include <hip/hip_fp16.h>
 
@@ -177,10 +179,13 @@ constexpr int pipeline_depth() {
 // first unit in it, units}; tiles[i] = {segment, entries, entry offset lo,
 // hi} of the non-empty tiles; part [runs][R].  (The template keeps BC for
 // the shared layout arithmetic; it is 1.)
+// order[blockIdx.x] = the run this workgroup takes (null: run blockIdx.x);
+// the run's index, not the launch position, places its partial sums.
 template <int W, int VT, int BC, int D>
 __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_kernel(
-    uint32_t R, uint32_t span, uint32_t ntl, const uint4 *__restrict__ wl, const uint4 *__restrict__ tiles,
-    const uint8_t *__restrict__ tdata, const float *__restrict__ x, float *__restrict__ part) {
+    uint32_t R, uint32_t span, uint32_t ntl, const uint4 *__restrict__ wl, const uint32_t *__restrict__ order,
+    const uint4 *__restrict__ tiles, const uint8_t *__restrict__ tdata, const float *__restrict__ x,
+    float *__restrict__ part) {
     constexpr int VB = VT == VAL_F32 ? 4 : 2;  // bytes per stored value
     constexpr int XF = W * BC / kWG;           // floats of an x segment per work-item
     constexpr int CB = col_bits<W>();
@@ -196,7 +201,8 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
     uint4 *tl = reinterpret_cast<uint4 *>(s_cf + kWG * BC);      // [span]
 
     const uint32_t tid = threadIdx.x;
-    const uint4 w = wl[blockIdx.x];
+    const uint32_t run = order ? order[blockIdx.x] : blockIdx.x;
+    const uint4 w = wl[run];
     for (uint32_t i = tid; i < R * BC; i += kWG) acc[i] = 0.0f;
     for (uint32_t i = tid; i < span; i += kWG)
         if (w.y + i < ntl) tl[i] = tiles[w.y + i];
@@ -359,7 +365,7 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
         }
     }
     __syncthreads();
-    float *pw = part + (uint64_t)blockIdx.x * R * BC;
+    float *pw = part + (uint64_t)run * R * BC;
     for (uint32_t i = tid; i < R * BC; i += kWG) pw[i] = acc[i];
 }
 
@@ -1071,7 +1077,7 @@ hipError_t launch_k(const TilePlan &p, const TiledMatrix &tm, const TileWork &wk
                     hipStream_t s) {
     hipLaunchKernelGGL((blocked_spmv_kernel<W, VT, BC, pipeline_depth<W, VT, BC>()>), dim3(wk.nwg), dim3(kWG),
                        lds_bytes(W, p.R, wk.span, BC), s,
-                       p.R, wk.span, wk.ntl, wk.wl, wk.tiles, tm.tdata, x, part);
+                       p.R, wk.span, wk.ntl, wk.wl, wk.order, wk.tiles, tm.tdata, x, part);
     return hipGetLastError();
 }
 
@@ -1303,6 +1309,19 @@ TileWorkList tile_work_list(const TilePlan &p, const uint32_t *nz, const uint64_
         }
     }
     L.wl_first[p.nblk] = L.nwg;
+    // Launch order: by the column segment of each run's first tile, then by
+    // row block, as for the block streams: the workgroups in flight together
+    // stage the x segments of a band of columns, which stay in L2.
+    // MLEM_TILE_ORDER=rows keeps row-block order (for measurements; the
+    // partial sums and the images are the same either way).
+    const char *ord = std::getenv("MLEM_TILE_ORDER");
+    if (!(ord && std::strcmp(ord, "rows") == 0)) {
+        L.order.resize(L.nwg);
+        for (uint32_t r = 0; r < L.nwg; ++r) L.order[r] = r;
+        auto first_seg = [&](uint32_t r) { return L.tiles[4 * (size_t)L.wl[4 * (size_t)r + 1]]; };
+        std::stable_sort(L.order.begin(), L.order.end(),
+                         [&](uint32_t a, uint32_t b) { return first_seg(a) < first_seg(b); });
+    }
     return L;
 }
 

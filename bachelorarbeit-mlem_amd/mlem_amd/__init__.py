@@ -25,9 +25,11 @@ import numpy as np
 PKG_ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
 LIB_DIR = os.path.join(PKG_ROOT, "lib")
 BIN_DIR = os.path.join(PKG_ROOT, "bin")
-HOST_LIB_PATH = os.path.join(LIB_DIR, "libmlem_host.so")
+# MLEM_HOST_LIB / MLEM_CLI select other builds of the host library and the
+# command line (the sanitizer build of `make asan`).
+HOST_LIB_PATH = os.environ.get("MLEM_HOST_LIB") or os.path.join(LIB_DIR, "libmlem_host.so")
 HIP_LIB_PATH = os.path.join(LIB_DIR, "libmlem_hip.so")
-CLI_PATH = os.path.join(BIN_DIR, "mlem")
+CLI_PATH = os.environ.get("MLEM_CLI") or os.path.join(BIN_DIR, "mlem")
 
 MLEM_OK = 0
 STATUS_NAMES = {
@@ -89,8 +91,12 @@ def geometry(n_rings, n_per_ring, radius, crystal, ring_pitch, grid, voxel, sub_
 # Scanner presets.  RING2D_288: 288 point crystals on a 40 mm ring, 64x64
 # voxels of 1 mm, exact Siddon lengths (41,328 LOR rows).  RING3D_1152: two
 # axial rings of 576 crystals (1 x 10 x 10 mm, inner radius 35.5 mm, ring
-# pitch 10 mm), 140x140x40 voxels of 0.5 mm, 16 sub-rays per LOR (662,976
-# rows x 784,000 columns).  RING3D_1152_FINE: the same scanner, 280x280x80
+# pitch 10 mm), 140x140x40 voxels of 0.5 mm, 22 sub-rays per LOR (662,976
+# rows x 784,000 columns, 1,527,217,536 nonzeros: 12.2 GB as f32 + u32; a
+# sub-ray crosses 124 voxels on average and the sub-rays of one LOR share
+# about 10% of them, so the count grows with the sub-rays: 16 give 1.15e9,
+# and crystal size, radius or voxel size move it by a few percent only).
+# RING3D_1152_FINE: the same scanner, 280x280x80
 # voxels of 0.25 mm and 128 sub-rays.  RING3D_576_MID: two rings of 288
 # (2 x 9 x 10 mm, radius 35.5 mm), 96x96x24 voxels of 0.75 mm, 16 sub-rays
 # (165,600 rows x 221,184 columns).
@@ -99,7 +105,7 @@ def RING2D_288() -> Geometry:
 
 
 def RING3D_1152() -> Geometry:
-    return geometry(2, 576, 35.5, (1.0, 10.0, 10.0), 10.0, (140, 140, 40), (0.5, 0.5, 0.5), 16)
+    return geometry(2, 576, 35.5, (1.0, 10.0, 10.0), 10.0, (140, 140, 40), (0.5, 0.5, 0.5), 22)
 
 
 def RING3D_1152_FINE() -> Geometry:

@@ -10,7 +10,8 @@ from typing import Optional
 
 import numpy as np
 
-from . import (HIP_LIB_PATH, MLEM_OK, Csr, MlemError, Opts, _load, _ptr, cint, f32, f64, i32, sz, u32, u64, vp)
+from . import (HIP_LIB_PATH, MLEM_OK, Csr, MlemError, Opts, _load, _ptr, cint, f32, f64, host_lib, i32, sz, u32,
+               u64, vp)
 
 __all__ = ["hip_lib", "device_count", "device_info", "DeviceArray", "Stream", "Event", "ResidentMatrix", "Recon",
            "Batch", "BatchOpts", "Server",
@@ -26,6 +27,7 @@ _hip = None
 def hip_lib() -> C.CDLL:
     global _hip
     if _hip is None:
+        host_lib()  # first, so that libmlem_hip.so binds to the host library selected (by its soname)
         lib = _load(HIP_LIB_PATH, "libmlem_hip.so")
         P = C.POINTER
         sigs = {
@@ -571,8 +573,9 @@ class ResidentMatrix:
         return s32, s64
 
     def close(self):
+        """Frees the matrix; MlemError (and the matrix kept) while a Recon or Batch of it is open."""
         if self.h:
-            hip_lib().mlem_matrix_destroy(self.h)
+            _check(hip_lib().mlem_matrix_destroy(self.h))
             self.h = vp()
 
     def __enter__(self):

@@ -4,30 +4,39 @@
 A step is one MLEM iteration (forward projection + ratio, back projection +
 update) over the resident system matrix.  Default workload: the MADPET-II
 scale 3D scanner (two rings of 576 crystals, 662,976 LOR rows x 784,000
-voxels of 140x140x40, 16 jittered sub-rays per LOR), a seeded ellipsoid
-phantom and Poisson counts totalling 1e8, generated on the host from a seed
-(no data files exist for it).
+voxels of 140x140x40, 22 jittered sub-rays per LOR: 1.53e9 nonzeros, 12.2 GB
+as f32 + u32), a seeded ellipsoid phantom and Poisson counts totalling 1e8,
+generated on the host from a seed (no data files exist for it).
 
 Prints one JSON line: iterations/s over the timed steps, the dominant
-kernel's achieved HBM bandwidth against the 8 TB/s peak (HIP events on the
-reconstruction's stream), and the FP32 OpenMP CPU loop (the oracle's CPU
-baseline) timed on a bounded number of iterations of the same workload.
+kernel's achieved bandwidth against the 8 TB/s HBM peak (HIP events on the
+reconstruction's stream), the FP32 OpenMP CPU loop timed on the host's
+physical cores (a bounded sample: 3 runs of 1 warm-up + k iterations, about
+25 s in all, median), and a witness: the GPU image after 100 iterations
+against the FP64 OpenMP oracle.
 
 Algorithmic bytes (f32 values, u32 indices, u64 row pointers):
   FP SpMV kernel  8*nnz + 8*(rows+1) + 4*cols      (A and x)
   BP SpMV kernel  8*nnz + 8*(cols+1) + 4*rows      (A^T and the ratio)
   iteration       16*nnz + 20*rows + 24*cols + 16  (both, with g, sens, f r/w)
-The blocked kernels read 6 bytes per stored entry (plus padding and the x
-segments), so their algorithmic rate can exceed the HBM rate they draw;
-`roofline.traffic` holds the measured HBM bytes per launch (rocprofv3
-FETCH_SIZE + WRITE_SIZE, scaled by the same counters on a copy kernel)
-from profiles/traffic.json when it was measured for this workload.
+`roofline.achieved` and `algorithmic_gbs` count these bytes.  The blocked
+kernels read 6 bytes per stored entry (plus padding and the x segments), so
+their algorithmic rate exceeds the HBM rate they draw: `roofline.traffic`
+holds the measured HBM bytes per launch (rocprofv3 FETCH_SIZE + WRITE_SIZE,
+scaled by the same counters on a copy kernel) from profiles/traffic.json when
+it was measured for this workload, and `roofline.drawn_gbs` / `drawn_frac`
+the rate and fraction of peak those bytes make.
 """
 import argparse
 import json
 import os
 import sys
 import time
+
+# The CPU baseline runs one OpenMP thread per physical core, bound close (set
+# before any library that links the OpenMP runtime is loaded).
+os.environ.setdefault("OMP_PROC_BIND", "close")
+os.environ.setdefault("OMP_PLACES", "cores")
 
 ROOT = os.path.dirname(os.path.abspath(__file__))
 for _p in (ROOT, os.path.join(ROOT, "bachelorarbeit-mlem_amd")):
@@ -53,6 +62,48 @@ def cpu_model():
     except OSError:
         pass
     return "unknown"
+
+
+def host_cpus():
+    """(logical CPUs this process may run on, physical cores among them) from sched_getaffinity and the sysfs
+    topology."""
+    cpus = sorted(os.sched_getaffinity(0))
+    cores = set()
+    for c in cpus:
+        try:
+            base = f"/sys/devices/system/cpu/cpu{c}/topology/"
+            with open(base + "physical_package_id") as f1, open(base + "core_id") as f2:
+                cores.add((f1.read().strip(), f2.read().strip()))
+        except OSError:
+            cores.add(("cpu", str(c)))
+    return len(cpus), len(cores)
+
+
+def baseline_threads():
+    """Threads of the CPU baseline: the physical cores this process may run on, at most OMP_NUM_THREADS (the
+    host's CPU share) when that is set."""
+    _, phys = host_cpus()
+    cap = os.environ.get("OMP_NUM_THREADS", "")
+    return max(1, min(phys, int(cap))) if cap.isdigit() and int(cap) > 0 else phys
+
+
+def cpu_baseline_f32(O, A, AT, g, s32, f0, threads, budget_s=25.0, runs=3):
+    """The FP32 OpenMP CPU loop (oracle_mlem_f32_omp) on the same arrays, on a bounded sample: one timed
+    iteration sizes it, then `runs` runs of 1 warm-up iteration + k iterations (the three together about
+    budget_s seconds).  Returns the median iterations/s, k, each run's seconds and the host description."""
+    _, secs1 = O.mlem_f32_omp(A, AT, g, s32, f0, 1, threads)
+    k = max(1, int(budget_s / runs / max(secs1, 1e-6)))
+    rates, secs_all = [], []
+    for r in range(runs):
+        O.mlem_f32_omp(A, AT, g, s32, f0, 1, threads)
+        _, secs = O.mlem_f32_omp(A, AT, g, s32, f0, k, threads)
+        rates.append(k / secs)
+        secs_all.append(round(secs, 3))
+        log(f"CPU baseline run {r + 1}/{runs}: {k} iterations in {secs:.1f} s ({k / secs:.2f} it/s)")
+    nproc, phys = host_cpus()
+    return float(np.median(rates)), k, secs_all, {
+        "nproc": nproc, "physical_cores": phys, "cpu_model": cpu_model(),
+        "omp": {v: os.environ.get(v) for v in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}}
 
 
 def measured_traffic(workload_name, side, tile_width, tile_rows):
@@ -112,7 +163,7 @@ def batch_sinograms(M, geo, A, B, threads):
     return sinos
 
 
-def run_batch(args, M, geo, A, g0, m, s32, threads, label, sinos=None, single=True):
+def run_batch(args, M, geo, A, g0, m, s32, s64, threads, label, sinos=None, single=True):
     """Batched reconstructions: B seeded sinograms iterated together over one matrix; compared with the
     one-at-a-time FP32 path (the single Recon) timed in the same process (single=False: a matrix resident in
     the batches' layout only has no one-at-a-time path to time)."""
@@ -154,27 +205,41 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label, sinos=None, single=Tr
         b.sync()
         M.device_sync()
         elapsed = time.perf_counter() - t0
-    # Accuracy: the first few sinograms reconstructed in full (recon_iters iterations) by the batch and by the
-    # one-at-a-time FP32 path (itself within 1e-5 rel-L2 of the FP64 oracle), compared per sinogram.
-    errs = []
-    K = min(B, args.batch_check) if single else 0
+    # cpu_baseline leg: the FP64 OpenMP oracle reconstructs the first K sinograms on the host's physical cores
+    # (timed: the CPU's recon/s), and, as the checker, its images are what every value type's batched images
+    # are compared with, sinogram by sinogram.
+    errs, cpu = {}, None
+    K = min(B, args.batch_check) if (single and not args.no_cpu) else 0
     if K:
-        with M.Batch(m, K, values=VALUES[args.values], per_pass=args.per_pass,
-                     tile_width=args.tile if args.tile > 1 else 0, tile_rows=args.tile_rows,
-                     tile_groups=args.tile_groups, tile_threads=args.tile_threads) as b2:
-            for k in range(K):
-                b2.set(k, sinos[k], f0)
-            b2.iterate(args.recon_iters)
-            b2.sync()
-            imgs = [b2.image(k) for k in range(K)]
-        for k in range(K):
-            with M.Recon(m, sinos[k], f0, sens=s32, options=M.opts(spmv=0, using_trans=0)) as r:
-                r.iterate(args.recon_iters)
-                r.sync()
-                ref = r.image().astype(np.float64)
-            errs.append(float(np.linalg.norm(imgs[k] - ref) / np.linalg.norm(ref)))
-        log(f"rel-L2 of the batched images vs the one-at-a-time FP32 images after {args.recon_iters} iterations: "
-            + ", ".join(f"{e:.2e}" for e in errs))
+        import oracle as O
+        AT = m.download_transpose()
+        ct = baseline_threads()
+        t = time.time()
+        G, ref = np.stack(sinos[:K]), np.stack([f0] * K)
+        for done in range(0, args.recon_iters, 10):  # chunks: the state between iterations is the images
+            ref = O.mlem_f64_omp(A, AT, G, s64, ref, min(10, args.recon_iters - done), ct)
+            log(f"FP64 oracle {min(done + 10, args.recon_iters)}/{args.recon_iters} iterations, {time.time() - t:.1f} s")
+        secs = time.time() - t
+        del AT
+        nproc, phys = host_cpus()
+        cpu = {"value": round(K / secs, 4), "unit": "recon/s", "cores": ct, "kind": "port",
+               "sample": f"{K} sinograms x {args.recon_iters} FP64 MLEM iterations in one OpenMP pass per iteration "
+                         f"(oracle_mlem_f64_omp, {secs:.1f} s) on {ct} threads, one per physical core of "
+                         f"{cpu_model()}", "nproc": nproc, "physical_cores": phys}
+        log(f"FP64 oracle: {K} sinograms x {args.recon_iters} iterations in {secs:.1f} s on {ct} threads")
+        for vname in ("uf16", "fp16", "bf16", "f32"):
+            with M.Batch(m, K, values=VALUES[vname], per_pass=args.per_pass,
+                         tile_width=args.tile if args.tile > 1 else 0, tile_rows=args.tile_rows,
+                         tile_groups=args.tile_groups, tile_threads=args.tile_threads) as b2:
+                for k in range(K):
+                    b2.set(k, sinos[k], f0)
+                b2.iterate(args.recon_iters)
+                b2.sync()
+                e = [float(np.linalg.norm(b2.image(k) - ref[k]) / np.linalg.norm(ref[k])) for k in range(K)]
+            errs[vname] = {"per_sinogram": [round(v, 8) for v in e], "max": round(max(e), 8),
+                           "meets_1e-3": bool(max(e) <= 1e-3)}
+            log(f"{vname} values, {args.per_pass} per pass: rel-L2 vs FP64 oracle after {args.recon_iters} "
+                f"iterations " + ", ".join(f"{v:.2e}" for v in e))
     t_iter = elapsed / args.steps
     recon_s = B / (args.recon_iters * t_iter)
     single_recon_s = 1.0 / (args.recon_iters * t_single) if t_single else None
@@ -199,8 +264,10 @@ def run_batch(args, M, geo, A, g0, m, s32, threads, label, sinos=None, single=Tr
              "speedup": round(t_plain / t_iter * B, 3)} if t_plain else None),
     }
     if errs:
-        out["rel_l2_vs_one_at_a_time_f32"] = {"iterations": args.recon_iters, "per_sinogram": [round(e, 8) for e in errs],
-                                              "max": round(max(errs), 8)}
+        out["rel_l2_vs_fp64_oracle"] = {"iterations": args.recon_iters, "sinograms": K,
+                                        "counts": [int(np.sum(sinos[k], dtype=np.int64)) for k in range(K)],
+                                        "per_pass": args.per_pass, "by_values": errs}
+    out["cpu_baseline"] = cpu
     print(json.dumps(out), flush=True)
     m.close()
 
@@ -374,7 +441,6 @@ def main():
     ap.add_argument("--autotune", action="store_true",
                     help="time the candidate tile shapes and workgroup counts first (mlem_autotune) and use the best")
     ap.add_argument("--threads", type=int, default=0, help="host threads (0: all)")
-    ap.add_argument("--cpu-seconds", type=float, default=15.0, help="budget of the CPU baseline sample")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--batch", type=int, default=0,
                     help="batched mode: B sinograms (seeded phantoms, counts log-uniform 1e6-1e9) over the matrix")
@@ -382,8 +448,9 @@ def main():
     ap.add_argument("--values", default=None, choices=["f32", "bf16", "fp16", "uf16"],
                     help="stored matrix values (default: f32 for one reconstruction, uf16 for --batch and --serve)")
     ap.add_argument("--recon-iters", type=int, default=100, help="iterations per reconstruction (recon/s)")
-    ap.add_argument("--batch-check", type=int, default=4,
-                    help="batched mode: sinograms whose full reconstructions are compared with the FP32 path")
+    ap.add_argument("--batch-check", type=int, default=8,
+                    help="batched mode: sinograms reconstructed by the FP64 oracle (CPU baseline and checker) and "
+                         "compared, for every value type, with their batched images")
     ap.add_argument("--serve", default="",
                     help="serving mode over these workloads (comma list of c1, mid, c2): continuous batching")
     ap.add_argument("--serve-requests", type=int, default=48, help="serving mode: number of requests")
@@ -446,7 +513,7 @@ def main():
         log(f"device transpose (stable counting sort{f', {ranges} column ranges' if ranges else ''}) "
             f"{time.time() - t:.2f} s")
     t = time.time()
-    s32, _ = m.sensitivity()
+    s32, s64 = m.sensitivity()
     log(f"FP64 sensitivity {time.time() - t:.2f} s")
     f0, init = M.initial_image(g, A.rows, s32)
     if args.autotune and args.tile != 1 and not tiles_only:
@@ -460,7 +527,7 @@ def main():
             f"(FP {tune['fp_ms']:.3f} ms, BP {tune['bp_ms']:.3f} ms)")
 
     if args.batch:
-        return run_batch(args, M, geo, A, g, m, s32, threads, label, sinos=batch_sinos, single=not tiles_only)
+        return run_batch(args, M, geo, A, g, m, s32, s64, threads, label, sinos=batch_sinos, single=not tiles_only)
 
     # FP and BP kernels chosen independently: spmv selects FP, bp_* selects BP.
     fp_merge = args.fp == "merge"
@@ -539,6 +606,8 @@ def main():
         if tr:
             roofline["traffic"] = tr["bytes"]
             roofline["traffic_source"] = tr["source"]
+            roofline["drawn_gbs"] = round(gbs(tr["bytes"], ms), 1)
+            roofline["drawn_frac"] = round(gbs(tr["bytes"], ms) / HBM_PEAK_GBS, 4)
             log(f"measured HBM traffic per {side.upper()} launch: {tr['bytes'] / 1e9:.2f} GB "
                 f"({tr['bytes'] / (ms * 1e-3) / 1e9:.0f} GB/s drawn)")
 
@@ -583,18 +652,41 @@ def main():
             os.remove(path)
             os.rmdir(d)
 
-    cpu = None
+    cpu = witness = None
     if not args.no_cpu:
+        # cpu_baseline leg: the FP32 OpenMP CPU loop timed on the host's physical cores (1 warm-up + the
+        # workload's iteration count, median of 3), and, as the checker, the FP64 OpenMP oracle over the same
+        # iterations against a GPU reconstruction of the timed configuration.
         import oracle as O
         AT = m.download_transpose()
-        n_it = 1
-        _, secs = O.mlem_f32_omp(A, AT, g, s32, f0, 1, threads)  # warm-up + size the sample
-        n_it = max(1, int(args.cpu_seconds / max(secs, 1e-6)))
-        _, secs = O.mlem_f32_omp(A, AT, g, s32, f0, n_it, threads)
-        cpu = {"value": round(n_it / secs, 4), "unit": "iters/s", "cores": threads, "kind": "port",
-               "sample": f"{n_it} FP32 MLEM iterations of the same workload on {threads} threads of {cpu_model()} "
-                         f"({secs:.1f} s), OpenMP row-parallel FP on A and BP on A^T (oracle_mlem_f32_omp)"}
-        log(f"CPU baseline {cpu['value']} it/s ({n_it} iterations, {secs:.1f} s)")
+        ct = baseline_threads()
+        rate, k, run_s, host = cpu_baseline_f32(O, A, AT, g, s32, f0, ct)
+        cpu = {"value": round(rate, 4), "unit": "iters/s", "cores": ct, "kind": "port",
+               "sample": f"FP32 MLEM of the same workload, 1 warm-up + {k} iterations, median of {len(run_s)} runs "
+                         f"({', '.join(f'{v:.1f}' for v in run_s)} s) on {ct} threads, one per physical core "
+                         f"({host['cpu_model']}); OpenMP row-parallel FP on A and BP on A^T (oracle_mlem_f32_omp)",
+               **host}
+        log(f"CPU baseline {cpu['value']} it/s on {ct} threads ({host['physical_cores']} physical cores of "
+            f"{host['nproc']} CPUs; runs {run_s} s)")
+        if args.values == "f32":
+            o_w = M.opts(spmv=0 if fp_merge else 1, using_trans=0, secsize_fp=args.secsize, secsize_bp=args.secsize,
+                         use_graph=args.graph, tile_width=args.tile, tile_rows=args.tile_rows,
+                         tile_groups=args.tile_groups)
+            o_w.bp_spmv = o.bp_spmv
+            img_w = M.mlem_run(m, g, f0, args.recon_iters, sens=s32, options=o_w)
+            t = time.time()
+            ref = f0
+            for done in range(0, args.recon_iters, 10):  # chunks: the state between iterations is the image
+                ref = O.mlem_f64_omp(A, AT, g, s64, ref, min(10, args.recon_iters - done), ct)
+                log(f"witness: FP64 oracle {min(done + 10, args.recon_iters)}/{args.recon_iters} iterations, "
+                    f"{time.time() - t:.1f} s")
+            err = float(np.linalg.norm(img_w - ref) / np.linalg.norm(ref))
+            witness = {"iterations": args.recon_iters, "rel_l2_vs_fp64_oracle": err, "bar": 1e-5,
+                       "oracle": "oracle_mlem_f64_omp (FP64, FP64 sensitivity; bitwise the serial oracle)",
+                       "oracle_s": round(time.time() - t, 1)}
+            log(f"witness: the GPU image after {args.recon_iters} iterations is {err:.2e} rel-L2 from the FP64 "
+                f"oracle ({witness['oracle_s']} s)")
+        del AT
     m.close()
 
     out = {
@@ -602,7 +694,7 @@ def main():
         "warmup": args.warmup, "ms_per_step": round(elapsed * 1e3 / args.steps, 4), "higher_is_better": True,
         "scaling": "weak", "vs_baseline": None,
         "dtype": "f32" if args.values == "f32" else f"{args.values} values, f32 arithmetic", "data": "synthetic",
-        "hbm_gbs": round(b_iter * its / 1e9, 1),
+        "algorithmic_gbs": round(b_iter * its / 1e9, 1),
         "config": {"workload": label, "rows": rows, "cols": cols, "nnz": nnz,
                    "matrix_gb": round(8 * nnz / 1e9, 2), "bytes_per_iteration": b_iter,
                    "fp": args.fp + (" blocked" if args.fp == "merge" and blocked else ""),
@@ -613,6 +705,8 @@ def main():
                                                     else "matrix and transpose resident")},
         "roofline": roofline, "cpu_baseline": cpu,
     }
+    if witness:
+        out["witness"] = witness
     if rel_vs_f32 is not None:
         out["rel_l2_vs_f32_values"] = {"iterations": args.warmup + args.steps, "value": rel_vs_f32}
     if load:

@@ -235,7 +235,9 @@ int mlem_matrix_create(int device, uint32_t rows, uint32_t cols, uint64_t nnz,
  * sptrans_scanTrans_specialized (sptrans.h:397-545) as called at
  * main.cu:99-105.  The one sort needs 20 bytes of scratch per nonzero; when
  * that is not free (the 100 GB matrices) the sort runs over column ranges
- * within the free memory, with the same result. */
+ * within the free memory, with the same result.  Refused
+ * (MLEM_ERR_INVALID_ARGUMENT) while a reconstruction or batch of the matrix
+ * exists: they hold the transpose's arrays, tiles and captured graphs. */
 int mlem_matrix_build_transpose(mlem_matrix *m);
 /* The same within scratch_bytes of scratch (0 = what is free); *ranges (may
  * be NULL) = column ranges used, 0 for the single sort.  MLEM_ERR_OUT_OF_MEMORY
@@ -283,6 +285,8 @@ typedef struct mlem_load_stats {
 } mlem_load_stats;
 int mlem_matrix_load_csr4(int device, const char *path, int threads,
                           mlem_matrix **out, mlem_load_stats *stats);
+/* Refused (MLEM_ERR_INVALID_ARGUMENT, nothing freed) while a reconstruction
+ * or batch of the matrix exists: destroy those first. */
 int mlem_matrix_destroy(mlem_matrix *m);
 int mlem_matrix_info(const mlem_matrix *m, uint32_t *rows, uint32_t *cols,
                      uint64_t *nnz, int *has_transpose, uint64_t *device_bytes);

@@ -49,6 +49,7 @@ def lib() -> C.CDLL:
             "oracle_mlem_f64": (None, [vp, vp, vp, u32, u32, vp, vp, vp, u32]),
             "oracle_loglik_f64": (f64, [vp, vp, vp, u32, vp, vp]),
             "oracle_mlem_f32_omp": (f64, [vp, vp, vp, vp, vp, vp, u32, u32, vp, vp, vp, u32, cint]),
+            "oracle_mlem_f64_omp": (cint, [vp, vp, vp, vp, vp, vp, u32, u32, u32, vp, vp, vp, u32, cint]),
             "oracle_max_threads": (cint, []),
         }
         for n, (r, a) in sig.items():
@@ -173,6 +174,28 @@ def mlem_f64(rowptr, col, val, rows, cols, g, sens, f0, iters):
     lib().oracle_mlem_f64(_p(_c(rowptr, np.uint64)), _p(_c(col, np.uint32)), _p(_c(val, np.float32)), rows, cols,
                           _p(_c(g, np.int32)), _p(_c(sens, np.float64)), _p(f), iters)
     return f
+
+
+def mlem_f64_omp(A, AT, g, sens, f0, iters, threads=0):
+    """oracle_mlem_f64 for one image (g 1-D, f0 1-D) or B images (g [B, rows], f0 [B, cols]) with OpenMP, FP over
+    rows of A and BP over rows of AT (the stable transpose of A); bitwise equal to mlem_f64 image by image.
+    Returns f64 images shaped like f0."""
+    g = np.asarray(g)
+    f0 = np.asarray(f0)
+    single = g.ndim == 1
+    G = _c(g.reshape(1, -1) if single else g, np.int32)
+    F0 = f0.reshape(1, -1) if single else f0
+    B = G.shape[0]
+    if F0.shape != (B, A.cols) or G.shape[1] != A.rows:
+        raise ValueError("mlem_f64_omp: counts or images of the wrong shape")
+    Gi = np.ascontiguousarray(G.T)  # [rows][B]
+    F = np.ascontiguousarray(F0.astype(np.float64).T)  # [cols][B]
+    st = lib().oracle_mlem_f64_omp(_p(A.rowptr), _p(A.col), _p(A.val), _p(AT.rowptr), _p(AT.col), _p(AT.val),
+                                   A.rows, A.cols, B, _p(Gi), _p(_c(sens, np.float64)), _p(F), iters, threads)
+    if st != 0:
+        raise ValueError(f"oracle_mlem_f64_omp failed ({st})")
+    out = np.ascontiguousarray(F.T)
+    return out[0] if single else out
 
 
 def loglik_f64(rowptr, col, val, rows, g, f):

@@ -18,6 +18,7 @@
  *   oracle_merge_spmv_f64     kernel.cu:190-286         merge-based SpMV walk
  *   oracle_spmv_f64           kernel.cu:297-336         CSR SpMV (row sums)
  *   oracle_mlem_f64           main.cu:329-449, kernel.cu:11-39, 347-370
+ *   oracle_mlem_f64_omp       the same, B images, OpenMP, bitwise equal to it
  *   oracle_loglik_f64         Poisson log-likelihood (the MLEM objective)
  *   oracle_mlem_f32_omp       main.cu:329-449 as an FP32 OpenMP CPU loop
  */
@@ -300,6 +301,52 @@ void oracle_mlem_f64(const uint64_t *rowptr, const uint32_t *col, const float *v
     }
     free(fp);
     free(bp);
+}
+
+/* oracle_mlem_f64 for B images at once, for matrices of 1e9+ entries: FP
+ * parallel over rows of A, BP parallel over rows of A^T (the stable
+ * transpose, so row c of A^T lists column c's entries in A's row order).
+ * Every sum is formed by one thread in the serial oracle's order -- FP in
+ * k order along the row, BP in increasing row of A, which is the order in
+ * which oracle_mlem_f64's scatter adds into bp[c] -- so each image is
+ * bitwise the one oracle_mlem_f64 gives for it, whatever the thread count.
+ * Vectors are interleaved, image b of voxel c at f[c * B + b], counts of
+ * image b of row r at g[r * B + b]; B <= 64. */
+int oracle_mlem_f64_omp(const uint64_t *rowptr, const uint32_t *col, const float *val, const uint64_t *rowptrT,
+                        const uint32_t *colT, const float *valT, uint32_t rows, uint32_t cols, uint32_t B,
+                        const int32_t *g, const double *sens, double *f, uint32_t iters, int threads) {
+    if (B == 0 || B > 64) return -1;
+    double *ratio = (double *)malloc(((size_t)rows * B + 1) * sizeof(double));
+    if (!ratio) return -2;
+    if (threads > 0) omp_set_num_threads(threads);
+    for (uint32_t it = 0; it < iters; ++it) {
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t r = 0; r < (int64_t)rows; ++r) {
+            double s[64];
+            for (uint32_t b = 0; b < B; ++b) s[b] = 0.0;
+            for (uint64_t k = rowptr[r]; k < rowptr[r + 1]; ++k) {
+                const double v = (double)val[k];
+                const double *fx = f + (size_t)col[k] * B;
+                for (uint32_t b = 0; b < B; ++b) s[b] += v * fx[b];
+            }
+            for (uint32_t b = 0; b < B; ++b)
+                ratio[(size_t)r * B + b] = (s[b] != 0.0) ? (double)g[(size_t)r * B + b] / s[b] : s[b];
+        }
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t c = 0; c < (int64_t)cols; ++c) {
+            double s[64];
+            for (uint32_t b = 0; b < B; ++b) s[b] = 0.0;
+            for (uint64_t k = rowptrT[c]; k < rowptrT[c + 1]; ++k) {
+                const double v = (double)valT[k];
+                const double *rx = ratio + (size_t)colT[k] * B;
+                for (uint32_t b = 0; b < B; ++b) s[b] += v * rx[b];
+            }
+            double *fc = f + (size_t)c * B;
+            for (uint32_t b = 0; b < B; ++b) fc[b] = (sens[c] == 0.0) ? fc[b] * s[b] : fc[b] * s[b] / sens[c];
+        }
+    }
+    free(ratio);
+    return 0;
 }
 
 /* Poisson log-likelihood sum_i g_i log(y_i) - y_i, y = A f (terms with

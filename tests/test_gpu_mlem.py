@@ -250,3 +250,23 @@ def test_tiling_refused_and_automatic_fallback():
         ref = O.mlem_f64(D.rowptr, D.col, D.val, D.rows, D.cols, g, s32.astype(np.float64), f0, 3)
         img = M.mlem_run(m, g, f0, 3, sens=s32, options=M.opts(spmv=0, tile_width=2048, tile_rows=1024))
         assert rel_l2(img, ref) <= REL_L2
+
+
+def test_matrix_refuses_transpose_rebuild_and_destroy_while_in_use(c1):
+    """A reconstruction or batch holds the matrix's transpose, tiles and graphs by address: rebuilding the
+    transpose or destroying the matrix is refused while one is open, and allowed again once they are closed."""
+    m = M.ResidentMatrix(c1["A"])
+    m.build_transpose()
+    f0 = np.ones(c1["A"].cols, np.float32)
+    r = M.Recon(m, c1["g"], f0, options=M.opts(spmv=0, using_trans=0))
+    b = M.Batch(m, 2, per_pass=4)
+    for closer in (r, b):
+        with pytest.raises(M.MlemError, match="still use"):
+            m.build_transpose()
+        with pytest.raises(M.MlemError, match="still use"):
+            m.close()
+        assert m.h
+        closer.close()
+    m.build_transpose()
+    m.close()
+    assert not m.h

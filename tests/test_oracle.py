@@ -313,6 +313,35 @@ def test_mlem_f32_cpu_baseline_matches_fp64(c1):
         assert secs > 0
 
 
+@pytest.mark.parametrize("threads", [1, 3, 8])
+def test_mlem_f64_omp_is_bitwise_the_serial_oracle(c1, threads):
+    """The OpenMP FP64 oracle (FP over rows of A, BP over rows of the stable transpose) forms every sum in the
+    serial oracle's order, so each image is bitwise the serial one, for one image and for B of them."""
+    A, AT, g = c1["A"], c1["AT"], c1["g"]
+    sens = O.colsum_f64(A.rowptr, A.col, A.val, A.rows, A.cols)
+    f0 = np.ones(A.cols)
+    ref = mlem(A, g, sens, f0, 6)
+    one = O.mlem_f64_omp(A, AT, g, sens, f0, 6, threads)
+    assert same_bits(one, ref)
+    rng = np.random.default_rng(threads)
+    G = np.stack([g, rng.integers(0, 30, A.rows).astype(np.int32), np.roll(g, 7)])
+    F0 = np.stack([f0, rng.random(A.cols) + 0.5, np.full(A.cols, 3.0)])
+    many = O.mlem_f64_omp(A, AT, G, sens, F0, 6, threads)
+    for b in range(3):
+        assert same_bits(many[b], mlem(A, G[b], sens, F0[b], 6)), b
+
+
+@pytest.mark.parametrize("kind", ["empty_rows", "dups", "heavy", "all_empty", "one_huge_row"])
+def test_mlem_f64_omp_edge_matrices(kind):
+    A = make(kind, seed=21)
+    AT = M.transpose(A)
+    rng = np.random.default_rng(21)
+    g = rng.integers(0, 40, A.rows).astype(np.int32)
+    sens = O.colsum_f64(A.rowptr, A.col, A.val, A.rows, A.cols)
+    f0 = rng.random(A.cols) + 0.25
+    assert same_bits(O.mlem_f64_omp(A, AT, g, sens, f0, 3, 4), mlem(A, g, sens, f0, 3))
+
+
 def test_planted_mistakes_are_caught():
     """Each check above must fail for a plausible slip: here the slips are
     planted in numpy restatements of one iteration (not in the oracle) and
