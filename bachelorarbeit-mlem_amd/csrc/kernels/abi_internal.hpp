@@ -156,14 +156,23 @@ struct IterEvents {
     hipEvent_t e[5];
 };
 
+// One pass of a batch: the counts, ratios and images of its BC slots,
+// [rows_x][BC] / [cols_x][BC] (right-hand side fastest).  Allocated when a
+// slot of the pass is first set, so a batch holds device memory for the
+// passes it has used, not for all its slots.
+struct BatchPass {
+    mlem::abi::DevBuf g, ratio, f;
+};
+
 struct mlem_batch {
     mlem_matrix *m = nullptr;
     uint32_t slots = 0, BC = 4, passes = 0;
     uint32_t nt = 0;                // work-items per workgroup of the stream kernel (0: its default)
     size_t rows_x = 0, cols_x = 0;  // padded vector lengths
     hipStream_t stream = nullptr;
-    // [passes][rows_x][BC] counts and ratios, [passes][cols_x][BC] images
-    mlem::abi::DevBuf g, ratio, f, sens, active, stage, part_fp, part_bp;
+    std::vector<std::unique_ptr<BatchPass>> pass;  // [passes], null until a slot of the pass is set
+    // sensitivity, [slots] active flags, a rows/cols-long staging vector, partial sums of FP and BP
+    mlem::abi::DevBuf sens, active, stage, part_fp, part_bp;
     const Tiled *tfp = nullptr, *tbp = nullptr;
     const TileWorkDev *wfp = nullptr, *wbp = nullptr;
     std::vector<uint32_t> active_host;
@@ -172,6 +181,17 @@ struct mlem_batch {
     bool use_graph = false;
     hipGraphExec_t gexec = nullptr;
     std::vector<uint8_t> graph_passes;
+    // Device bytes of one pass, and of everything else the batch holds.
+    uint64_t pass_bytes() const { return (uint64_t)BC * 4u * (2u * rows_x + cols_x); }
+    uint64_t fixed_bytes() const {
+        return sens.bytes + active.bytes + stage.bytes + part_fp.bytes + part_bp.bytes;
+    }
+    uint64_t device_bytes() const {
+        uint64_t b = fixed_bytes();
+        for (const auto &q : pass)
+            if (q) b += q->g.bytes + q->ratio.bytes + q->f.bytes;
+        return b;
+    }
     ~mlem_batch() {
         if (stream) (void)hipStreamSynchronize(stream);  // no replay still running
         if (gexec) (void)hipGraphExecDestroy(gexec);
@@ -224,6 +244,17 @@ int sens_on_device(const mlem_matrix *m, float *s32, double *s64_or_null, hipStr
 // sums (4 bytes per row of its block and right-hand side) would then cost
 // about as much as its entries -- else 2048 (one) or 1024 (several).
 uint32_t default_tile_rows(uint64_t nnz, uint32_t groups, uint32_t BC);
+// Batch slots without waiting on the host (the serving loop): the set
+// copies g (rows) and f0 (cols) in and marks the slot active, allocating its
+// pass if that is the first slot set in it (*allocated = bytes allocated,
+// may be null); the image fetch copies the slot's image out; deactivation
+// clears the flag.  All are queued on the batch's stream: host arrays must
+// stay valid, and the image is not there, until the stream has passed them.
+int batch_set_async(mlem_batch *b, uint32_t slot, const int32_t *g, const float *f0, uint64_t *allocated);
+int batch_get_image_async(mlem_batch *b, uint32_t slot, float *f_out);
+int batch_deactivate_async(mlem_batch *b, uint32_t slot);
+// Frees pass p when none of its slots is active (returns the bytes freed).
+uint64_t batch_release_pass(mlem_batch *b, uint32_t p);
 // Progress of long builds on stderr when MLEM_VERBOSE is set.
 void progress(const char *fmt, ...) __attribute__((format(printf, 1, 2)));
 

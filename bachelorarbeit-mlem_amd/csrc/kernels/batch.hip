@@ -12,12 +12,12 @@ int enqueue_batch_iteration(mlem_batch *b, const std::vector<uint8_t> &passes) {
     hipStream_t s = b->stream;
     const uint32_t BC = b->BC;
     for (uint32_t p = 0; p < b->passes; ++p) {
-        if (!passes[p]) continue;
-        float *f = b->f.as<float>() + (size_t)p * b->cols_x * BC;
-        float *ratio = b->ratio.as<float>() + (size_t)p * b->rows_x * BC;
+        if (!passes[p] || !b->pass[p]) continue;
+        float *f = b->pass[p]->f.as<float>();
+        float *ratio = b->pass[p]->ratio.as<float>();
         EpiArgs fa{};
         fa.out = ratio;
-        fa.g = b->g.as<int32_t>() + (size_t)p * b->rows_x * BC;
+        fa.g = b->pass[p]->g.as<int32_t>();
         HIPCHK(launch_tiled_partials(b->tfp->plan, b->tfp->view, b->wfp->view, f, b->part_fp.as<float>(), BC, b->nt,
                                      s));
         HIPCHK(launch_tiled_reduce(b->tfp->plan, b->tfp->view, b->wfp->view, b->part_fp.as<float>(), BC, EPI_RATIO, fa,
@@ -35,6 +35,73 @@ int enqueue_batch_iteration(mlem_batch *b, const std::vector<uint8_t> &passes) {
 }
 
 }  // namespace
+
+namespace mlem {
+namespace abi {
+
+int batch_set_async(mlem_batch *b, uint32_t slot, const int32_t *g, const float *f0, uint64_t *allocated) {
+    if (allocated) *allocated = 0;
+    ARGCHK(b && g && f0, "mlem_batch_set: null argument");
+    ARGCHK(slot < b->slots, "mlem_batch_set: slot %u >= %u", slot, b->slots);
+    HIPCHK(hipSetDevice(b->m->device));
+    const uint32_t p = slot / b->BC, q = slot % b->BC;
+    const mlem_matrix *m = b->m;
+    hipStream_t s = b->stream;
+    if (!b->pass[p]) {  // first slot of this pass: its vectors, zero-filled (padding included)
+        auto np = std::make_unique<BatchPass>();
+        HIPCHK(np->g.alloc(sizeof(int32_t) * b->rows_x * b->BC));
+        HIPCHK(np->ratio.alloc(sizeof(float) * b->rows_x * b->BC));
+        HIPCHK(np->f.alloc(sizeof(float) * b->cols_x * b->BC));
+        HIPCHK(launch_fill_u32(np->g.as<uint32_t>(), 0u, b->rows_x * b->BC, s));
+        HIPCHK(launch_fill(np->ratio.as<float>(), 0.0f, b->rows_x * b->BC, s));
+        HIPCHK(launch_fill(np->f.as<float>(), 0.0f, b->cols_x * b->BC, s));
+        if (allocated) *allocated = np->g.bytes + np->ratio.bytes + np->f.bytes;
+        b->pass[p] = std::move(np);
+    }
+    BatchPass &bp = *b->pass[p];
+    // One staging vector serves both copies: the stream orders the second
+    // copy after the first scatter.
+    HIPCHK(hipMemcpyAsync(b->stage.p, g, sizeof(int32_t) * m->rows, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_strided_put(bp.g.as<int32_t>() + q, b->BC, b->stage.p, m->rows, s));
+    HIPCHK(hipMemcpyAsync(b->stage.p, f0, sizeof(float) * m->cols, hipMemcpyHostToDevice, s));
+    HIPCHK(launch_strided_put(bp.f.as<float>() + q, b->BC, b->stage.p, m->cols, s));
+    HIPCHK(launch_fill_u32(b->active.as<uint32_t>() + slot, 1u, 1, s));
+    b->active_host[slot] = 1;
+    return MLEM_OK;
+}
+
+int batch_get_image_async(mlem_batch *b, uint32_t slot, float *f_out) {
+    ARGCHK(b && f_out, "mlem_batch_get_image: null argument");
+    ARGCHK(slot < b->slots, "mlem_batch_get_image: slot %u >= %u", slot, b->slots);
+    const uint32_t p = slot / b->BC, q = slot % b->BC;
+    ARGCHK(b->pass[p], "mlem_batch_get_image: slot %u was never set", slot);
+    HIPCHK(hipSetDevice(b->m->device));
+    HIPCHK(launch_strided_get(b->stage.p, b->pass[p]->f.as<float>() + q, b->BC, b->m->cols, b->stream));
+    HIPCHK(hipMemcpyAsync(f_out, b->stage.p, sizeof(float) * b->m->cols, hipMemcpyDeviceToHost, b->stream));
+    return MLEM_OK;
+}
+
+int batch_deactivate_async(mlem_batch *b, uint32_t slot) {
+    ARGCHK(b, "mlem_batch_deactivate: null handle");
+    ARGCHK(slot < b->slots, "mlem_batch_deactivate: slot %u >= %u", slot, b->slots);
+    HIPCHK(hipSetDevice(b->m->device));
+    b->active_host[slot] = 0;
+    HIPCHK(launch_fill_u32(b->active.as<uint32_t>() + slot, 0u, 1, b->stream));
+    return MLEM_OK;
+}
+
+uint64_t batch_release_pass(mlem_batch *b, uint32_t p) {
+    if (!b || p >= b->passes || !b->pass[p]) return 0;
+    for (uint32_t q = 0; q < b->BC; ++q)
+        if (b->active_host[(size_t)p * b->BC + q]) return 0;
+    if (hipSetDevice(b->m->device) != hipSuccess || hipStreamSynchronize(b->stream) != hipSuccess) return 0;
+    const uint64_t bytes = b->pass[p]->g.bytes + b->pass[p]->ratio.bytes + b->pass[p]->f.bytes;
+    b->pass[p].reset();
+    return bytes;
+}
+
+}  // namespace abi
+}  // namespace mlem
 
 extern "C" {
 
@@ -229,16 +296,11 @@ int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opt
     if (int st = setup_tiled(m, 0, W, R, o.values, BC, groups, &b->tfp, &b->wfp, &b->part_fp)) return st;
     if (int st = setup_tiled(m, 1, W, R, o.values, BC, groups, &b->tbp, &b->wbp, &b->part_bp)) return st;
     const size_t per = (size_t)BC * b->passes;
-    HIPCHK(b->g.alloc(sizeof(int32_t) * b->rows_x * per));
-    HIPCHK(b->ratio.alloc(sizeof(float) * b->rows_x * per));
-    HIPCHK(b->f.alloc(sizeof(float) * b->cols_x * per));
+    b->pass.resize(b->passes);  // passes are allocated when first used (batch_set_async)
     HIPCHK(b->sens.alloc(sizeof(float) * m->cols));
     HIPCHK(b->active.alloc(sizeof(uint32_t) * per));
-    HIPCHK(b->stage.alloc(sizeof(float) * std::max<size_t>(m->rows, m->cols)));
-    HIPCHK(launch_fill(b->g.as<float>(), 0.0f, b->rows_x * per, b->stream));  // 0.0f has the bits of int 0
-    HIPCHK(launch_fill(b->ratio.as<float>(), 0.0f, b->rows_x * per, b->stream));
-    HIPCHK(launch_fill(b->f.as<float>(), 0.0f, b->cols_x * per, b->stream));
-    HIPCHK(launch_fill(b->active.as<float>(), 0.0f, per, b->stream));
+    HIPCHK(b->stage.alloc(sizeof(float) * std::max<size_t>(1, std::max(m->rows, m->cols))));
+    HIPCHK(launch_fill_u32(b->active.as<uint32_t>(), 0u, per, b->stream));
     b->active_host.assign(per, 0);
     if (o.values != VAL_F32)  // the quantized matrix's own sensitivity
         HIPCHK(hipMemcpyAsync(b->sens.p, b->tbp->sens.p, sizeof(float) * m->cols, hipMemcpyDeviceToDevice,
@@ -252,34 +314,12 @@ int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opt
 }
 
 int mlem_batch_set(mlem_batch *b, uint32_t slot, const int32_t *g, const float *f0) {
-    ARGCHK(b && g && f0, "mlem_batch_set: null argument");
-    ARGCHK(slot < b->slots, "mlem_batch_set: slot %u >= %u", slot, b->slots);
-    HIPCHK(hipSetDevice(b->m->device));
-    const uint32_t p = slot / b->BC, q = slot % b->BC;
-    const mlem_matrix *m = b->m;
-    hipStream_t s = b->stream;
-    HIPCHK(hipMemcpyAsync(b->stage.p, g, sizeof(int32_t) * m->rows, hipMemcpyHostToDevice, s));
-    HIPCHK(launch_strided_put(b->g.as<int32_t>() + ((size_t)p * b->rows_x) * b->BC + q, b->BC, b->stage.p, m->rows, s));
-    HIPCHK(hipStreamSynchronize(s));  // the stage is reused below
-    HIPCHK(hipMemcpyAsync(b->stage.p, f0, sizeof(float) * m->cols, hipMemcpyHostToDevice, s));
-    HIPCHK(launch_strided_put(b->f.as<float>() + ((size_t)p * b->cols_x) * b->BC + q, b->BC, b->stage.p, m->cols, s));
-    b->active_host[slot] = 1;
-    HIPCHK(hipMemcpyAsync(b->active.p, b->active_host.data(), sizeof(uint32_t) * b->active_host.size(),
-                          hipMemcpyHostToDevice, s));
-    HIPCHK(hipStreamSynchronize(s));
+    if (int st = batch_set_async(b, slot, g, f0, nullptr)) return st;
+    HIPCHK(hipStreamSynchronize(b->stream));  // g and f0 may be freed after the return
     return MLEM_OK;
 }
 
-int mlem_batch_deactivate(mlem_batch *b, uint32_t slot) {
-    ARGCHK(b, "mlem_batch_deactivate: null handle");
-    ARGCHK(slot < b->slots, "mlem_batch_deactivate: slot %u >= %u", slot, b->slots);
-    HIPCHK(hipSetDevice(b->m->device));
-    b->active_host[slot] = 0;
-    HIPCHK(hipMemcpyAsync(b->active.p, b->active_host.data(), sizeof(uint32_t) * b->active_host.size(),
-                          hipMemcpyHostToDevice, b->stream));
-    HIPCHK(hipStreamSynchronize(b->stream));
-    return MLEM_OK;
-}
+int mlem_batch_deactivate(mlem_batch *b, uint32_t slot) { return batch_deactivate_async(b, slot); }
 
 int mlem_batch_iterate(mlem_batch *b, uint32_t iters) {
     ARGCHK(b, "mlem_batch_iterate: null handle");
@@ -329,13 +369,7 @@ int mlem_batch_sync(mlem_batch *b) {
 }
 
 int mlem_batch_get_image(mlem_batch *b, uint32_t slot, float *f_out) {
-    ARGCHK(b && f_out, "mlem_batch_get_image: null argument");
-    ARGCHK(slot < b->slots, "mlem_batch_get_image: slot %u >= %u", slot, b->slots);
-    HIPCHK(hipSetDevice(b->m->device));
-    const uint32_t p = slot / b->BC, q = slot % b->BC;
-    HIPCHK(launch_strided_get(b->stage.p, b->f.as<float>() + ((size_t)p * b->cols_x) * b->BC + q, b->BC, b->m->cols,
-                              b->stream));
-    HIPCHK(hipMemcpyAsync(f_out, b->stage.p, sizeof(float) * b->m->cols, hipMemcpyDeviceToHost, b->stream));
+    if (int st = batch_get_image_async(b, slot, f_out)) return st;
     HIPCHK(hipStreamSynchronize(b->stream));
     return MLEM_OK;
 }
@@ -346,9 +380,7 @@ int mlem_batch_info(const mlem_batch *b, uint32_t *slots, uint32_t *per_pass, ui
     if (slots) *slots = b->slots;
     if (per_pass) *per_pass = b->BC;
     if (passes) *passes = b->passes;
-    if (device_bytes)
-        *device_bytes = b->g.bytes + b->ratio.bytes + b->f.bytes + b->sens.bytes + b->active.bytes + b->stage.bytes +
-                        b->part_fp.bytes + b->part_bp.bytes;
+    if (device_bytes) *device_bytes = b->device_bytes();
     return MLEM_OK;
 }
 

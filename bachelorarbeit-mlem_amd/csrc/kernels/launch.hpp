@@ -63,6 +63,7 @@ hipError_t launch_update(const float *f, const float *norm, float *bwproj, uint3
 hipError_t launch_update_image(float *f, const float *norm, float *bp, uint32_t cols, hipStream_t s);
 hipError_t launch_fill(float *dst, float v, uint64_t n, hipStream_t s);
 hipError_t launch_fill_f64(double *dst, double v, uint64_t n, hipStream_t s);
+hipError_t launch_fill_u32(uint32_t *dst, uint32_t v, uint64_t n, hipStream_t s);
 // csr4 AoS entries [first, first+n) -> col/val; bit 1 of *bad when a
 // column is >= cols.
 hipError_t launch_aos_split(const void *aos, uint64_t n, uint64_t first, uint32_t cols, uint32_t *col, float *val,

@@ -2,15 +2,24 @@
 // serving pattern of many MLEM requests against a few system matrices; the
 // reference has one reconstruction per process, main.cu:42-143).
 //
-// Each registered matrix gets a batch of slots (mlem_batch_*), sized by the
-// HBM budget.  Requests are queued per matrix; a step admits queued requests
-// into free slots (a request joins its batch between two iterations), runs
-// one MLEM iteration of every batch that has active slots and none in flight
-// (each batch on its own stream, so batches of different matrices overlap on
-// the device and go at their own pace), retires the requests that reached
-// their iteration count (image copied out, slot freed) and records each
-// request's latency from submission.  A
-// request's image does not depend on what else is in its batch.
+// Each registered matrix gets a batch of slots (mlem_batch_*).  Requests are
+// queued per matrix; a step admits queued requests into free slots (a
+// request joins its batch between two iterations), runs one MLEM iteration
+// of every batch that has active slots and none in flight (each batch on its
+// own stream, so batches of different matrices overlap on the device and go
+// at their own pace), retires the requests that reached their iteration
+// count (image copied out, slot freed) and records each request's latency
+// from submission.  A request's image does not depend on what else is in
+// its batch.
+//
+// HBM budget: a matrix is admitted with its tiles and its batch's fixed
+// buffers; a request is admitted into a slot of a pass that holds memory
+// already, or, when that pass's vectors (BC slots' counts, ratios and
+// images) fit in what is left of the budget, into a new one; otherwise it
+// waits, and passes of other matrices that have no active slot are freed to
+// make room.  Admission and retirement queue their copies on the batch's
+// stream without waiting (one wait per step that retires requests, for the
+// images); a request's inputs are kept until its first iteration has ended.
 #include <hip/hip_runtime.h>
 
 #include <chrono>
@@ -24,8 +33,7 @@
 #include <thread>
 #include <vector>
 
-#include "launch.hpp"
-#include "mlem_hip.h"
+#include "abi_internal.hpp"
 
 namespace {
 
@@ -45,7 +53,7 @@ int fail(int st, const char *fmt, ...) {
 struct Request {
     uint64_t id = 0;
     uint32_t matrix = 0, iters = 0, done_iters = 0;
-    std::vector<int32_t> g;
+    std::vector<int32_t> g;  // inputs, kept until the first iteration has ended (their copies are queued)
     std::vector<float> f0, image;
     Clock::time_point submitted, finished;
     bool finished_flag = false;
@@ -55,7 +63,7 @@ struct Served {
     mlem_matrix *m = nullptr;
     mlem_batch *batch = nullptr;
     uint32_t rows = 0, cols = 0, slots = 0;
-    uint64_t bytes = 0;                     // batch buffers
+    uint64_t bytes = 0;                     // matrix + batch, fixed part
     std::vector<uint64_t> slot_req;         // request id per slot, 0 = free
     std::deque<uint64_t> queue;             // waiting requests
     bool inflight = false;                  // an iteration is running on the batch's stream
@@ -94,21 +102,24 @@ int mlem_server_add_matrix(mlem_server *s, mlem_matrix *m, uint32_t max_slots, u
     int has_t = 0;
     if (int st = mlem_matrix_info(m, &rows, &cols, &nnz, &has_t, &mbytes))
         return fail(st, "mlem_server_add_matrix: %s", std::string(mlem_last_error()).c_str());
-    if (!has_t) return fail(MLEM_ERR_INVALID_ARGUMENT, "mlem_server_add_matrix: the matrix needs its transpose");
+    if (!has_t && !m->tiles_only)
+        return fail(MLEM_ERR_INVALID_ARGUMENT, "mlem_server_add_matrix: the matrix needs its transpose");
     if (max_slots == 0) return fail(MLEM_ERR_INVALID_ARGUMENT, "mlem_server_add_matrix: no slots");
     // The batch (its tiles included) is built first; it is kept only when
-    // the matrix and the batch fit in what is left of the budget.
+    // the matrix, the batch's fixed buffers and one pass fit in what is left
+    // of the budget.
     mlem_batch *b = nullptr;
     if (int st = mlem_batch_create(m, max_slots, &s->bopts, &b))
         return fail(st, "mlem_server_add_matrix: %s", std::string(mlem_last_error()).c_str());
-    uint64_t bbytes = 0;
-    mlem_batch_info(b, nullptr, nullptr, nullptr, &bbytes);
+    const uint64_t bbytes = b->fixed_bytes(), pbytes = b->pass_bytes();
     mlem_matrix_info(m, nullptr, nullptr, nullptr, nullptr, &mbytes);  // with the tiles the batch built
-    if (s->budget && s->used + mbytes + bbytes > s->budget) {
+    if (s->budget && s->used + mbytes + bbytes + pbytes > s->budget) {
         mlem_batch_destroy(b);
         return fail(MLEM_ERR_OUT_OF_MEMORY,
-                    "mlem_server_add_matrix: %.2f GB (matrix %.2f + batch %.2f) exceed the %.2f GB left of the budget",
-                    (mbytes + bbytes) / 1e9, mbytes / 1e9, bbytes / 1e9, (s->budget - s->used) / 1e9);
+                    "mlem_server_add_matrix: %.3f GB (matrix %.3f + batch %.3f + one pass %.3f) exceed the %.3f GB "
+                    "left of the budget",
+                    (mbytes + bbytes + pbytes) / 1e9, mbytes / 1e9, bbytes / 1e9, pbytes / 1e9,
+                    (s->budget - s->used) / 1e9);
     }
     auto sv = std::make_unique<Served>();
     sv->m = m;
@@ -147,37 +158,77 @@ int mlem_server_submit(mlem_server *s, uint32_t matrix_id, const int32_t *g, con
 
 namespace {
 
+// Frees passes without active slots of the batches other than `keep` that
+// have no iteration in flight, until `need` bytes fit in the budget; true
+// when they do.
+bool make_room(mlem_server *s, const Served *keep, uint64_t need) {
+    for (auto &svp : s->mats) {
+        if (s->used + need <= s->budget) break;
+        Served &o = *svp;
+        if (&o == keep || o.inflight) continue;
+        for (uint32_t p = 0; p < o.batch->passes && s->used + need > s->budget; ++p)
+            s->used -= mlem::abi::batch_release_pass(o.batch, p);
+    }
+    return s->used + need <= s->budget;
+}
+
+// Queued requests into free slots: first slots of passes that hold memory,
+// then slots of new passes while they fit in the budget.
 int admit(mlem_server *s, Served &sv) {
-    for (uint32_t slot = 0; slot < sv.slots && !sv.queue.empty(); ++slot) {
-        if (sv.slot_req[slot]) continue;
+    mlem_batch *b = sv.batch;
+    while (!sv.queue.empty()) {
+        int64_t pick = -1;
+        for (uint32_t slot = 0; slot < sv.slots && pick < 0; ++slot)
+            if (!sv.slot_req[slot] && b->pass[slot / b->BC]) pick = slot;
+        if (pick < 0) {
+            for (uint32_t slot = 0; slot < sv.slots && pick < 0; ++slot)
+                if (!sv.slot_req[slot]) pick = slot;
+            if (pick < 0) break;  // every slot busy
+            if (s->budget && s->used + b->pass_bytes() > s->budget && !make_room(s, &sv, b->pass_bytes()))
+                break;  // wait for memory
+        }
         Request &r = *s->reqs.at(sv.queue.front());
         sv.queue.pop_front();
-        if (int st = mlem_batch_set(sv.batch, slot, r.g.data(), r.f0.data()))
+        uint64_t allocated = 0;
+        if (int st = mlem::abi::batch_set_async(b, (uint32_t)pick, r.g.data(), r.f0.data(), &allocated))
             return fail(st, "mlem_server_step: %s", std::string(mlem_last_error()).c_str());
-        sv.slot_req[slot] = r.id;
-        std::vector<int32_t>().swap(r.g);  // the inputs are on the GPU now
-        std::vector<float>().swap(r.f0);
+        s->used += allocated;
+        sv.slot_req[pick] = r.id;
     }
     return MLEM_OK;
 }
 
 // The iteration in flight on sv's stream has ended: count it for every
-// request in the batch and retire those that reached their count.
+// request in the batch and retire those that reached their count (their
+// images copied out with one wait for all of them).
 int complete(mlem_server *s, Served &sv) {
     sv.inflight = false;
+    std::vector<Request *> done;
     for (uint32_t slot = 0; slot < sv.slots; ++slot) {
         const uint64_t id = sv.slot_req[slot];
         if (!id) continue;
         Request &r = *s->reqs.at(id);
+        if (r.done_iters == 0) {  // its input copies ran before this iteration
+            std::vector<int32_t>().swap(r.g);
+            std::vector<float>().swap(r.f0);
+        }
         if (++r.done_iters < r.iters) continue;
         r.image.resize(sv.cols);
-        if (int st = mlem_batch_get_image(sv.batch, slot, r.image.data()))
+        if (int st = mlem::abi::batch_get_image_async(sv.batch, slot, r.image.data()))
             return fail(st, "mlem_server_step: %s", std::string(mlem_last_error()).c_str());
-        if (int st = mlem_batch_deactivate(sv.batch, slot))
+        if (int st = mlem::abi::batch_deactivate_async(sv.batch, slot))
             return fail(st, "mlem_server_step: %s", std::string(mlem_last_error()).c_str());
-        r.finished = Clock::now();
-        r.finished_flag = true;
         sv.slot_req[slot] = 0;
+        done.push_back(&r);
+    }
+    if (!done.empty()) {
+        const hipError_t e = hipStreamSynchronize(sv.batch->stream);
+        if (e != hipSuccess) return fail(MLEM_ERR_HIP, "mlem_server_step: %s", hipGetErrorString(e));
+        const Clock::time_point now = Clock::now();
+        for (Request *r : done) {
+            r->finished = now;
+            r->finished_flag = true;
+        }
     }
     return MLEM_OK;
 }
@@ -216,7 +267,7 @@ int mlem_server_step(mlem_server *s, uint32_t max_wait_us, uint32_t *active, uin
         for (auto &svp : s->mats) {
             Served &sv = *svp;
             if (!sv.inflight) continue;
-            const hipError_t q = hipStreamQuery(static_cast<hipStream_t>(mlem_batch_stream(sv.batch)));
+            const hipError_t q = hipStreamQuery(sv.batch->stream);
             if (q == hipErrorNotReady) continue;
             if (q != hipSuccess) return fail(MLEM_ERR_HIP, "mlem_server_step: %s", hipGetErrorString(q));
             if (int st = complete(s, sv)) return st;

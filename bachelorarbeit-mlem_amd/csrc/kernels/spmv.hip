@@ -643,6 +643,14 @@ hipError_t launch_strided_get(void *dst, const void *src, uint32_t stride, uint6
     return hipGetLastError();
 }
 
+hipError_t launch_fill_u32(uint32_t *dst, uint32_t v, uint64_t n, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    uint64_t blocks = (n + kWG - 1) / kWG;
+    if (blocks > 65536) blocks = 65536;
+    hipLaunchKernelGGL(fill_kernel<uint32_t>, dim3((uint32_t)blocks), dim3(kWG), 0, s, dst, v, n);
+    return hipGetLastError();
+}
+
 hipError_t launch_fill_f64(double *dst, double v, uint64_t n, hipStream_t s) {
     if (n == 0) return hipSuccess;
     uint64_t blocks = (n + kWG - 1) / kWG;

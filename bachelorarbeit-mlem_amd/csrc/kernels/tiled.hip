@@ -1309,13 +1309,16 @@ TileWorkList tile_work_list(const TilePlan &p, const uint32_t *nz, const uint64_
         }
     }
     L.wl_first[p.nblk] = L.nwg;
-    // Launch order: by the column segment of each run's first tile, then by
-    // row block, as for the block streams: the workgroups in flight together
-    // stage the x segments of a band of columns, which stay in L2.
-    // MLEM_TILE_ORDER=rows keeps row-block order (for measurements; the
-    // partial sums and the images are the same either way).
+    // Launch order: row-block order (workgroup w takes run w).  With
+    // MLEM_TILE_ORDER=cols the runs go by the column segment of their first
+    // tile, then by row block, as for the block streams, so that the
+    // workgroups in flight together stage the x segments of a band of
+    // columns: on the 3D scanner matrix that fetches the stored bytes + 2.3%
+    // instead of + 4.8%, but each launch takes 1.5-3% longer (the streams in
+    // flight are spread over the whole matrix), so row-block order stays the
+    // default.  The partial sums and the images are the same either way.
     const char *ord = std::getenv("MLEM_TILE_ORDER");
-    if (!(ord && std::strcmp(ord, "rows") == 0)) {
+    if (ord && std::strcmp(ord, "cols") == 0) {
         L.order.resize(L.nwg);
         for (uint32_t r = 0; r < L.nwg; ++r) L.order[r] = r;
         auto first_seg = [&](uint32_t r) { return L.tiles[4 * (size_t)L.wl[4 * (size_t)r + 1]]; };

@@ -64,10 +64,13 @@ def cpu_model():
     return "unknown"
 
 
+_AFFINITY = sorted(os.sched_getaffinity(0))  # before the OpenMP runtime binds this thread to one place
+
+
 def host_cpus():
-    """(logical CPUs this process may run on, physical cores among them) from sched_getaffinity and the sysfs
-    topology."""
-    cpus = sorted(os.sched_getaffinity(0))
+    """(logical CPUs this process may run on, physical cores among them) from the affinity it started with and
+    the sysfs topology."""
+    cpus = _AFFINITY
     cores = set()
     for c in cpus:
         try:
@@ -356,7 +359,7 @@ def run_serve(args, M, threads):
         mats.append(dict(name=name, label=label, m=m, sinos=sinos, f0=f0, t_single=t_single, s32=s32, rows=A.rows,
                          cols=A.cols, nnz=A.nnz))
         del A
-    srv = M.Server(values=VALUES[args.values], per_pass=args.per_pass)
+    srv = M.Server(values=VALUES[args.values], per_pass=args.per_pass, use_graph=args.graph)
     t = time.time()
     for d in mats:
         d["id"] = srv.add_matrix(d["m"], slots=args.serve_slots)
@@ -404,7 +407,8 @@ def run_serve(args, M, threads):
         "vs_baseline": None, "dtype": f"{args.values} values, f32 arithmetic", "data": "synthetic",
         "config": {"workload": "served mix: " + " + ".join(d["label"] for d in mats), "matrices": names,
                    "requests": n, "iterations_per_request": iters_desc, "slots_per_matrix": args.serve_slots,
-                   "per_pass": args.per_pass, "values": args.values, "hbm_used_bytes": info["used"]},
+                   "per_pass": args.per_pass, "values": args.values, "graph": bool(args.graph),
+                   "hbm_used_bytes": info["used"]},
         "latency_ms": {"arrivals": f"Poisson, {head['rate']:.3f} req/s ({head['load']} x backlog throughput)",
                        "p50": head["p50"], "p99": head["p99"], "max": head["max"],
                        "backlog_p50": pct(lat1, 50), "backlog_p99": pct(lat1, 99)},

@@ -2,7 +2,9 @@
 different iteration counts on two matrices join and leave their batches
 between iterations; each result is bitwise the image the same request gets
 alone in a batch of its own, and within the FP64-oracle bar (1e-5 rel-L2 for
-f32 values).  Also: the HBM budget refuses a matrix that does not fit, more
+f32 values).  Also: the HBM budget refuses a matrix that does not fit and
+admits requests by the memory of their slots' passes (waiting while a new
+pass does not fit, freeing idle passes of other matrices for them), more
 requests than slots queue, and bad arguments are reported."""
 import numpy as np
 import pytest
@@ -120,17 +122,74 @@ def test_budget_refuses_a_matrix_that_does_not_fit(served):
     assert "budget" in str(e.value)
     assert srv.info()["used"] == 0 and srv.info()["matrices"] == 0
     srv.close()
-    # a budget that holds the small matrix and its batch but not both matrices
-    probe = M.Server()
-    probe.add_matrix(d1["m"], slots=4)
-    need1 = probe.info()["used"]
-    probe.close()
-    srv = M.Server(hbm_budget_bytes=need1 + 1024)
+    # a budget that holds the small matrix, its batch and one pass but not both matrices
+    need1, pass1 = footprint(d1, 4)
+    srv = M.Server(hbm_budget_bytes=need1 + pass1 + 1024)
     srv.add_matrix(d1["m"], slots=4)
     assert srv.info()["used"] == need1
     with pytest.raises(M.MlemError):
         srv.add_matrix(d0["m"], slots=4)
     assert srv.info()["matrices"] == 1
+    srv.close()
+
+
+def footprint(d, per_pass):
+    """(matrix + batch bytes a server accounts when the matrix is added, bytes of one pass of slots)."""
+    with M.Server(per_pass=per_pass) as probe:
+        mid = probe.add_matrix(d["m"], slots=per_pass)
+        need = probe.info()["used"]
+        rid = probe.submit(mid, d["sinos"][0], d["f0"], 1)
+        while probe.poll(rid, mid) is None:
+            probe.step()
+        return need, probe.info()["used"] - need
+
+
+def drive(srv, submitted, budget):
+    """Steps srv until every (request id, matrix id) in `submitted` is done; checks the budget after every step.
+    Returns ({request id: image}, most requests active at once)."""
+    done, most = {}, 0
+    for _ in range(5000):
+        active, _ = srv.step()
+        most = max(most, active)
+        assert srv.info()["used"] <= budget
+        for rid, mid in submitted:
+            if rid not in done:
+                out = srv.poll(rid, mid)
+                if out is not None:
+                    done[rid] = out[0]
+        if len(done) == len(submitted):
+            return done, most
+    raise AssertionError("requests did not finish")
+
+
+def test_budget_admits_requests_by_pass(served):
+    """Slots beyond what the budget holds stay unused: with room for one pass of 4, at most 4 of 8 slots run,
+    the other requests wait, and each result is the same as alone."""
+    d = served[1]
+    need, pas = footprint(d, 4)
+    budget = need + pas + pas // 2
+    srv = M.Server(hbm_budget_bytes=budget, per_pass=4)
+    mid = srv.add_matrix(d["m"], slots=8)
+    subs = [(srv.submit(mid, d["sinos"][j], d["f0"], 3 + j), mid) for j in range(6)]
+    done, most = drive(srv, subs, budget)
+    assert most == 4
+    for j, (rid, _) in enumerate(subs):
+        assert np.array_equal(done[rid].view(np.uint32), alone(d, d["sinos"][j], 3 + j, 4).view(np.uint32))
+    srv.close()
+
+
+def test_budget_frees_idle_passes_of_other_matrices(served):
+    """Two matrices and room for one pass between them: the second matrix's requests wait until the first
+    matrix's pass has no active slot, which is then freed for them."""
+    (n0, p0), (n1, p1) = footprint(served[0], 4), footprint(served[1], 4)
+    budget = n0 + n1 + max(p0, p1)
+    srv = M.Server(hbm_budget_bytes=budget, per_pass=4)
+    ids = [srv.add_matrix(d["m"], slots=4) for d in served]
+    subs = [(srv.submit(ids[k], served[k]["sinos"][j], served[k]["f0"], 4), ids[k]) for k in (0, 1) for j in (0, 1)]
+    done, _ = drive(srv, subs, budget)
+    for (rid, mid), (k, j) in zip(subs, [(k, j) for k in (0, 1) for j in (0, 1)]):
+        d = served[k]
+        assert np.array_equal(done[rid].view(np.uint32), alone(d, d["sinos"][j], 4, 4).view(np.uint32))
     srv.close()
 
 
