@@ -15,7 +15,8 @@ from . import (HIP_LIB_PATH, MLEM_OK, Csr, MlemError, Opts, _load, _ptr, cint, f
 
 __all__ = ["hip_lib", "device_count", "device_info", "DeviceArray", "Stream", "Event", "ResidentMatrix", "Recon",
            "Batch", "BatchOpts", "Server",
-           "mlem_run", "default_opts", "opts", "merge_workspace_bytes", "fp_merge", "bp_merge", "fp_csr_vector",
+           "mlem_run", "default_opts", "opts", "merge_workspace_bytes", "fp_merge", "fp_merge_noalloc", "bp_merge",
+           "fp_csr_vector",
            "bp_csr_vector", "bp_scatter", "fp_ratio_merge", "fp_ratio_csr_vector", "bp_update_merge",
            "bp_update_csr_vector", "ratio", "update", "colsum_f64", "colsum_scatter_f64", "fill_f32", "copy_f32",
            "device_sync",
@@ -56,6 +57,7 @@ def hip_lib() -> C.CDLL:
             "mlem_update": (cint, [vp, vp, vp, u32, vp]),
             "mlem_merge_workspace_bytes": (sz, [u32, u64, u32]),
             "mlem_fp_merge": (cint, [vp, vp, vp, vp, vp, u32, u32, u64, vp, sz, vp]),
+            "mlem_fp_merge_noalloc": (cint, [vp, vp, vp, vp, vp, u32, u32, u64, vp]),
             "mlem_bp_merge": (cint, [vp, vp, vp, vp, vp, u32, u32, u64, vp, sz, vp]),
             "mlem_fp_csr_vector": (cint, [vp, vp, vp, vp, vp, u32, u64, u32, vp]),
             "mlem_bp_csr_vector": (cint, [vp, vp, vp, vp, vp, u32, u64, u32, vp]),
@@ -293,6 +295,12 @@ def merge_workspace_bytes(rows: int, nnz: int, secsize: int) -> int:
 def fp_merge(rowptr, col, val, f, out, secsize, rows, nnz, ws, ws_bytes, stream=None):
     _check(hip_lib().mlem_fp_merge(_dp(rowptr), _dp(col), _dp(val), _dp(f), _dp(out), secsize, rows, nnz, _dp(ws),
                                    ws_bytes, _sh(stream)))
+
+
+def fp_merge_noalloc(rowptr, col, val, f, out, secsize, rows, nnz, stream=None):
+    """fp_merge with the library's fixed scratch instead of a workspace (allocates nothing)."""
+    _check(hip_lib().mlem_fp_merge_noalloc(_dp(rowptr), _dp(col), _dp(val), _dp(f), _dp(out), secsize, rows, nnz,
+                                           _sh(stream)))
 
 
 def bp_merge(rowptrT, colT, valT, correl, out, secsize, cols, nnz, ws, ws_bytes, stream=None):

@@ -107,6 +107,21 @@ int mlem_bp_merge(const uint64_t *rowptrT, const uint32_t *colT,
                   const float *valT, const float *correl, float *bwproj,
                   uint32_t secsize, uint32_t cols, uint64_t nnz,
                   void *workspace, size_t workspace_bytes, void *stream);
+/* mlem_fp_merge without a workspace and without allocating anything: no
+ * counterpart in the reference.  For callers that cannot allocate device
+ * memory (code under stream capture, tight serving loops, harnesses that
+ * forbid hipMalloc).  The partition and carries live in a fixed array of
+ * MLEM_NOALLOC_SCRATCH_BYTES per device (held until the process ends), so
+ * matrices needing more (mlem_merge_workspace_bytes() above it: rows + nnz
+ * beyond about 1.6 million x 256 x secsize) are refused with
+ * MLEM_ERR_INVALID_ARGUMENT.  Calls that overlap on one device share that
+ * array and race: issue them on one stream, or wait between them.  Output
+ * bitwise that of mlem_fp_merge. */
+#define MLEM_NOALLOC_SCRATCH_BYTES (32u << 20)
+int mlem_fp_merge_noalloc(const uint64_t *rowptr, const uint32_t *col,
+                          const float *val, const float *f, float *fwproj,
+                          uint32_t secsize, uint32_t rows, uint64_t nnz,
+                          void *stream);
 
 /* CSR-vector SpMV (calcFwProj_csr_vector / calcBwProj_csr_vector,
  * kernel.cu:118-153,297-336): a group of `lanes` work-items per row
