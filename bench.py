@@ -109,7 +109,7 @@ def cpu_baseline_f32(O, A, AT, g, s32, f0, threads, budget_s=25.0, runs=3):
         "omp": {v: os.environ.get(v) for v in ("OMP_NUM_THREADS", "OMP_PROC_BIND", "OMP_PLACES")}}
 
 
-def measured_traffic(workload_name, side, tile_width, tile_rows):
+def measured_traffic(workload_name, nnz, side, tile_width, tile_rows):
     """HBM bytes per launch of the blocked FP or BP kernel from profiles/traffic.json (rocprofv3 FETCH_SIZE and
     WRITE_SIZE in KiB per dispatch, each scaled by bytes/KiB read from the copy kernel), or None when it was not
     measured for this workload and tiling."""
@@ -117,7 +117,7 @@ def measured_traffic(workload_name, side, tile_width, tile_rows):
     try:
         with open(path) as f:
             t = json.load(f)
-        if t.get("workload") != workload_name or side not in t:
+        if t.get("workload") != workload_name or t.get("nnz") != nnz or side not in t:
             return None
         if t.get("tile_width") != (tile_width or 1024) or t.get("tile_rows") != (tile_rows or 2048):
             return None
@@ -605,8 +605,9 @@ def main():
                     "algorithmic_bytes_per_launch": kb, "avg_launch_ms": round(ms, 4),
                     "phases_ms": {"fp_kernel": round(ph[0], 4), "fp_epilogue": round(ph[1], 4),
                                   "bp_kernel": round(ph[2], 4), "bp_epilogue": round(ph[3], 4)}}
-        tr = (measured_traffic(args.workload, side, args.tile, args.tile_rows)
-              if (algo == "merge" and blocked and not args.tile_groups and args.values == "f32") else None)
+        tr = (measured_traffic(args.workload, nnz, side, args.tile, args.tile_rows)
+              if (algo == "merge" and blocked and not args.tile_groups and args.values == "f32"
+                  and os.environ.get("MLEM_TILE_ORDER", "") != "cols") else None)
         if tr:
             roofline["traffic"] = tr["bytes"]
             roofline["traffic_source"] = tr["source"]
