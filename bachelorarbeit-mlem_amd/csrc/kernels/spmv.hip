@@ -2,8 +2,9 @@
 //
 //  * csr_vector_kernel<G, EPI>: a group of G work-items per row
 //    (calcFwProj_csr_vector / calcBwProj_csr_vector, kernel.cu:118-153,
-//    297-336).  16-byte non-temporal loads of col/val, fixed-order xor
-//    shuffle reduction, plain store through the epilogue.
+//    297-336).  Consecutive 4-byte non-temporal loads of col/val, eight
+//    rounds in flight, fixed-order xor shuffle reduction, plain store
+//    through the epilogue.
 //  * merge_*: block-level merge-path SpMV (calcFwProj_merge_based /
 //    calcBwProj_merge_based, kernel.cu:52-105,190-286).  The merge path of
 //    (row ends, nonzeros) is cut into tiles of 256*S items; each workgroup
@@ -66,38 +67,42 @@ __device__ __forceinline__ double group_sum_f64(double v) {
 
 // Row sum of val[k] * x[col[k]] over [b, e) split over the G work-items of a
 // group (this work-item is `lane`); per-lane partials, not yet reduced.
+// Consecutive work-items take consecutive entries (4-byte loads), so one
+// gather instruction reads neighbouring columns, which share cache lines.
+// Rounds of U = 8 entries per work-item are loaded before any is used; the
+// last (partial) round is issued the same way with the entries past the row
+// end masked off, so a short row costs one load latency, not one per entry.
 template <int G>
 __device__ __forceinline__ float row_partial(const uint32_t *__restrict__ col, const float *__restrict__ val,
                                              const float *__restrict__ x, uint64_t b, uint64_t e, uint32_t lane) {
+    constexpr int U = 8;
     float acc = 0.0f;
-    uint64_t a = (b + 3) & ~uint64_t(3);
-    if (a > e) a = e;
-    if (b + lane < a) acc = ldnt(val + b + lane) * x[ldnt(col + b + lane)];
-    const uint64_t be = a + ((e - a) & ~uint64_t(4 * G - 1));
-    uint64_t k = a + 4ull * lane;
-    for (; k + 4ull * G < be; k += 8ull * G) {
-        const u32x4 c0 = ldnt(reinterpret_cast<const u32x4 *>(col + k));
-        const f32x4 v0 = ldnt(reinterpret_cast<const f32x4 *>(val + k));
-        const u32x4 c1 = ldnt(reinterpret_cast<const u32x4 *>(col + k + 4 * G));
-        const f32x4 v1 = ldnt(reinterpret_cast<const f32x4 *>(val + k + 4 * G));
-        acc += v0.x * x[c0.x];
-        acc += v0.y * x[c0.y];
-        acc += v0.z * x[c0.z];
-        acc += v0.w * x[c0.w];
-        acc += v1.x * x[c1.x];
-        acc += v1.y * x[c1.y];
-        acc += v1.z * x[c1.z];
-        acc += v1.w * x[c1.w];
+    uint64_t k = b + lane;
+    for (; k + (uint64_t)(U - 1) * G < e; k += (uint64_t)U * G) {
+        uint32_t c[U];
+        float v[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            c[u] = ldnt(col + k + (uint64_t)u * G);
+            v[u] = ldnt(val + k + (uint64_t)u * G);
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u) acc += v[u] * x[c[u]];
     }
-    if (k < be) {
-        const u32x4 c0 = ldnt(reinterpret_cast<const u32x4 *>(col + k));
-        const f32x4 v0 = ldnt(reinterpret_cast<const f32x4 *>(val + k));
-        acc += v0.x * x[c0.x];
-        acc += v0.y * x[c0.y];
-        acc += v0.z * x[c0.z];
-        acc += v0.w * x[c0.w];
+    if (k < e) {  // fewer than U rounds left
+        uint32_t c[U];
+        float v[U];
+        bool in[U];
+#pragma unroll
+        for (int u = 0; u < U; ++u) {
+            in[u] = k + (uint64_t)u * G < e;
+            c[u] = in[u] ? ldnt(col + k + (uint64_t)u * G) : 0u;
+            v[u] = in[u] ? ldnt(val + k + (uint64_t)u * G) : 0.0f;
+        }
+#pragma unroll
+        for (int u = 0; u < U; ++u)
+            if (in[u]) acc += v[u] * x[c[u]];
     }
-    for (uint64_t t = be + lane; t < e; t += G) acc += ldnt(val + t) * x[ldnt(col + t)];
     return acc;
 }
 

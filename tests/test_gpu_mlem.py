@@ -48,6 +48,17 @@ def test_mlem_50_iterations_vs_fp64_oracle(c1, resident, ref50, trans, spmv):
     assert err <= REL_L2
 
 
+@pytest.mark.parametrize("tile", [0, 1])
+def test_merge_fp_with_csr_vector_bp_vs_fp64_oracle(c1, resident, ref50, tile):
+    """The 2D workload's own pairing: merge-based FP (blocked, or the plain merge path) with CSR-vector BP on A^T
+    (bp_spmv = 2), 50 iterations from x0 = 1."""
+    o = M.opts(spmv=0, using_trans=0, secsize_fp=5, secsize_bp=5, tile_width=tile, bp_spmv=2)
+    img = M.mlem_run(resident, c1["g"], ref50["f0"], 50, sens=ref50["s32"], options=o)
+    err = rel_l2(img, ref50["ref"])
+    print(f"merge FP (tile {tile}) + CSR-vector BP: rel-L2 vs FP64 oracle {err:.3e}", flush=True)
+    assert err <= REL_L2
+
+
 def test_sensitivity_fp64_matches_oracle(c1, resident, ref50):
     A = c1["A"]
     ref = O.colsum_f64(A.rowptr, A.col, A.val, A.rows, A.cols)
