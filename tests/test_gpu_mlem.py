@@ -201,6 +201,23 @@ def test_blocked_work_split(c1, resident, ref50, groups):
     assert rel_l2(img, ref50["ref"]) <= REL_L2
 
 
+def test_blocked_column_segment_order_is_bitwise_row_order(c1, ref50, monkeypatch):
+    """MLEM_TILE_ORDER=cols launches the blocked kernel's runs by column segment instead of by row block: each
+    run still writes its own partial sums and the epilogue adds them in run order, so the images are bitwise the
+    row-order ones (two tile shapes, several workgroup targets, 30 iterations from x0 = 1)."""
+    A, AT, g, f0, s32 = c1["A"], c1["AT"], c1["g"], ref50["f0"], ref50["s32"]
+    for kw in (dict(tile_width=1024, tile_rows=512), dict(tile_width=2048, tile_rows=2048, tile_groups=64),
+               dict(tile_width=1024, tile_rows=1024, tile_groups=100000)):
+        imgs = []
+        for order in ("rows", "cols"):
+            monkeypatch.setenv("MLEM_TILE_ORDER", order)
+            print(f"blocked launch order {order} {kw}", flush=True)
+            with M.ResidentMatrix(A, AT) as m:  # tiles and work lists are built per matrix, under the order set
+                imgs.append(M.mlem_run(m, g, f0, 30, sens=s32, options=M.opts(spmv=0, using_trans=0, **kw)))
+        assert np.array_equal(imgs[0].view(np.uint32), imgs[1].view(np.uint32))
+        assert np.isfinite(imgs[0]).all() and imgs[0].sum() > 0
+
+
 def test_default_row_blocks_follow_matrix_size(c1, resident, ref50):
     """The 2D ring has under two units of entries per run at the default workgroup counts, so its default row
     blocks are 512 rows: the default run is bitwise the 512-row run (one reconstruction and a batch of 4 per pass),
