@@ -106,6 +106,42 @@ __device__ __forceinline__ uint32_t group_col(const u32x4 &w, int j) {
     return (uint32_t)v & ((1u << CB) - 1u);
 }
 
+// The same with the field extract written as v_bfe_u32 (after a
+// v_alignbit_b32 for a field that crosses a dword), so that with the
+// address arithmetic that follows (v_lshl_add_u32) a column takes two or
+// three instructions; the compiler turns an extract followed by a shift
+// into a shift, a mask and an add, three or four.
+template <int O, int N>
+__device__ __forceinline__ uint32_t bfe_u32(uint32_t w) {
+    uint32_t r;
+    asm("v_bfe_u32 %0, %1, %2, %3" : "=v"(r) : "v"(w), "n"(O), "n"(N));
+    return r;
+}
+template <int CB, int J>
+__device__ __forceinline__ uint32_t group_col_j(const u32x4 &w) {
+    constexpr int bit = 32 + J * CB, d = bit / 32, o = bit % 32;
+    const uint32_t lo = d == 1 ? w.y : d == 2 ? w.z : w.w;
+    if constexpr (o + CB <= 32) {
+        return bfe_u32<o, CB>(lo);
+    } else {
+        const uint32_t hi = d == 1 ? w.z : w.w;
+        return bfe_u32<0, CB>(__builtin_amdgcn_alignbit(hi, lo, o));
+    }
+}
+template <int CB>
+__device__ __forceinline__ uint32_t group_col_bfe(const u32x4 &w, int j) {
+    switch (j) {
+        case 0: return group_col_j<CB, 0>(w);
+        case 1: return group_col_j<CB, 1>(w);
+        case 2: return group_col_j<CB, 2>(w);
+        case 3: return group_col_j<CB, 3>(w);
+        case 4: return group_col_j<CB, 4>(w);
+        case 5: return group_col_j<CB, 5>(w);
+        case 6: return group_col_j<CB, 6>(w);
+        default: return group_col_j<CB, 7>(w);
+    }
+}
+
 // A run position: staged tile ti (index into the workgroup's tile list),
 // unit c of that tile, n entries in the unit.
 struct Unit {
@@ -136,12 +172,17 @@ __device__ __forceinline__ Unit make_unit(const uint4 *tl, uint32_t ti, uint32_t
 template <int EB>
 __device__ __forceinline__ float uf16_lo(uint32_t w) {  // bits 0-15 of w
     constexpr int SH = 7 + EB, EMAX = (1 << EB) - 1;
-    return __uint_as_float(((w << 16) >> (16 - SH)) + ((uint32_t)(127 - EMAX) << 23));
+    // The extract as written (the compiler turns a 16-bit extract at bit 0
+    // and the shift into a shift, a mask and an or: three instructions
+    // where these are two).
+    uint32_t c;
+    asm("v_bfe_u32 %0, %1, 0, 16" : "=v"(c) : "v"(w));
+    return __uint_as_float((c << SH) + ((uint32_t)(127 - EMAX) << 23));
 }
 template <int EB>
 __device__ __forceinline__ float uf16_hi(uint32_t w) {  // bits 16-31 of w
     constexpr int SH = 7 + EB, EMAX = (1 << EB) - 1;
-    return __uint_as_float(((w >> (16 - SH)) & (0xffffu << SH)) + ((uint32_t)(127 - EMAX) << 23));
+    return __uint_as_float(((w >> 16) << SH) + ((uint32_t)(127 - EMAX) << 23));
 }
 
 // Value j (0..7) of a work-item's group from its loaded 16 bytes: f32
@@ -392,16 +433,51 @@ __device__ __forceinline__ uint32_t dpp_get(uint32_t v) {
 // One step of a segmented inclusive scan across the wave: (v, f) combined
 // with the running value of an earlier lane, where f marks that a segment
 // starts at or before this lane (the earlier part then does not count).
-template <int BC, int CTRL, int ROWM>
-__device__ __forceinline__ void seg_scan_step(float (&v)[BC], uint32_t &f) {
-    const uint32_t nf = dpp_get<CTRL, ROWM>(f);
-#pragma unroll
-    for (int b = 0; b < BC; ++b) {
-        const float nv = __uint_as_float(dpp_get<CTRL, ROWM>(__float_as_uint(v[b])));
-        v[b] = f ? v[b] : nv + v[b];
-    }
-    f |= nf;
+// The same for floats, `none` where there is no such lane.
+template <int CTRL, int ROWM, bool ZERO>
+__device__ __forceinline__ float dpp_getf(float v, float none) {
+    return __uint_as_float(
+        (uint32_t)__builtin_amdgcn_update_dpp((int)__float_as_uint(none), (int)__float_as_uint(v), CTRL, ROWM, 0xf, ZERO));
 }
+
+// v += (v of the lane DPP control DSTR reads) * m, and then m *= (m of
+// that lane), in the rows ROWS: one v_fmac_f32_dpp per value (the compiler
+// does not fold a DPP read into an FMA).  Lanes without a source lane, and
+// rows outside ROWS, are not written (bound_ctrl off): v and m stay.  The
+// s_nop before each gives the DPP read of a register that the previous
+// VALU instruction may have written its two wait states.
+#define MLEM_SEG_STEP_ASM(DSTR, ROWS)                                                                        \
+    do {                                                                                                     \
+        _Pragma("unroll") for (int b = 0; b < BC; ++b) asm volatile(                                         \
+            "s_nop 1\n\tv_fmac_f32_dpp %0, %0, %1 " DSTR " row_mask:" ROWS " bank_mask:0xf"                     \
+            : "+v"(v[b])                                                                                     \
+            : "v"(m));                                                                                       \
+        asm volatile("s_nop 1\n\tv_mul_f32_dpp %0, %0, %0 " DSTR " row_mask:" ROWS " bank_mask:0xf" : "+v"(m)); \
+    } while (0)
+
+// One step of a segmented inclusive scan across the wave: v combined with
+// the running value of an earlier lane while this lane's segment is still
+// open, m = 1.0 (no segment starts at or before the lane in the range seen
+// so far) or 0.0.  v + nv * m is exact for m in {0, 1}, and the masks
+// combine by a product.
+template <int BC, int CTRL>
+__device__ __forceinline__ void seg_scan_step(float (&v)[BC], float &m) {
+    if constexpr (CTRL == 0x111)
+        MLEM_SEG_STEP_ASM("row_shr:1", "0xf");
+    else if constexpr (CTRL == 0x112)
+        MLEM_SEG_STEP_ASM("row_shr:2", "0xf");
+    else if constexpr (CTRL == 0x114)
+        MLEM_SEG_STEP_ASM("row_shr:4", "0xf");
+    else if constexpr (CTRL == 0x118)
+        MLEM_SEG_STEP_ASM("row_shr:8", "0xf");
+    else if constexpr (CTRL == 0x142)
+        MLEM_SEG_STEP_ASM("row_bcast:15", "0xa");
+    else {
+        static_assert(CTRL == 0x143, "scan steps: row_shr 1, 2, 4, 8, row_bcast 15, 31");
+        MLEM_SEG_STEP_ASM("row_bcast:31", "0xc");
+    }
+}
+#undef MLEM_SEG_STEP_ASM
 
 template <int W, int VT, int BC, int NT>
 __global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t nseg,
@@ -535,27 +611,29 @@ __global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t ns
         const float *xw = xs + (inA ? slotA : slotA ^ 1u) * XS;
         // Every quad is summed on its own (padding reads column W, whose
         // slot holds 0); work-items past the window's end skip the sums.
+        // Work-items past the window's end sum their re-read first group
+        // too (finite values, columns within the slot) and drop the sums:
+        // they end no row and carry 0.
+        // A quad's four x reads are issued together, then its products.
         float sq[NQ][BC];
 #pragma unroll
-        for (int q = 0; q < NQ; ++q)
+        for (int q = 0; q < NQ; ++q) {
+            f32x4 t4[4][BC / 4];
 #pragma unroll
-            for (int c = 0; c < BC; ++c) sq[q][c] = 0.0f;
-        if (lane_valid) {
+            for (int jj = 0; jj < 4; ++jj) {
+                const float *xg = xw + group_col_bfe<CB>(cg[q / 2], 4 * (q & 1) + jj) * BC;
 #pragma unroll
-            for (int q = 0; q < NQ; ++q) {
+                for (int qq = 0; qq < BC / 4; ++qq) t4[jj][qq] = *reinterpret_cast<const f32x4 *>(xg + 4 * qq);
+            }
 #pragma unroll
-                for (int jj = 0; jj < 4; ++jj) {
-                    const int m = 4 * (q & 1) + jj;
-                    const float v = group_val<VT>(cv0[q / 2], cv1[q / 2], cq[q / 2], m);
-                    const float *xg = xw + group_col<CB>(cg[q / 2], m) * BC;
+            for (int jj = 0; jj < 4; ++jj) {
+                const float v = group_val<VT>(cv0[q / 2], cv1[q / 2], cq[q / 2], 4 * (q & 1) + jj);
 #pragma unroll
-                    for (int qq = 0; qq < BC / 4; ++qq) {
-                        const f32x4 t4 = *reinterpret_cast<const f32x4 *>(xg + 4 * qq);
+                for (int qq = 0; qq < BC / 4; ++qq)
 #pragma unroll
-                        for (int bb = 0; bb < 4; ++bb)
-                            sq[q][4 * qq + bb] = jj == 0 ? v * t4[bb] : fmaf(v, t4[bb], sq[q][4 * qq + bb]);
-                    }
-                }
+                    for (int bb = 0; bb < 4; ++bb)
+                        sq[q][4 * qq + bb] =
+                            jj == 0 ? v * t4[jj][qq][bb] : fmaf(v, t4[jj][qq][bb], sq[q][4 * qq + bb]);
             }
         }
         // A row ends after quad q where the next quad (the next work-item's
@@ -581,18 +659,31 @@ __global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t ns
         // registers across the wave, then across the waves from their
         // totals in LDS.
         float a[BC], s[BC];
-        uint32_t f;
+        const bool open = lane_valid && !endq[NQ - 1];
 #pragma unroll
-        for (int b = 0; b < BC; ++b) a[b] = endq[NQ - 1] ? 0.0f : sq[NQ - 1][b];
+        for (int b = 0; b < BC; ++b) a[b] = open ? sq[NQ - 1][b] : 0.0f;
 #pragma unroll
-        for (int b = 0; b < BC; ++b) s[b] = __uint_as_float(dpp_get<0x138, 0xf>(__float_as_uint(a[b])));  // wave_shr:1
-        f = dpp_get<0x138, 0xf>(ends ? 1u : 0u);
-        seg_scan_step<BC, 0x111, 0xf>(s, f);  // row_shr:1
-        seg_scan_step<BC, 0x112, 0xf>(s, f);  // row_shr:2
-        seg_scan_step<BC, 0x114, 0xf>(s, f);  // row_shr:4
-        seg_scan_step<BC, 0x118, 0xf>(s, f);  // row_shr:8
-        seg_scan_step<BC, 0x142, 0xa>(s, f);  // row_bcast:15
-        seg_scan_step<BC, 0x143, 0xc>(s, f);  // row_bcast:31
+        for (int b = 0; b < BC; ++b) s[b] = dpp_getf<0x138, 0xf, true>(a[b], 0.0f);  // wave_shr:1
+        float m = dpp_getf<0x138, 0xf, false>(ends ? 0.0f : 1.0f, 1.0f);
+        // The scan stops as soon as no lane that uses its carry in (one that
+        // ends a row, and lane 63 for the wave's carry out) still lacks the
+        // start of its segment: after the steps of shift 1, 2, 4 a lane has
+        // the 7 lanes before it (within its row of 16), after shift 8 its
+        // row, after the broadcasts the wave.  Rows of a few quads (the
+        // transpose's voxel rows) rarely need more than the first three
+        // steps; the later steps would change no lane that is used.
+        seg_scan_step<BC, 0x111>(s, m);  // row_shr:1
+        seg_scan_step<BC, 0x112>(s, m);  // row_shr:2
+        seg_scan_step<BC, 0x114>(s, m);  // row_shr:4
+        const bool uses = ends || lane == 63u;
+        if (__builtin_amdgcn_ballot_w64(uses && m != 0.0f && lane >= 8u)) {
+            seg_scan_step<BC, 0x118>(s, m);  // row_shr:8
+            if (__builtin_amdgcn_ballot_w64(uses && m != 0.0f && lane >= 16u)) {
+                seg_scan_step<BC, 0x142>(s, m);  // row_bcast:15
+                seg_scan_step<BC, 0x143>(s, m);  // row_bcast:31
+            }
+        }
+        const bool f = m == 0.0f;  // a row ends in an earlier lane of this wave
         if (lane == 63) {  // the wave's carry out and whether it ends a row
 #pragma unroll
             for (int b = 0; b < BC; ++b) s_wt[wv][b] = ends ? a[b] : s[b] + a[b];
@@ -608,14 +699,12 @@ __global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t ns
             float c[BC];
 #pragma unroll
             for (int b = 0; b < BC; ++b) c[b] = s[b];
-            if (!f) {  // the row started before this wave: the earlier waves' carries
+            if (!f) {  // the row started before this wave: the earlier waves' carries, nearest first,
+                       // back to the nearest wave that ends a row (its carry out included)
+                for (int u = (int)wv - 1; u >= 0; --u) {
 #pragma unroll
-                for (int u = 0; u < NW - 1; ++u) {
-                    if (u < (int)wv) {
-                        const uint32_t hu = s_wh[u];
-#pragma unroll
-                        for (int b = 0; b < BC; ++b) c[b] = hu ? s_wt[u][b] + s[b] : c[b] + s_wt[u][b];
-                    }
+                    for (int b = 0; b < BC; ++b) c[b] += s_wt[u][b];
+                    if (s_wh[u]) break;
                 }
             }
 #pragma unroll
