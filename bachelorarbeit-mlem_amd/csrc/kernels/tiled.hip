@@ -45,8 +45,9 @@
 // whatever the tiles' fill (a work list built on the host from the tile
 // sizes).  Row ends are where the row key changes.  A row split between
 // work-items is finished by the work-item holding its last entry, which adds
-// the carries of the work-items before it (nearest first, back to the
-// previous row end) from LDS; so each row that has entries in a unit is
+// the carries of the work-items before it (back to the previous row end: a
+// segmented scan across the wave in registers, then the waves' totals from
+// LDS); so each row that has entries in a unit is
 // updated by exactly one work-item per unit, and the LDS accumulator of the
 // block's rows needs no atomics (a read and a write, ordered by the unit's
 // barriers).  The loads of the next unit (and, for a new tile, its x
@@ -206,7 +207,7 @@ __device__ __forceinline__ float group_val(const f32x4 &h0, const f32x4 &h1, con
 
 template <int W, int VT, int BC>
 constexpr int min_waves() {
-    return BC == 1 && VT == VAL_F32 ? (W == 1024 ? 8 : 7) : 1;
+    return BC == 1 && VT == VAL_F32 ? 7 : 1;
 }
 
 // Units whose loads are in flight per workgroup (the kernel takes any depth):
@@ -215,6 +216,62 @@ template <int W, int VT, int BC>
 constexpr int pipeline_depth() {
     return 1;
 }
+
+// Lane i of the wave gets v from lane i - (shift given by the DPP control
+// CTRL) of the rows in ROWM, 0 where there is none.
+template <int CTRL, int ROWM>
+__device__ __forceinline__ uint32_t dpp_get(uint32_t v) {
+    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWM, 0xf, false);
+}
+
+// One step of a segmented inclusive scan across the wave: (v, f) combined
+// with the running value of an earlier lane, where f marks that a segment
+// starts at or before this lane (the earlier part then does not count).
+// The same for floats, `none` where there is no such lane.
+template <int CTRL, int ROWM, bool ZERO>
+__device__ __forceinline__ float dpp_getf(float v, float none) {
+    return __uint_as_float(
+        (uint32_t)__builtin_amdgcn_update_dpp((int)__float_as_uint(none), (int)__float_as_uint(v), CTRL, ROWM, 0xf, ZERO));
+}
+
+// v += (v of the lane DPP control DSTR reads) * m, and then m *= (m of
+// that lane), in the rows ROWS: one v_fmac_f32_dpp per value (the compiler
+// does not fold a DPP read into an FMA).  Lanes without a source lane, and
+// rows outside ROWS, are not written (bound_ctrl off): v and m stay.  The
+// s_nop before each gives the DPP read of a register that the previous
+// VALU instruction may have written its two wait states.
+#define MLEM_SEG_STEP_ASM(DSTR, ROWS)                                                                        \
+    do {                                                                                                     \
+        _Pragma("unroll") for (int b = 0; b < BC; ++b) asm volatile(                                         \
+            "s_nop 1\n\tv_fmac_f32_dpp %0, %0, %1 " DSTR " row_mask:" ROWS " bank_mask:0xf"                     \
+            : "+v"(v[b])                                                                                     \
+            : "v"(m));                                                                                       \
+        asm volatile("s_nop 1\n\tv_mul_f32_dpp %0, %0, %0 " DSTR " row_mask:" ROWS " bank_mask:0xf" : "+v"(m)); \
+    } while (0)
+
+// One step of a segmented inclusive scan across the wave: v combined with
+// the running value of an earlier lane while this lane's segment is still
+// open, m = 1.0 (no segment starts at or before the lane in the range seen
+// so far) or 0.0.  v + nv * m is exact for m in {0, 1}, and the masks
+// combine by a product.
+template <int BC, int CTRL>
+__device__ __forceinline__ void seg_scan_step(float (&v)[BC], float &m) {
+    if constexpr (CTRL == 0x111)
+        MLEM_SEG_STEP_ASM("row_shr:1", "0xf");
+    else if constexpr (CTRL == 0x112)
+        MLEM_SEG_STEP_ASM("row_shr:2", "0xf");
+    else if constexpr (CTRL == 0x114)
+        MLEM_SEG_STEP_ASM("row_shr:4", "0xf");
+    else if constexpr (CTRL == 0x118)
+        MLEM_SEG_STEP_ASM("row_shr:8", "0xf");
+    else if constexpr (CTRL == 0x142)
+        MLEM_SEG_STEP_ASM("row_bcast:15", "0xa");
+    else {
+        static_assert(CTRL == 0x143, "scan steps: row_shr 1, 2, 4, 8, row_bcast 15, 31");
+        MLEM_SEG_STEP_ASM("row_bcast:31", "0xc");
+    }
+}
+#undef MLEM_SEG_STEP_ASM
 
 // One right-hand side.  wl[w] = {row block, first tile (index into tiles),
 // first unit in it, units}; tiles[i] = {segment, entries, entry offset lo,
@@ -233,15 +290,16 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
     static_assert(BC == 1, "several right-hand sides: stream_spmv_kernel");
     static_assert(XF % 4 == 0, "x segments move in 16-byte pieces");
     extern __shared__ __attribute__((aligned(16))) float smem[];
+    constexpr int NW = kWG / 64;        // waves
     __shared__ uint32_t s_k0[kWG + 1];  // first key of every work-item (+ sentinel)
-    __shared__ uint32_t s_hf[kWG];      // work-item saw a row end (one right-hand side)
+    __shared__ float s_wt[NW];          // carry out of every wave
+    __shared__ uint32_t s_wh[NW];       // the wave ends a row
     constexpr uint32_t kDummy = BC == 1 ? kWG : 0;               // dummy rows (one right-hand side only)
     float *xs = smem;                                            // [(W + 4) * BC]; slot W = 0 for padding
     float *acc = xs + (W + 4) * BC;                              // [(R + kDummy) * BC]: rows, then dummies
-    float *s_cf = acc + (R + kDummy) * BC;                       // [kWG * BC] carries
-    uint4 *tl = reinterpret_cast<uint4 *>(s_cf + kWG * BC);      // [span]
+    uint4 *tl = reinterpret_cast<uint4 *>(acc + (R + kDummy) * BC);  // [span]
 
-    const uint32_t tid = threadIdx.x;
+    const uint32_t tid = threadIdx.x, lane = tid & 63u, wv = tid >> 6;
     const uint32_t run = order ? order[blockIdx.x] : blockIdx.x;
     const uint4 w = wl[run];
     for (uint32_t i = tid; i < R * BC; i += kWG) acc[i] = 0.0f;
@@ -386,19 +444,39 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
 #pragma unroll
                 for (int j = 1; j < 8; ++j) ends |= step[j];
                 if constexpr (D > 1) issue_next();
-                // the part of the last row that continues in the next work-item
-                s_cf[tid] = a[0];
-                s_hf[tid] = ends;
+                // a[0] is now the part of the last row that continues in the
+                // next work-item.  The row of entry 0 may have started in
+                // earlier work-items: when it ends here, it gets their carries
+                // (back to the last one with a row end, whose carry belongs to
+                // this row too): a segmented scan over the carries shifted by
+                // one work-item, across the wave in registers (stopping once
+                // no lane that uses its result needs more), then across the
+                // waves from their totals in LDS.
+                float sc[1] = {dpp_getf<0x138, 0xf, true>(a[0], 0.0f)};  // wave_shr:1
+                float mo = dpp_getf<0x138, 0xf, false>(ends ? 0.0f : 1.0f, 1.0f);
+                seg_scan_step<1, 0x111>(sc, mo);  // row_shr:1
+                seg_scan_step<1, 0x112>(sc, mo);  // row_shr:2
+                seg_scan_step<1, 0x114>(sc, mo);  // row_shr:4
+                const bool uses = ends || lane == 63u;
+                if (__builtin_amdgcn_ballot_w64(uses && mo != 0.0f && lane >= 8u)) {
+                    seg_scan_step<1, 0x118>(sc, mo);  // row_shr:8
+                    if (__builtin_amdgcn_ballot_w64(uses && mo != 0.0f && lane >= 16u)) {
+                        seg_scan_step<1, 0x142>(sc, mo);  // row_bcast:15
+                        seg_scan_step<1, 0x143>(sc, mo);  // row_bcast:31
+                    }
+                }
+                if (lane == 63u) {
+                    s_wt[wv] = ends ? a[0] : sc[0] + a[0];
+                    s_wh[wv] = (mo == 0.0f || ends) ? 1u : 0u;
+                }
                 __syncthreads();
-                // The row of entry 0 may have started in earlier work-items:
-                // when it ends here, add their carries (nearest first, back to
-                // the last one with a row end, whose carry belongs to this row
-                // too).
                 if (ends) {
-                    float cin = 0.0f;
-                    for (int jj = (int)tid - 1; jj >= 0; --jj) {
-                        cin += s_cf[jj];
-                        if (s_hf[jj]) break;
+                    float cin = sc[0];
+                    if (mo != 0.0f) {  // the row started before this wave: nearest wave first
+                        for (int u = (int)wv - 1; u >= 0; --u) {
+                            cin += s_wt[u];
+                            if (s_wh[u]) break;
+                        }
                     }
                     acc[key0] += cin;
                 }
@@ -423,61 +501,6 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
 // wl[w] = {row block, first window, windows}; win[i] = {first group,
 // groups | split << 16 | slot of A << 31, segment of A, segment of B}.
 
-// Lane i of the wave gets v from lane i - (shift given by the DPP control
-// CTRL) of the rows in ROWM, 0 where there is none.
-template <int CTRL, int ROWM>
-__device__ __forceinline__ uint32_t dpp_get(uint32_t v) {
-    return (uint32_t)__builtin_amdgcn_update_dpp(0, (int)v, CTRL, ROWM, 0xf, false);
-}
-
-// One step of a segmented inclusive scan across the wave: (v, f) combined
-// with the running value of an earlier lane, where f marks that a segment
-// starts at or before this lane (the earlier part then does not count).
-// The same for floats, `none` where there is no such lane.
-template <int CTRL, int ROWM, bool ZERO>
-__device__ __forceinline__ float dpp_getf(float v, float none) {
-    return __uint_as_float(
-        (uint32_t)__builtin_amdgcn_update_dpp((int)__float_as_uint(none), (int)__float_as_uint(v), CTRL, ROWM, 0xf, ZERO));
-}
-
-// v += (v of the lane DPP control DSTR reads) * m, and then m *= (m of
-// that lane), in the rows ROWS: one v_fmac_f32_dpp per value (the compiler
-// does not fold a DPP read into an FMA).  Lanes without a source lane, and
-// rows outside ROWS, are not written (bound_ctrl off): v and m stay.  The
-// s_nop before each gives the DPP read of a register that the previous
-// VALU instruction may have written its two wait states.
-#define MLEM_SEG_STEP_ASM(DSTR, ROWS)                                                                        \
-    do {                                                                                                     \
-        _Pragma("unroll") for (int b = 0; b < BC; ++b) asm volatile(                                         \
-            "s_nop 1\n\tv_fmac_f32_dpp %0, %0, %1 " DSTR " row_mask:" ROWS " bank_mask:0xf"                     \
-            : "+v"(v[b])                                                                                     \
-            : "v"(m));                                                                                       \
-        asm volatile("s_nop 1\n\tv_mul_f32_dpp %0, %0, %0 " DSTR " row_mask:" ROWS " bank_mask:0xf" : "+v"(m)); \
-    } while (0)
-
-// One step of a segmented inclusive scan across the wave: v combined with
-// the running value of an earlier lane while this lane's segment is still
-// open, m = 1.0 (no segment starts at or before the lane in the range seen
-// so far) or 0.0.  v + nv * m is exact for m in {0, 1}, and the masks
-// combine by a product.
-template <int BC, int CTRL>
-__device__ __forceinline__ void seg_scan_step(float (&v)[BC], float &m) {
-    if constexpr (CTRL == 0x111)
-        MLEM_SEG_STEP_ASM("row_shr:1", "0xf");
-    else if constexpr (CTRL == 0x112)
-        MLEM_SEG_STEP_ASM("row_shr:2", "0xf");
-    else if constexpr (CTRL == 0x114)
-        MLEM_SEG_STEP_ASM("row_shr:4", "0xf");
-    else if constexpr (CTRL == 0x118)
-        MLEM_SEG_STEP_ASM("row_shr:8", "0xf");
-    else if constexpr (CTRL == 0x142)
-        MLEM_SEG_STEP_ASM("row_bcast:15", "0xa");
-    else {
-        static_assert(CTRL == 0x143, "scan steps: row_shr 1, 2, 4, 8, row_bcast 15, 31");
-        MLEM_SEG_STEP_ASM("row_bcast:31", "0xc");
-    }
-}
-#undef MLEM_SEG_STEP_ASM
 
 template <int W, int VT, int BC, int NT>
 __global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t nseg,
@@ -1153,7 +1176,7 @@ inline uint32_t blocks_for(uint64_t n, uint32_t per) { return (uint32_t)((n + pe
 constexpr size_t align256(size_t b) { return (b + 255) & ~size_t(255); }
 
 size_t lds_bytes(uint32_t W, uint32_t R, uint32_t span, uint32_t BC) {
-    return ((size_t)W + 4 + R + (BC == 1 ? kWG : 0) + kWG) * BC * 4 + (size_t)span * 16;
+    return ((size_t)W + 4 + R + (BC == 1 ? kWG : 0)) * BC * 4 + (size_t)span * 16;
 }
 
 // stream_spmv_kernel: two x slots, the accumulator, the carries.
