@@ -98,11 +98,14 @@ int get_partition(mlem_matrix *m, int which, uint32_t secsize, const Partition *
 
 // Tiles of A (which 0) or A^T (which 1) with segment width W and R rows per
 // block, built on first use.  MLEM_ERR_UNSUPPORTED (and *flags) when the
-// matrix does not allow them (a row not sorted by column).
+// matrix does not allow them (a row not sorted by column).  Quad tiles (the
+// batched layouts) of a matrix with a row order are built from A with its
+// rows in that order, and from that matrix's transpose for A^T.
 int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bool quad, const Tiled **out,
               uint32_t *flags) {
     std::lock_guard<std::mutex> lk(m->mu);
-    auto key = std::make_tuple(which, W, R, vt, quad);
+    const bool ordered = quad && m->has_order;
+    auto key = std::make_tuple(which, W, R, vt, quad, ordered);
     *flags = 0;
     auto it = m->tiles.find(key);
     if (it != m->tiles.end()) {
@@ -118,11 +121,50 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bo
         return set_err(MLEM_ERR_UNSUPPORTED, "tiling refused (flags %u)", *flags);
     }
     auto t = std::make_unique<Tiled>();
+    t->ordered = ordered;
     const uint32_t nrow = which == 0 ? m->rows : m->cols, ncol = which == 0 ? m->cols : m->rows;
     t->plan = tile_plan(nrow, ncol, m->nnz, W, R, vt, quad);
     const uint64_t *rp = which == 0 ? m->rowptr.as<uint64_t>() : m->rowptrT.as<uint64_t>();
     const uint32_t *ci = which == 0 ? m->col.as<uint32_t>() : m->colT.as<uint32_t>();
     const float *va = which == 0 ? m->val.as<float>() : m->valT.as<float>();
+    size_t free_b = 0, total_b = 0;
+    const size_t margin = (size_t)1 << 30;
+    uint32_t nr = 0;
+    // A in the row order (A'), and for A^T its transpose: A^T's columns are
+    // then the places of A's rows.  Freed once the tiles are filled.
+    DevBuf o_rowptr, o_col, o_val, tcp, trow, o_valT;
+    const uint64_t *arp = m->rowptr.as<uint64_t>();  // the A whose codes A^T's 16-bit tiles take
+    const uint32_t *aci = m->col.as<uint32_t>();
+    const float *ava = m->val.as<float>();
+    if (ordered) {
+        HIPCHK(o_rowptr.alloc(sizeof(uint64_t) * ((size_t)m->rows + 1)));
+        HIPCHK(o_col.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, m->nnz)));
+        HIPCHK(o_val.alloc(sizeof(float) * std::max<uint64_t>(1, m->nnz)));
+        HIPCHK(reorder_rows(m->rows, m->nnz, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(), m->val.as<float>(),
+                            m->row_order.as<uint32_t>(), o_rowptr.as<uint64_t>(), o_col.as<uint32_t>(),
+                            o_val.as<float>(), nullptr));
+        arp = o_rowptr.as<uint64_t>();
+        aci = o_col.as<uint32_t>();
+        ava = o_val.as<float>();
+        if (which == 0) {
+            rp = arp;
+            ci = aci;
+            va = ava;
+        } else {
+            HIPCHK(tcp.alloc(sizeof(uint64_t) * ((size_t)m->cols + 1)));
+            HIPCHK(trow.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, m->nnz)));
+            rp = tcp.as<uint64_t>();
+            ci = trow.as<uint32_t>();
+            if (vt == VAL_F32) {
+                HIPCHK(o_valT.alloc(sizeof(float) * std::max<uint64_t>(1, m->nnz)));
+                HIPCHK(hipMemGetInfo(&free_b, &total_b));
+                HIPCHK(transpose_in_ranges(m->rows, m->cols, m->nnz, arp, aci, ava, tcp.as<uint64_t>(),
+                                           trow.as<uint32_t>(), o_valT.as<float>(),
+                                           free_b > margin ? free_b - margin : 0, &nr, nullptr));
+                va = o_valT.as<float>();
+            }
+        }
+    }
     // 16-bit values on A^T: the codes of A's quantization, transposed, stored
     // as they are, so that the forward and back projections use one quantized
     // matrix (and the sensitivity is its column sums).
@@ -132,23 +174,24 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bo
         HIPCHK(scl.alloc(sizeof(float) * std::max<size_t>(1, m->rows)));
         HIPCHK(codes.alloc(sizeof(float) * std::max<uint64_t>(1, m->nnz)));
         HIPCHK(q.alloc(sizeof(uint16_t) * std::max<uint64_t>(1, m->nnz)));
-        HIPCHK(launch_quantize_codes(m->rowptr.as<uint64_t>(), m->val.as<float>(), m->rows, m->nnz, vt,
-                                     scl.as<float>(), codes.as<float>(), q.as<uint16_t>(), nullptr));
+        HIPCHK(launch_quantize_codes(arp, ava, m->rows, m->nnz, vt, scl.as<float>(), codes.as<float>(),
+                                     q.as<uint16_t>(), nullptr));
         q.reset();
-        DevBuf tcp, trow;
+        o_val.reset();
+        DevBuf tcp_l, trow_l;
+        if (!ordered) {  // the resident transpose holds the structure; these are scratch
+            HIPCHK(tcp_l.alloc(sizeof(uint64_t) * ((size_t)m->cols + 1)));
+            HIPCHK(trow_l.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, m->nnz)));
+        }
+        uint64_t *tc = ordered ? tcp.as<uint64_t>() : tcp_l.as<uint64_t>();
+        uint32_t *tr = ordered ? trow.as<uint32_t>() : trow_l.as<uint32_t>();
         HIPCHK(codesT.alloc(sizeof(float) * std::max<uint64_t>(1, m->nnz)));
-        HIPCHK(tcp.alloc(sizeof(uint64_t) * ((size_t)m->cols + 1)));
-        HIPCHK(trow.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, m->nnz)));
-        size_t free_b = 0, total_b = 0;
         HIPCHK(hipMemGetInfo(&free_b, &total_b));
-        const size_t margin = (size_t)1 << 30;
-        uint32_t nr = 0;
-        HIPCHK(transpose_in_ranges(m->rows, m->cols, m->nnz, m->rowptr.as<uint64_t>(), m->col.as<uint32_t>(),
-                                   codes.as<float>(), tcp.as<uint64_t>(), trow.as<uint32_t>(), codesT.as<float>(),
-                                   free_b > margin ? free_b - margin : 0, &nr, nullptr));
+        HIPCHK(transpose_in_ranges(m->rows, m->cols, m->nnz, arp, aci, codes.as<float>(), tc, tr,
+                                   codesT.as<float>(), free_b > margin ? free_b - margin : 0, &nr, nullptr));
         HIPCHK(t->sens.alloc(sizeof(float) * std::max<size_t>(1, m->cols)));
-        HIPCHK(launch_colsum_weighted_f64(m->rowptrT.as<uint64_t>(), trow.as<uint32_t>(), codesT.as<float>(),
-                                          scl.as<float>(), m->cols, nullptr, t->sens.as<float>(), nullptr));
+        HIPCHK(launch_colsum_weighted_f64(m->rowptrT.as<uint64_t>(), tr, codesT.as<float>(), scl.as<float>(),
+                                          m->cols, nullptr, t->sens.as<float>(), nullptr));
         HIPCHK(hipStreamSynchronize(nullptr));
         va = codesT.as<float>();
         t->plan.unit_scale = true;
@@ -172,8 +215,8 @@ int get_tiled(mlem_matrix *m, int which, uint32_t W, uint32_t R, uint32_t vt, bo
     }
     if (fl & 8u) return set_err(MLEM_ERR_INVALID_ARGUMENT, "uf16 values need a matrix without negative values");
     if (fl) return set_err(MLEM_ERR_FORMAT, "tiling (width %u, rows %u): layout check failed (flags %u)", W, R, fl);
-    progress("tiles of %s: %u columns x %u rows, %s, values %u: %llu entries stored for %llu nonzeros (+%.1f%%)",
-             which ? "A^T" : "A", W, R, quad ? "quad groups" : "row steps", vt,
+    progress("tiles of %s: %u columns x %u rows, %s%s, values %u: %llu entries stored for %llu nonzeros (+%.1f%%)",
+             which ? "A^T" : "A", W, R, quad ? "quad groups" : "row steps", ordered ? " in the row order" : "", vt,
              (unsigned long long)(t->plan.cap - 8), (unsigned long long)m->nnz,
              m->nnz ? 100.0 * ((double)(t->plan.cap - 8) / (double)m->nnz - 1.0) : 0.0);
     *out = t.get();
@@ -930,6 +973,35 @@ int mlem_matrix_build_transpose_limited(mlem_matrix *m, uint64_t scratch_bytes, 
         it = std::get<0>(it->first) == 1 ? m->tiles.erase(it) : std::next(it);
     for (auto it = m->tile_refused.begin(); it != m->tile_refused.end();)
         it = std::get<0>(it->first) == 1 ? m->tile_refused.erase(it) : std::next(it);
+    return MLEM_OK;
+}
+
+int mlem_matrix_set_row_order(mlem_matrix *m, const uint32_t *order) {
+    ARGCHK(m, "mlem_matrix_set_row_order: null matrix");
+    ARGCHK(!m->tiles_only, "mlem_matrix_set_row_order: a tiles-only matrix has no CSR to reorder");
+    ARGCHK(m->users.load() == 0, "mlem_matrix_set_row_order: %u reconstructions or batches still use the matrix",
+           m->users.load());
+    HIPCHK(hipSetDevice(m->device));
+    DevBuf buf;
+    if (order) {
+        std::vector<uint8_t> seen(m->rows, 0);
+        for (uint32_t i = 0; i < m->rows; ++i) {
+            ARGCHK(order[i] < m->rows && !seen[order[i]],
+                   "mlem_matrix_set_row_order: not a permutation of the %u rows (entry %u is %u)", m->rows, i,
+                   order[i]);
+            seen[order[i]] = 1;
+        }
+        HIPCHK(buf.alloc(sizeof(uint32_t) * std::max<uint32_t>(1, m->rows)));
+        if (m->rows) HIPCHK(hipMemcpy(buf.p, order, sizeof(uint32_t) * m->rows, hipMemcpyHostToDevice));
+    }
+    std::lock_guard<std::mutex> lk(m->mu);
+    for (auto it = m->tiles.begin(); it != m->tiles.end();)  // tiles built in the previous order
+        it = std::get<5>(it->first) ? m->tiles.erase(it) : std::next(it);
+    for (auto it = m->tile_refused.begin(); it != m->tile_refused.end();)
+        it = std::get<5>(it->first) ? m->tile_refused.erase(it) : std::next(it);
+    std::swap(m->row_order.p, buf.p);
+    std::swap(m->row_order.bytes, buf.bytes);
+    m->has_order = order != nullptr;
     return MLEM_OK;
 }
 

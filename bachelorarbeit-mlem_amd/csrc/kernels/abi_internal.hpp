@@ -115,6 +115,7 @@ struct Tiled {
     mlem::abi::DevBuf mem;
     mlem::TiledMatrix view;
     mlem::abi::DevBuf sens;  // A^T tiles of 16-bit values: the quantized matrix's column sums (f32)
+    bool ordered = false;    // built with A's rows in the matrix's row order (mlem_matrix_set_row_order)
     std::map<uint32_t, std::unique_ptr<TileWorkDev>> works;  // by target workgroup count
 };
 
@@ -131,10 +132,15 @@ struct mlem_matrix {
     bool tiles_only = false;
     uint32_t tile_W = 0, tile_R = 0, tile_vt = 0, tile_bc = 1;  // tile_bc: right-hand sides per pass
     mlem::abi::DevBuf sens32, sens64;
+    // Row order for the batched (quad) tiles: row_order[i] = place of row i
+    // of A (mlem_matrix_set_row_order); has_order false: A's own order.
+    mlem::abi::DevBuf row_order;
+    bool has_order = false;
     std::mutex mu;
     std::map<std::pair<int, uint32_t>, std::unique_ptr<Partition>> parts;  // (0: A / 1: A^T, secsize)
-    // Tiles keyed by (0: A / 1: A^T, width W, rows R, value type, quad groups).
-    using TileKey = std::tuple<int, uint32_t, uint32_t, uint32_t, bool>;
+    // Tiles keyed by (0: A / 1: A^T, width W, rows R, value type, quad groups,
+    // rows in the row order).
+    using TileKey = std::tuple<int, uint32_t, uint32_t, uint32_t, bool, bool>;
     std::map<TileKey, std::unique_ptr<Tiled>> tiles;
     std::map<TileKey, uint32_t> tile_refused;  // build flags of refused tilings
     // Reconstructions and batches alive on this matrix: they hold its arrays,
@@ -144,7 +150,7 @@ struct mlem_matrix {
 
     uint64_t device_bytes() const {
         uint64_t b = rowptr.bytes + col.bytes + val.bytes + rowptrT.bytes + colT.bytes + valT.bytes + sens32.bytes +
-                     sens64.bytes;
+                     sens64.bytes + row_order.bytes;
         for (const auto &t : tiles) b += t.second->mem.bytes;
         return b;
     }
@@ -175,6 +181,7 @@ struct mlem_batch {
     mlem::abi::DevBuf sens, active, stage, part_fp, part_bp;
     const Tiled *tfp = nullptr, *tbp = nullptr;
     const TileWorkDev *wfp = nullptr, *wbp = nullptr;
+    const uint32_t *row_order = nullptr;  // tiles in the matrix's row order: where each count of g goes
     std::vector<uint32_t> active_host;
     // One iteration captured as a graph (use_graph), for the passes that
     // held an active slot when it was captured.

@@ -62,7 +62,10 @@ int batch_set_async(mlem_batch *b, uint32_t slot, const int32_t *g, const float 
     // One staging vector serves both copies: the stream orders the second
     // copy after the first scatter.
     HIPCHK(hipMemcpyAsync(b->stage.p, g, sizeof(int32_t) * m->rows, hipMemcpyHostToDevice, s));
-    HIPCHK(launch_strided_put(bp.g.as<int32_t>() + q, b->BC, b->stage.p, m->rows, s));
+    if (b->row_order)  // the tiles hold A's rows in the matrix's row order: so do the counts
+        HIPCHK(launch_ordered_put(bp.g.as<int32_t>() + q, b->BC, b->stage.p, b->row_order, m->rows, s));
+    else
+        HIPCHK(launch_strided_put(bp.g.as<int32_t>() + q, b->BC, b->stage.p, m->rows, s));
     HIPCHK(hipMemcpyAsync(b->stage.p, f0, sizeof(float) * m->cols, hipMemcpyHostToDevice, s));
     HIPCHK(launch_strided_put(bp.f.as<float>() + q, b->BC, b->stage.p, m->cols, s));
     HIPCHK(launch_fill_u32(b->active.as<uint32_t>() + slot, 1u, 1, s));
@@ -295,6 +298,9 @@ int mlem_batch_create(mlem_matrix *m, uint32_t slots, const mlem_batch_opts *opt
     const uint32_t groups = o.tile_groups ? o.tile_groups : (BC == 1 ? 24u : 8u) * (uint32_t)std::max(1, cus);
     if (int st = setup_tiled(m, 0, W, R, o.values, BC, groups, &b->tfp, &b->wfp, &b->part_fp)) return st;
     if (int st = setup_tiled(m, 1, W, R, o.values, BC, groups, &b->tbp, &b->wbp, &b->part_bp)) return st;
+    if (b->tfp->ordered != b->tbp->ordered)  // a row order set from another thread between the two builds
+        return set_err(MLEM_ERR_INVALID_ARGUMENT, "mlem_batch_create: the row order changed while the tiles were built");
+    if (b->tfp->ordered) b->row_order = m->row_order.as<uint32_t>();
     const size_t per = (size_t)BC * b->passes;
     b->pass.resize(b->passes);  // passes are allocated when first used (batch_set_async)
     HIPCHK(b->sens.alloc(sizeof(float) * m->cols));

@@ -70,6 +70,15 @@ hipError_t launch_aos_split(const void *aos, uint64_t n, uint64_t first, uint32_
                             uint32_t *bad, hipStream_t s);
 // 4-byte elements: dst[i*stride] = src[i] / dst[i] = src[i*stride], i < n.
 hipError_t launch_strided_put(void *dst, uint32_t stride, const void *src, uint64_t n, hipStream_t s);
+// 4-byte elements in row order: dst[order[i]*stride] = src[i], i < n.
+hipError_t launch_ordered_put(void *dst, uint32_t stride, const void *src, const uint32_t *order, uint32_t n,
+                              hipStream_t s);
+// The matrix with row i of A in place order[i] (a permutation of the rows):
+// rowptr_out u64[rows+1], col_out / val_out [nnz], each row's entries as in
+// A.  Allocates its scratch and waits for the stream (reorder.hip).
+hipError_t reorder_rows(uint32_t rows, uint64_t nnz, const uint64_t *rowptr, const uint32_t *col, const float *val,
+                        const uint32_t *order, uint64_t *rowptr_out, uint32_t *col_out, float *val_out,
+                        hipStream_t s);
 hipError_t launch_strided_get(void *dst, const void *src, uint32_t stride, uint64_t n, hipStream_t s);
 // dst = src (16-byte aligned), non-temporal streaming copy.
 hipError_t launch_copy(float *dst, const float *src, uint64_t n, hipStream_t s);

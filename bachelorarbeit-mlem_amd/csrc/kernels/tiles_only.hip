@@ -263,8 +263,8 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
         HIPCHK(tT->sens.alloc(sizeof(float) * cols));
         HIPCHK(hipMemcpy(tT->sens.p, m->sens32.p, sizeof(float) * cols, hipMemcpyDeviceToDevice));
     }
-    m->tiles.emplace(std::make_tuple(0, W, R, values, quad), std::move(tA));
-    m->tiles.emplace(std::make_tuple(1, W, R, values, quad), std::move(tT));
+    m->tiles.emplace(std::make_tuple(0, W, R, values, quad, false), std::move(tA));
+    m->tiles.emplace(std::make_tuple(1, W, R, values, quad, false), std::move(tT));
     *out = m.release();
     return MLEM_OK;
 }

@@ -157,6 +157,7 @@ def host_lib() -> C.CDLL:
             "mlem_initial_image": (cint, [vp, u32, vp, u32, vp, C.POINTER(f32)]),
             "mlem_transpose": (cint, [u32, u32, u64, vp, vp, vp, vp, vp, vp, cint]),
             "mlem_partition": (cint, [vp, u64, u32, u32, vp, vp, vp, vp, vp]),
+            "mlem_lor_order": (cint, [vp, vp, vp, u32, u32, vp, cint, vp]),
             "mlem_geometry_rows": (u32, [C.POINTER(Geometry)]),
             "mlem_geometry_cols": (u32, [C.POINTER(Geometry)]),
             "mlem_gen_system_matrix": (cint, [C.POINTER(Geometry), u64, u32, u32, cint, C.POINTER(CsrHost)]),
@@ -289,6 +290,16 @@ def partition(rowptr: np.ndarray, nnz: int, rows: int, parts: int):
     _hcheck(host_lib().mlem_partition(_ptr(rowptr), nnz, rows, parts, _ptr(seg), _ptr(srows), _ptr(snnz),
                                       _ptr(snnz64), _ptr(off)))
     return seg, srows, snnz, snnz64, off
+
+
+def lor_order(A: Csr, grid, threads: int = 0) -> np.ndarray:
+    """Row order for the batched layouts (mlem_lor_order): order[i] = place of LOR i, from each row's transaxial
+    line fitted to its voxels on the image grid (nx, ny, nz)."""
+    g = np.ascontiguousarray(grid, dtype=np.uint32)
+    order = np.empty(A.rows, np.uint32)
+    _hcheck(host_lib().mlem_lor_order(_ptr(A.rowptr), _ptr(A.col), _ptr(A.val), A.rows, A.cols, _ptr(g), threads,
+                                      _ptr(order)))
+    return order
 
 
 def colsum_f32(A: Csr) -> np.ndarray:

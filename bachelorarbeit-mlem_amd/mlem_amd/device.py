@@ -72,6 +72,7 @@ def hip_lib() -> C.CDLL:
             "mlem_matrix_create": (cint, [cint, u32, u32, u64, vp, vp, vp, vp, vp, vp, P(vp)]),
             "mlem_matrix_build_transpose": (cint, [vp]),
             "mlem_matrix_build_transpose_limited": (cint, [vp, u64, P(u32)]),
+            "mlem_matrix_set_row_order": (cint, [vp, vp]),
             "mlem_matrix_create_tiled": (cint, [cint, u32, u32, u64, vp, vp, vp, u32, u32, u32, u32, u64, P(vp)]),
             "mlem_autotune": (cint, [vp, u32, u32, vp]),
             "mlem_matrix_load_csr4": (cint, [cint, C.c_char_p, cint, P(vp), vp]),
@@ -547,6 +548,17 @@ class ResidentMatrix:
         n = u32()
         _check(hip_lib().mlem_matrix_build_transpose_limited(self.h, scratch_bytes, C.byref(n)))
         return n.value
+
+    def set_row_order(self, order=None):
+        """Row order of the batched layouts (mlem_matrix_set_row_order): order[i] = place of row i (a LOR) for
+        batches created afterwards; None clears it."""
+        if order is None:
+            _check(hip_lib().mlem_matrix_set_row_order(self.h, None))
+            return
+        o = np.ascontiguousarray(order, dtype=np.uint32)
+        if o.shape != (self.rows,):
+            raise ValueError(f"row order of {o.shape[0] if o.ndim == 1 else o.shape} entries for {self.rows} rows")
+        _check(hip_lib().mlem_matrix_set_row_order(self.h, o.ctypes.data_as(vp)))
 
     def autotune(self, per_pass: int = 1, values: int = 0) -> dict:
         """The fastest blocked-kernel tiling for this matrix (mlem_autotune): tile_width, tile_rows, tile_groups

@@ -257,7 +257,7 @@ def run_batch(args, M, geo, A, g0, m, s32, s64, threads, label, sinos=None, sing
         "vs_baseline": None, "dtype": f"{args.values} values, f32 arithmetic", "data": "synthetic",
         "config": {"workload": label + f", batch of {B} sinograms", "batch": B, "per_pass": args.per_pass,
                    "values": args.values, "iterations_per_recon": args.recon_iters, "passes": info["passes"],
-                   "graph": bool(args.graph),
+                   "graph": bool(args.graph), "lor_order": bool(args.lor_order and single),
                    "rows": A.rows, "cols": A.cols, "nnz": A.nnz},
         "one_at_a_time_f32": ({"ms_per_iteration": round(t_single * 1e3, 4), "recon_s": round(single_recon_s, 3)}
                               if t_single else None),
@@ -330,6 +330,13 @@ def serial_fifo(M, reqs, arrivals, mats):
     return lat, time.perf_counter() - t0
 
 
+def set_lor_order(M, m, A, geo, threads):
+    """The batches' row order (mlem_matrix_set_row_order) from each LOR's transaxial line on the image grid."""
+    t = time.time()
+    m.set_row_order(M.lor_order(A, tuple(geo.grid), threads=threads))
+    log(f"row order of the batched layouts (LORs by transaxial line) {time.time() - t:.2f} s")
+
+
 def run_serve(args, M, threads):
     """Serving mode: reconstruction requests over several resident matrices, continuous batching
     (mlem_server_*).  Phase 1 submits every request at once (throughput, recon/s); phase 2 replays the mix with
@@ -342,6 +349,8 @@ def run_serve(args, M, threads):
         m = M.ResidentMatrix(A)
         m.build_transpose()
         s32, _ = m.sensitivity()
+        if args.lor_order:
+            set_lor_order(M, m, A, geo, threads)
         rng = np.random.default_rng(len(mats) + 5)
         counts = float(np.sum(g, dtype=np.float64))
         sinos = [g] + [M.gen_sinogram(A, M.gen_phantom(geo, seed=70 + k, n_shapes=4 + k, a_min=0.5, a_max=4.0),
@@ -408,7 +417,7 @@ def run_serve(args, M, threads):
         "config": {"workload": "served mix: " + " + ".join(d["label"] for d in mats), "matrices": names,
                    "requests": n, "iterations_per_request": iters_desc, "slots_per_matrix": args.serve_slots,
                    "per_pass": args.per_pass, "values": args.values, "graph": bool(args.graph),
-                   "hbm_used_bytes": info["used"]},
+                   "lor_order": bool(args.lor_order), "hbm_used_bytes": info["used"]},
         "latency_ms": {"arrivals": f"Poisson, {head['rate']:.3f} req/s ({head['load']} x backlog throughput)",
                        "p50": head["p50"], "p99": head["p99"], "max": head["max"],
                        "backlog_p50": pct(lat1, 50), "backlog_p99": pct(lat1, 99)},
@@ -452,6 +461,9 @@ def main():
     ap.add_argument("--values", default=None, choices=["f32", "bf16", "fp16", "uf16"],
                     help="stored matrix values (default: f32 for one reconstruction, uf16 for --batch and --serve)")
     ap.add_argument("--recon-iters", type=int, default=100, help="iterations per reconstruction (recon/s)")
+    ap.add_argument("--lor-order", type=int, default=1,
+                    help="batched and serving modes: 1 = the batches' tiles take A's rows (LORs) in the order of "
+                         "mlem_lor_order (fewer padded entries in A^T's tiles), 0 = as stored")
     ap.add_argument("--batch-check", type=int, default=8,
                     help="batched mode: sinograms reconstructed by the FP64 oracle (CPU baseline and checker) and "
                          "compared, for every value type, with their batched images")
@@ -520,6 +532,8 @@ def main():
     s32, s64 = m.sensitivity()
     log(f"FP64 sensitivity {time.time() - t:.2f} s")
     f0, init = M.initial_image(g, A.rows, s32)
+    if args.batch and args.lor_order and not tiles_only:
+        set_lor_order(M, m, A, geo, threads)
     if args.autotune and args.tile != 1 and not tiles_only:
         t = time.time()
         tune = m.autotune(per_pass=args.per_pass if args.batch else 1,

@@ -141,6 +141,18 @@ int mlem_partition(const uint64_t *rowptr, uint64_t nnz, uint32_t rows,
                    uint32_t *segment_nnzs, uint64_t *segment_nnzs64,
                    uint64_t *offsets);
 
+/* An order of the rows (LORs) of a system matrix for the batched layouts
+ * (mlem_matrix_set_row_order in mlem_hip.h): order[i] = place of row i.
+ * From the matrix and its image grid (nx, ny, nz with x fastest, as the
+ * columns of the csr4 scanner block's grid) alone: each row's weighted voxel
+ * moments give its line in the transaxial plane (angle phi, offset s) and
+ * its mean slice; rows sort by (10-degree block of phi, s, phi, slice), so
+ * that the LORs through one voxel take nearby places.  Deterministic for any
+ * `threads` (<= 0: OpenMP default).  No counterpart in the reference. */
+int mlem_lor_order(const uint64_t *rowptr, const uint32_t *col,
+                   const float *val, uint32_t rows, uint32_t cols,
+                   const uint32_t grid[3], int threads, uint32_t *order);
+
 /* ------------------------------------------------------------------------ */
 /* synthetic data (the reference's data came from external tools)           */
 /* ------------------------------------------------------------------------ */
