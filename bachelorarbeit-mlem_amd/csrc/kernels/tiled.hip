@@ -238,8 +238,11 @@ __device__ __forceinline__ float dpp_getf(float v, float none) {
 // that lane), in the rows ROWS: one v_fmac_f32_dpp per value (the compiler
 // does not fold a DPP read into an FMA).  Lanes without a source lane, and
 // rows outside ROWS, are not written (bound_ctrl off): v and m stay.  The
-// s_nop before each gives the DPP read of a register that the previous
-// VALU instruction may have written its two wait states.
+// compiler cannot see the DPP read inside the asm, so the s_nop 1 before
+// each gives it the two wait states a DPP read needs after a VALU write of
+// the register it reads.  The other DPP hazard, five wait states after a
+// VALU write of EXEC (v_cmpx), cannot arise: no code in this file writes
+// EXEC from the VALU (tests/test_kernel_asm.py checks the compiled file).
 #define MLEM_SEG_STEP_ASM(DSTR, ROWS)                                                                        \
     do {                                                                                                     \
         _Pragma("unroll") for (int b = 0; b < BC; ++b) asm volatile(                                         \
