@@ -14,6 +14,22 @@ using mlem_host::fail;
 
 extern "C" {
 
+// A parallel memcpy on the library's OpenMP threads (which follow
+// OMP_PROC_BIND / OMP_PLACES, unlike threads started elsewhere in a process
+// whose initial thread OpenMP has bound to one core): page-aligned slices.
+int mlem_host_copy(void *dst, const void *src, uint64_t bytes, int threads) {
+    if (bytes && (!dst || !src)) return fail(MLEM_ERR_INVALID_ARGUMENT, "mlem_host_copy: null argument");
+    if (bytes == 0) return MLEM_OK;
+    const int nt = threads > 0 ? threads : omp_get_max_threads();
+    const uint64_t per = ((bytes + (uint64_t)nt - 1) / (uint64_t)nt + 4095) & ~uint64_t(4095);  // >= 4096
+#pragma omp parallel for schedule(static) num_threads(nt)
+    for (int t = 0; t < nt; ++t) {
+        const uint64_t b = std::min<uint64_t>(bytes, per * (uint64_t)t), e = std::min<uint64_t>(bytes, per * (uint64_t)(t + 1));
+        if (b < e) std::memcpy(static_cast<uint8_t *>(dst) + b, static_cast<const uint8_t *>(src) + b, e - b);
+    }
+    return MLEM_OK;
+}
+
 // calcColumnSums (main.cu:558-580).  The reference reduces thread-private
 // float vectors with an OpenMP user-defined reduction, so its rounding
 // depends on the thread count; this is the one-thread order: each column

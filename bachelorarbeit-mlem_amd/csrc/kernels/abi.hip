@@ -35,34 +35,79 @@ int hip_err(hipError_t e, const char *what, const char *file, int line) {
     return set_err(st, "%s failed: %s (%s:%d)", what, hipGetErrorString(e), file, line);
 }
 
-// Host-side checks of a CSR before it reaches the GPU: a bad row pointer or
-// column index would make the kernels read outside their arrays.
+// Host-side checks of a CSR's row pointers before it reaches the GPU (a bad
+// row pointer would make the kernels read outside their arrays); its column
+// indices are checked on the device once uploaded (check_columns_device).
 int check_csr_host(const char *what, uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
                    const uint32_t *col) {
+    (void)cols;
+    (void)col;
     if (rowptr[0] != 0 || rowptr[rows] != nnz)
         return set_err(MLEM_ERR_INVALID_ARGUMENT, "%s: row pointers must run from 0 to nnz=%llu", what,
                        (unsigned long long)nnz);
     for (uint32_t r = 0; r < rows; ++r)
         if (rowptr[r + 1] < rowptr[r])
             return set_err(MLEM_ERR_INVALID_ARGUMENT, "%s: row pointers decrease at row %u", what, r);
-    unsigned nt = std::max(1u, std::min(16u, std::thread::hardware_concurrency()));
-    if (nnz < (1u << 20)) nt = 1;
-    std::vector<uint64_t> bad(nt, UINT64_MAX);
-    std::vector<std::thread> th;
-    for (unsigned t = 0; t < nt; ++t)
-        th.emplace_back([&, t] {
-            uint64_t b = nnz * t / nt, e = nnz * (t + 1) / nt;
-            for (uint64_t k = b; k < e; ++k)
-                if (col[k] >= cols) {
-                    bad[t] = k;
-                    break;
-                }
-        });
-    for (auto &x : th) x.join();
-    for (unsigned t = 0; t < nt; ++t)
-        if (bad[t] != UINT64_MAX)
-            return set_err(MLEM_ERR_INVALID_ARGUMENT, "%s: column index %u at nonzero %llu >= %u", what,
-                           col[bad[t]], (unsigned long long)bad[t], cols);
+    return MLEM_OK;
+}
+
+int check_columns_device(const char *what, const uint32_t *col, uint64_t nnz, uint32_t cols) {
+    if (nnz == 0) return MLEM_OK;
+    DevBuf first;
+    HIPCHK(first.alloc(sizeof(uint64_t)));
+    const uint64_t none = UINT64_MAX;
+    HIPCHK(hipMemcpy(first.p, &none, sizeof(none), hipMemcpyHostToDevice));
+    HIPCHK(launch_first_bad_column(col, nnz, cols, first.as<uint64_t>(), nullptr));
+    uint64_t k = none;
+    HIPCHK(hipMemcpy(&k, first.p, sizeof(k), hipMemcpyDeviceToHost));
+    if (k == none) return MLEM_OK;
+    uint32_t c = 0;
+    HIPCHK(hipMemcpy(&c, col + k, sizeof(c), hipMemcpyDeviceToHost));
+    return set_err(MLEM_ERR_INVALID_ARGUMENT, "%s: column index %u at nonzero %llu >= %u", what, c,
+                   (unsigned long long)k, cols);
+}
+
+// Host array -> device: small copies directly, large ones through four
+// pinned 64 MiB buffers in rotation, filled by the host library's OpenMP
+// threads while the previous buffers go to the device (pageable copies run
+// on the calling thread, which OpenMP may have bound to one core).
+int upload_h2d(void *dst, const void *src, size_t bytes) {
+    constexpr size_t kChunk = size_t(64) << 20;
+    if (bytes <= 2 * kChunk) {
+        if (bytes) HIPCHK(hipMemcpy(dst, src, bytes, hipMemcpyHostToDevice));
+        return MLEM_OK;
+    }
+    constexpr int kBufs = 4;
+    struct Staging {
+        void *host[kBufs] = {};
+        hipEvent_t done[kBufs] = {};
+        hipStream_t s = nullptr;
+        ~Staging() {
+            if (s) (void)hipStreamSynchronize(s);
+            for (int i = 0; i < kBufs; ++i) {
+                if (host[i]) (void)hipHostFree(host[i]);
+                if (done[i]) (void)hipEventDestroy(done[i]);
+            }
+            if (s) (void)hipStreamDestroy(s);
+        }
+    } st;
+    HIPCHK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    for (int i = 0; i < kBufs; ++i) {
+        HIPCHK(hipHostMalloc(&st.host[i], kChunk, hipHostMallocDefault));
+        HIPCHK(hipEventCreateWithFlags(&st.done[i], hipEventDisableTiming));
+    }
+    bool used[kBufs] = {};
+    int i = 0;
+    for (size_t off = 0; off < bytes; off += kChunk, i = (i + 1) % kBufs) {
+        const size_t len = std::min(kChunk, bytes - off);
+        if (used[i]) HIPCHK(hipEventSynchronize(st.done[i]));
+        if (int e = mlem_host_copy(st.host[i], static_cast<const uint8_t *>(src) + off, len, 0))
+            return set_err(e, "upload: %s", mlem_host_last_error());
+        HIPCHK(hipMemcpyAsync(static_cast<uint8_t *>(dst) + off, st.host[i], len, hipMemcpyHostToDevice, st.s));
+        HIPCHK(hipEventRecord(st.done[i], st.s));
+        used[i] = true;
+    }
+    HIPCHK(hipStreamSynchronize(st.s));
     return MLEM_OK;
 }
 
@@ -799,20 +844,22 @@ int mlem_matrix_create(int device, uint32_t rows, uint32_t cols, uint64_t nnz, c
     HIPCHK(m->rowptr.alloc(sizeof(uint64_t) * ((size_t)rows + 1)));
     HIPCHK(m->col.alloc(sizeof(uint32_t) * nnz));
     HIPCHK(m->val.alloc(sizeof(float) * nnz));
-    HIPCHK(hipMemcpy(m->rowptr.p, rowptr, m->rowptr.bytes, hipMemcpyHostToDevice));
+    if (int st = upload_h2d(m->rowptr.p, rowptr, m->rowptr.bytes)) return st;
     if (nnz) {
-        HIPCHK(hipMemcpy(m->col.p, col, m->col.bytes, hipMemcpyHostToDevice));
-        HIPCHK(hipMemcpy(m->val.p, val, m->val.bytes, hipMemcpyHostToDevice));
+        if (int st = upload_h2d(m->col.p, col, m->col.bytes)) return st;
+        if (int st = upload_h2d(m->val.p, val, m->val.bytes)) return st;
     }
+    if (int st = check_columns_device("mlem_matrix_create (A)", m->col.as<uint32_t>(), nnz, cols)) return st;
     if (rowptrT) {
         HIPCHK(m->rowptrT.alloc(sizeof(uint64_t) * ((size_t)cols + 1)));
         HIPCHK(m->colT.alloc(sizeof(uint32_t) * nnz));
         HIPCHK(m->valT.alloc(sizeof(float) * nnz));
-        HIPCHK(hipMemcpy(m->rowptrT.p, rowptrT, m->rowptrT.bytes, hipMemcpyHostToDevice));
+        if (int st = upload_h2d(m->rowptrT.p, rowptrT, m->rowptrT.bytes)) return st;
         if (nnz) {
-            HIPCHK(hipMemcpy(m->colT.p, colT, m->colT.bytes, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(m->valT.p, valT, m->valT.bytes, hipMemcpyHostToDevice));
+            if (int st = upload_h2d(m->colT.p, colT, m->colT.bytes)) return st;
+            if (int st = upload_h2d(m->valT.p, valT, m->valT.bytes)) return st;
         }
+        if (int st = check_columns_device("mlem_matrix_create (A^T)", m->colT.as<uint32_t>(), nnz, rows)) return st;
         m->has_T = true;
     }
     *out = m.release();
@@ -853,62 +900,71 @@ int mlem_matrix_load_csr4(int device, const char *path, int threads, mlem_matrix
         if (rows) std::memcpy(rp.data() + 1, ends, sizeof(uint64_t) * rows);
         HIPCHK(hipMemcpy(m->rowptr.p, rp.data(), m->rowptr.bytes, hipMemcpyHostToDevice));
     }
-    // Three pinned buffers of 64 MiB in rotation: while buffer i is copied
-    // to the device and split, the host threads fill buffer i+1.
-    constexpr int kBufs = 3;
-    constexpr size_t kChunk = size_t(64) << 20;  // a multiple of the 8-byte entry
+    // Four pinned buffers of 128 MiB in rotation: while buffer i goes to the
+    // device (copy stream) and is split into col / val (split stream), the
+    // host library's OpenMP threads fill buffer i+1 from the mapped file.
+    // (Threads started here would inherit the one core that OpenMP binds a
+    // process's initial thread to under OMP_PROC_BIND: a fifth of the rate.)
+    constexpr int kBufs = 4;
+    constexpr size_t kChunk = size_t(128) << 20;  // a multiple of the 8-byte entry
     const uint64_t total = 8 * nnz;
-    const int nthr = threads > 0 ? threads : (int)std::max(1u, std::min(32u, std::thread::hardware_concurrency()));
     struct Staging {
         void *host[kBufs] = {};
-        hipEvent_t ev[kBufs] = {};
-        hipStream_t s = nullptr;
+        hipEvent_t copied[kBufs] = {}, split[kBufs] = {};
+        hipStream_t s = nullptr, c = nullptr;  // split stream, copy stream
         ~Staging() {
+            if (c) (void)hipStreamSynchronize(c);
             if (s) (void)hipStreamSynchronize(s);
             for (int i = 0; i < kBufs; ++i) {
                 if (host[i]) (void)hipHostFree(host[i]);
-                if (ev[i]) (void)hipEventDestroy(ev[i]);
+                if (copied[i]) (void)hipEventDestroy(copied[i]);
+                if (split[i]) (void)hipEventDestroy(split[i]);
             }
+            if (c) (void)hipStreamDestroy(c);
             if (s) (void)hipStreamDestroy(s);
         }
     } st;
     DevBuf dev[kBufs], bad;
-    HIPCHK(hipStreamCreate(&st.s));
+    HIPCHK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
+    HIPCHK(hipStreamCreateWithFlags(&st.c, hipStreamNonBlocking));
     HIPCHK(bad.alloc(sizeof(uint32_t)));
-    HIPCHK(hipMemsetAsync(bad.p, 0, sizeof(uint32_t), st.s));
+    HIPCHK(launch_fill_u32(bad.as<uint32_t>(), 0u, 1, st.s));
     if (total) {
         for (int i = 0; i < kBufs; ++i) {
             HIPCHK(hipHostMalloc(&st.host[i], kChunk, hipHostMallocDefault));
-            HIPCHK(hipEventCreateWithFlags(&st.ev[i], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&st.copied[i], hipEventDisableTiming));
+            HIPCHK(hipEventCreateWithFlags(&st.split[i], hipEventDisableTiming));
             HIPCHK(dev[i].alloc(kChunk));
         }
     }
+    double t_fill = 0, t_wait = 0;
     bool used[kBufs] = {};
     int i = 0;
     for (uint64_t off = 0; off < total; off += kChunk, i = (i + 1) % kBufs) {
         const size_t len = (size_t)std::min<uint64_t>(kChunk, total - off);
-        if (used[i]) HIPCHK(hipEventSynchronize(st.ev[i]));
-        {
-            // host threads copy slices of the mapped file into the pinned buffer
-            const size_t per = ((len / nthr) + 4095) & ~size_t(4095);
-            std::vector<std::thread> pool;
-            for (int t = 0; t < nthr; ++t) {
-                const size_t b = std::min(len, per * t), e = std::min(len, per * (t + 1));
-                if (b >= e) break;
-                pool.emplace_back([&, b, e] { std::memcpy(static_cast<uint8_t *>(st.host[i]) + b, data + off + b, e - b); });
-            }
-            for (auto &th : pool) th.join();
-        }
-        HIPCHK(hipMemcpyAsync(dev[i].p, st.host[i], len, hipMemcpyHostToDevice, st.s));
+        const auto w0 = clk::now();
+        if (used[i]) HIPCHK(hipEventSynchronize(st.split[i]));  // the buffer's last chunk is on the device and split
+        const auto w1 = clk::now();
+        if (int e = mlem_host_copy(st.host[i], data + off, len, threads))
+            return set_err(e, "mlem_matrix_load_csr4: %s", mlem_host_last_error());
+        const auto w2 = clk::now();
+        t_wait += secs(w0, w1);
+        t_fill += secs(w1, w2);
+        HIPCHK(hipMemcpyAsync(dev[i].p, st.host[i], len, hipMemcpyHostToDevice, st.c));
+        HIPCHK(hipEventRecord(st.copied[i], st.c));
+        HIPCHK(hipStreamWaitEvent(st.s, st.copied[i], 0));
         HIPCHK(launch_aos_split(dev[i].p, len / 8, off / 8, cols, m->col.as<uint32_t>(), m->val.as<float>(),
                                 bad.as<uint32_t>(), st.s));
-        HIPCHK(hipEventRecord(st.ev[i], st.s));
+        HIPCHK(hipEventRecord(st.split[i], st.s));
         used[i] = true;
     }
     uint32_t flag = 0;
     HIPCHK(hipMemcpyAsync(&flag, bad.p, sizeof(uint32_t), hipMemcpyDeviceToHost, st.s));
     HIPCHK(hipStreamSynchronize(st.s));
     const auto t2 = clk::now();
+    progress("csr4 load: %.2f GB in %.3f s: host threads filling the pinned buffers %.3f s, waiting for a buffer "
+             "%.3f s (%d buffers of %zu MiB)",
+             total / 1e9, secs(t1, t2), t_fill, t_wait, kBufs, kChunk >> 20);
     if (flag) return set_err(MLEM_ERR_FORMAT, "mlem_matrix_load_csr4: %s has a column index >= %u", path, cols);
     if (stats) {
         stats->bytes = total;

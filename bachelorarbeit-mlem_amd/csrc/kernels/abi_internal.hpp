@@ -82,10 +82,16 @@ struct DevBuf {
     }
 };
 
-// Host-side checks of a CSR before it reaches the GPU: a bad row pointer or
-// column index would make the kernels read outside their arrays.
+// Host-side checks of a CSR's row pointers before it reaches the GPU (a bad
+// one would make the kernels read outside their arrays); the column indices
+// are checked on the device once uploaded.
 int check_csr_host(const char *what, uint32_t rows, uint32_t cols, uint64_t nnz, const uint64_t *rowptr,
                    const uint32_t *col);
+// Column indices of an uploaded CSR: MLEM_ERR_INVALID_ARGUMENT naming the
+// first index >= cols.
+int check_columns_device(const char *what, const uint32_t *col, uint64_t nnz, uint32_t cols);
+// Host -> device copy, large ones through pinned buffers (waits).
+int upload_h2d(void *dst, const void *src, size_t bytes);
 
 inline uint32_t clamp_secsize(uint32_t s) { return std::min(std::max(s, 1u), kMaxSecsize); }
 

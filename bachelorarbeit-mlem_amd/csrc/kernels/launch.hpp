@@ -68,6 +68,8 @@ hipError_t launch_fill_u32(uint32_t *dst, uint32_t v, uint64_t n, hipStream_t s)
 // column is >= cols.
 hipError_t launch_aos_split(const void *aos, uint64_t n, uint64_t first, uint32_t cols, uint32_t *col, float *val,
                             uint32_t *bad, hipStream_t s);
+// *first = min(*first, smallest k < n with col[k] >= cols).
+hipError_t launch_first_bad_column(const uint32_t *col, uint64_t n, uint32_t cols, uint64_t *first, hipStream_t s);
 // 4-byte elements: dst[i*stride] = src[i] / dst[i] = src[i*stride], i < n.
 hipError_t launch_strided_put(void *dst, uint32_t stride, const void *src, uint64_t n, hipStream_t s);
 // 4-byte elements in row order: dst[order[i]*stride] = src[i], i < n.

@@ -374,6 +374,17 @@ __global__ __launch_bounds__(kWG) void aos_split_kernel(const uint32_t *__restri
     if (flag) atomicOr(bad, 1u);
 }
 
+// *first = the smallest k with col[k] >= cols (unchanged when there is none;
+// the caller starts it at UINT64_MAX).
+__global__ void first_bad_column_kernel(const uint32_t *__restrict__ col, uint64_t n, uint32_t cols,
+                                        unsigned long long *__restrict__ first) {
+    for (uint64_t k = blockIdx.x * (uint64_t)blockDim.x + threadIdx.x; k < n; k += (uint64_t)gridDim.x * blockDim.x)
+        if (col[k] >= cols) {
+            atomicMin(first, (unsigned long long)k);
+            return;
+        }
+}
+
 // One slot of an interleaved [n][stride] array <-> a contiguous vector.
 template <typename T>
 __global__ void strided_put_kernel(T *__restrict__ dst, uint32_t stride, const T *__restrict__ src, uint64_t n) {
@@ -629,6 +640,14 @@ hipError_t launch_aos_split(const void *aos, uint64_t n, uint64_t first, uint32_
     const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((n + kWG - 1) / kWG, 16384));
     hipLaunchKernelGGL(aos_split_kernel, dim3((uint32_t)blocks), dim3(kWG), 0, s, static_cast<const uint32_t *>(aos),
                        n, first, cols, col, val, bad);
+    return hipGetLastError();
+}
+
+hipError_t launch_first_bad_column(const uint32_t *col, uint64_t n, uint32_t cols, uint64_t *first, hipStream_t s) {
+    if (n == 0) return hipSuccess;
+    const uint64_t blocks = std::max<uint64_t>(1, std::min<uint64_t>((n + kWG - 1) / kWG, 16384));
+    hipLaunchKernelGGL(first_bad_column_kernel, dim3((uint32_t)blocks), dim3(kWG), 0, s, col, n, cols,
+                       reinterpret_cast<unsigned long long *>(first));
     return hipGetLastError();
 }
 

@@ -142,11 +142,12 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
         HIPCHK(rp.alloc(sizeof(uint64_t) * ((size_t)rows + 1)));
         HIPCHK(ci.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, nnz)));
         HIPCHK(va.alloc(sizeof(float) * std::max<uint64_t>(1, nnz)));
-        HIPCHK(hipMemcpy(rp.p, rowptr, rp.bytes, hipMemcpyHostToDevice));
+        if (int st = upload_h2d(rp.p, rowptr, rp.bytes)) return st;
         if (nnz) {
-            HIPCHK(hipMemcpy(ci.p, col, sizeof(uint32_t) * nnz, hipMemcpyHostToDevice));
-            HIPCHK(hipMemcpy(va.p, val, sizeof(float) * nnz, hipMemcpyHostToDevice));
+            if (int st = upload_h2d(ci.p, col, sizeof(uint32_t) * nnz)) return st;
+            if (int st = upload_h2d(va.p, val, sizeof(float) * nnz)) return st;
         }
+        if (int st = check_columns_device("mlem_matrix_create_tiled", ci.as<uint32_t>(), nnz, cols)) return st;
         HIPCHK(colptrT.alloc(sizeof(uint64_t) * ((size_t)cols + 1)));
         HIPCHK(srp.alloc(sizeof(uint64_t) * ((size_t)cols + 1)));
         if (values != VAL_F32) {  // A's values become their decoded codes: what A^T's tiles store
@@ -228,10 +229,12 @@ int mlem_matrix_create_tiled(int device, uint32_t rows, uint32_t cols, uint64_t 
             HIPCHK(rp.alloc(sizeof(uint64_t) * (rc + 1)));
             HIPCHK(ci.alloc(sizeof(uint32_t) * std::max<uint64_t>(1, n)));
             HIPCHK(hipMemcpy(rp.p, hrp.data(), rp.bytes, hipMemcpyHostToDevice));
-            if (n) HIPCHK(hipMemcpy(ci.p, col + rowptr[r0], sizeof(uint32_t) * n, hipMemcpyHostToDevice));
+            if (n)
+                if (int st = upload_h2d(ci.p, col + rowptr[r0], sizeof(uint32_t) * n)) return st;
             if (pass == 0) return bA.count(r0, rc, n, rp.as<uint64_t>(), ci.as<uint32_t>());
             HIPCHK(va.alloc(sizeof(float) * std::max<uint64_t>(1, n)));
-            if (n) HIPCHK(hipMemcpy(va.p, val + rowptr[r0], sizeof(float) * n, hipMemcpyHostToDevice));
+            if (n)
+                if (int st = upload_h2d(va.p, val + rowptr[r0], sizeof(float) * n)) return st;
             return bA.fill(r0, rc, n, rp.as<uint64_t>(), ci.as<uint32_t>(), va.as<float>());
         };
         for (uint32_t r0 = 0; r0 < rows; ++chunks) {

@@ -141,6 +141,12 @@ int mlem_partition(const uint64_t *rowptr, uint64_t nnz, uint32_t rows,
                    uint32_t *segment_nnzs, uint64_t *segment_nnzs64,
                    uint64_t *offsets);
 
+/* memcpy of `bytes` on the library's OpenMP threads (<= 0: OpenMP default),
+ * which follow OMP_PROC_BIND / OMP_PLACES: the device library's csr4 loader
+ * fills its pinned staging buffers with it.  No counterpart in the
+ * reference. */
+int mlem_host_copy(void *dst, const void *src, uint64_t bytes, int threads);
+
 /* An order of the rows (LORs) of a system matrix for the batched layouts
  * (mlem_matrix_set_row_order in mlem_hip.h): order[i] = place of row i.
  * From the matrix and its image grid (nx, ny, nz with x fastest, as the

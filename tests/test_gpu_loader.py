@@ -31,9 +31,9 @@ def test_load_matches_source(tmp_path, kind):
 
 
 def test_load_crosses_chunks_and_buffers(tmp_path):
-    # ~30M entries = 240 MB: four 64 MiB chunks, the three staging buffers reused
+    # ~85M entries = 680 MB: six 128 MiB chunks, the four staging buffers reused
     rng = np.random.default_rng(5)
-    lens = rng.integers(0, 2000, 30000)
+    lens = rng.integers(0, 2000, 85000)
     A = from_lengths(lens, 1 << 20, seed=6, dup=True)
     p = str(tmp_path / "big.csr4")
     M.write_csr4(p, A)

@@ -174,3 +174,22 @@ def test_raw_vectors(tmp_path):
     p = str(tmp_path / "empty")
     open(p, "wb").close()
     assert M.read_raw(p, np.int32).size == 0
+
+
+def test_host_copy_is_memcpy():
+    """mlem_host_copy (the csr4 loader's fill of its staging buffers): byte for byte, any size and thread count,
+    slices split on 4 KiB boundaries."""
+    import ctypes as C
+    lib = M.host_lib()
+    fn = lib.mlem_host_copy
+    fn.restype = C.c_int
+    fn.argtypes = [C.c_void_p, C.c_void_p, C.c_uint64, C.c_int]
+    rng = np.random.default_rng(9)
+    for n in (0, 1, 4095, 4097, 1 << 20, (3 << 20) + 13):
+        src = rng.integers(0, 256, n, dtype=np.uint8)
+        for threads in (0, 1, 3, 16):
+            dst = np.zeros(n + 8, np.uint8)
+            assert fn(dst.ctypes.data, src.ctypes.data, n, threads) == M.MLEM_OK
+            assert np.array_equal(dst[:n], src) and not dst[n:].any()
+    assert fn(None, None, 0, 0) == M.MLEM_OK
+    assert fn(None, src.ctypes.data, 16, 0) != M.MLEM_OK
