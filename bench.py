@@ -11,9 +11,10 @@ generated on the host from a seed (no data files exist for it).
 Prints one JSON line: iterations/s over the timed steps, the dominant
 kernel's achieved bandwidth against the 8 TB/s HBM peak (HIP events on the
 reconstruction's stream), the FP32 OpenMP CPU loop timed on the host's
-physical cores (a bounded sample: 3 runs of 1 warm-up + k iterations, about
-25 s in all, median), and a witness: the GPU image after 100 iterations
-against the FP64 OpenMP oracle.
+physical cores (3 runs of 1 warm-up + k iterations, median; k = the
+workload's 100 iterations, more on small matrices so that the three runs
+take about 25 s), and a witness: the GPU image after 100 iterations against
+the FP64 OpenMP oracle.
 
 Algorithmic bytes (f32 values, u32 indices, u64 row pointers):
   FP SpMV kernel  8*nnz + 8*(rows+1) + 4*cols      (A and x)
@@ -90,12 +91,13 @@ def baseline_threads():
     return max(1, min(phys, int(cap))) if cap.isdigit() and int(cap) > 0 else phys
 
 
-def cpu_baseline_f32(O, A, AT, g, s32, f0, threads, budget_s=25.0, runs=3):
-    """The FP32 OpenMP CPU loop (oracle_mlem_f32_omp) on the same arrays, on a bounded sample: one timed
-    iteration sizes it, then `runs` runs of 1 warm-up iteration + k iterations (the three together about
-    budget_s seconds).  Returns the median iterations/s, k, each run's seconds and the host description."""
+def cpu_baseline_f32(O, A, AT, g, s32, f0, threads, budget_s=25.0, runs=3, min_iters=100):
+    """The FP32 OpenMP CPU loop (oracle_mlem_f32_omp) on the same arrays: `runs` runs of 1 warm-up iteration +
+    k iterations, median.  k is the workload's 100 iterations, or more where 100 take less than budget_s / runs
+    seconds (small matrices, to time a long enough run; one timed iteration sizes it).  Returns the median
+    iterations/s, k, each run's seconds and the host description."""
     _, secs1 = O.mlem_f32_omp(A, AT, g, s32, f0, 1, threads)
-    k = max(1, int(budget_s / runs / max(secs1, 1e-6)))
+    k = max(min_iters, int(budget_s / runs / max(secs1, 1e-6)))
     rates, secs_all = [], []
     for r in range(runs):
         O.mlem_f32_omp(A, AT, g, s32, f0, 1, threads)
