@@ -92,26 +92,12 @@ constexpr int col_bits() {
     return W == 512 ? 10 : W == 1024 ? 11 : 12;  // columns 0..W (W = padding)
 }
 
-// Column j of a group word: bits [32 + j*CB, 32 + (j+1)*CB).
-template <int CB>
-__device__ __forceinline__ uint32_t group_col(const u32x4 &w, int j) {
-    const int bit = j * CB;
-    const uint64_t lo = (uint64_t)w.y | ((uint64_t)w.z << 32);
-    uint64_t v;
-    if (bit + CB <= 64)
-        v = lo >> bit;
-    else if (bit >= 64)
-        v = (uint64_t)w.w >> (bit - 64);
-    else
-        v = (lo >> bit) | ((uint64_t)w.w << (64 - bit));
-    return (uint32_t)v & ((1u << CB) - 1u);
-}
-
-// The same with the field extract written as v_bfe_u32 (after a
-// v_alignbit_b32 for a field that crosses a dword), so that with the
-// address arithmetic that follows (v_lshl_add_u32) a column takes two or
-// three instructions; the compiler turns an extract followed by a shift
-// into a shift, a mask and an add, three or four.
+// Column j of a group word: bits [32 + j*CB, 32 + (j+1)*CB).  The field
+// extract is written as v_bfe_u32 (after a v_alignbit_b32 for a field that
+// crosses a dword), so that with the address arithmetic that follows
+// (v_lshl_add_u32) a column takes two or three instructions; the compiler
+// turns an extract followed by a shift into a shift, a mask and an add,
+// three or four.
 template <int O, int N>
 __device__ __forceinline__ uint32_t bfe_u32(uint32_t w) {
     uint32_t r;
@@ -130,7 +116,7 @@ __device__ __forceinline__ uint32_t group_col_j(const u32x4 &w) {
     }
 }
 template <int CB>
-__device__ __forceinline__ uint32_t group_col_bfe(const u32x4 &w, int j) {
+__device__ __forceinline__ uint32_t group_col(const u32x4 &w, int j) {
     switch (j) {
         case 0: return group_col_j<CB, 0>(w);
         case 1: return group_col_j<CB, 1>(w);
@@ -647,7 +633,7 @@ __global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t ns
             f32x4 t4[4][BC / 4];
 #pragma unroll
             for (int jj = 0; jj < 4; ++jj) {
-                const float *xg = xw + group_col_bfe<CB>(cg[q / 2], 4 * (q & 1) + jj) * BC;
+                const float *xg = xw + group_col<CB>(cg[q / 2], 4 * (q & 1) + jj) * BC;
 #pragma unroll
                 for (int qq = 0; qq < BC / 4; ++qq) t4[jj][qq] = *reinterpret_cast<const f32x4 *>(xg + 4 * qq);
             }
