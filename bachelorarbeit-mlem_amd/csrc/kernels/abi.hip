@@ -544,8 +544,11 @@ int mlem_set_device(int device) {
 int mlem_device_info(int device, char *name, size_t name_len, uint64_t *total_mem, int *cu_count) {
     hipDeviceProp_t p;
     HIPCHK(hipGetDeviceProperties(&p, device));
-    if (name && name_len) {
-        std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+    if (name && name_len) {  // the marketing name can be empty (no device-id table): the architecture alone
+        if (p.name[0])
+            std::snprintf(name, name_len, "%s (%s)", p.name, p.gcnArchName);
+        else
+            std::snprintf(name, name_len, "%s", p.gcnArchName);
     }
     if (total_mem) *total_mem = p.totalGlobalMem;
     if (cu_count) *cu_count = p.multiProcessorCount;
