@@ -911,6 +911,9 @@ int mlem_matrix_load_csr4(int device, const char *path, int threads, mlem_matrix
     constexpr int kBufs = 4;
     constexpr size_t kChunk = size_t(128) << 20;  // a multiple of the 8-byte entry
     const uint64_t total = 8 * nnz;
+    // Declared before the streams: on an early return the streams are
+    // synchronized (Staging's destructor) before these are freed.
+    DevBuf dev[kBufs], bad;
     struct Staging {
         void *host[kBufs] = {};
         hipEvent_t copied[kBufs] = {}, split[kBufs] = {};
@@ -927,7 +930,6 @@ int mlem_matrix_load_csr4(int device, const char *path, int threads, mlem_matrix
             if (s) (void)hipStreamDestroy(s);
         }
     } st;
-    DevBuf dev[kBufs], bad;
     HIPCHK(hipStreamCreateWithFlags(&st.s, hipStreamNonBlocking));
     HIPCHK(hipStreamCreateWithFlags(&st.c, hipStreamNonBlocking));
     HIPCHK(bad.alloc(sizeof(uint32_t)));
