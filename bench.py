@@ -453,7 +453,8 @@ def main():
     ap.add_argument("--tile-threads", type=int, default=0,
                     help="work-items per workgroup of the kernel for 4 or 8 per pass: 128, 256, 512 (0: default)")
     ap.add_argument("--graph", type=int, default=None,
-                    help="replay captured hipGraphs (default: 1 when serving, as the serving workload names, else 0)")
+                    help="replay captured hipGraphs (default: 1 for batches and serving, whose workloads ask for "
+                         "graph-captured iterations, else 0)")
     ap.add_argument("--autotune", action="store_true",
                     help="time the candidate tile shapes and workgroup counts first (mlem_autotune) and use the best")
     ap.add_argument("--threads", type=int, default=0, help="host threads (0: all)")
@@ -487,7 +488,7 @@ def main():
     if args.values is None:
         args.values = "uf16" if (args.batch or args.serve) else "f32"
     if args.graph is None:
-        args.graph = 1 if args.serve else 0
+        args.graph = 1 if (args.serve or args.batch) else 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
