@@ -455,8 +455,9 @@ def main():
     ap.add_argument("--graph", type=int, default=None,
                     help="replay captured hipGraphs (default: 1 for batches and serving, whose workloads ask for "
                          "graph-captured iterations, else 0)")
-    ap.add_argument("--autotune", action="store_true",
-                    help="time the candidate tile shapes and workgroup counts first (mlem_autotune) and use the best")
+    ap.add_argument("--autotune", type=int, nargs="?", const=1, default=None,
+                    help="time the candidate tile shapes and workgroup counts first (mlem_autotune) and use the best "
+                         "(default: 1 for batches, whose workload asks for autotuned shapes, else 0)")
     ap.add_argument("--threads", type=int, default=0, help="host threads (0: all)")
     ap.add_argument("--no-cpu", action="store_true", help="skip the CPU baseline (profiling runs)")
     ap.add_argument("--batch", type=int, default=0,
@@ -489,6 +490,8 @@ def main():
         args.values = "uf16" if (args.batch or args.serve) else "f32"
     if args.graph is None:
         args.graph = 1 if (args.serve or args.batch) else 0
+    if args.autotune is None:
+        args.autotune = 1 if args.batch else 0
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
