@@ -445,7 +445,9 @@ def main():
     ap.add_argument("--c3-csr", action="store_true",
                     help="c3: keep A and A^T resident as CSR instead (plain merge / CSR-vector kernels)")
     ap.add_argument("--fp", default="merge", choices=["merge", "vector"])
-    ap.add_argument("--bp", default="merge", choices=["merge", "vector"])
+    ap.add_argument("--bp", default=None, choices=["merge", "vector"],
+                    help="default: vector for the 2D ring (merge-based FP with CSR-vector BP on A^T is the pairing "
+                         "that workload names), else merge")
     ap.add_argument("--secsize", type=int, default=8)
     ap.add_argument("--tile", type=int, default=0, help="merge-path tiling: 0 auto, 1 off, else segment width")
     ap.add_argument("--tile-rows", type=int, default=0, help="rows per tile block (0: default)")
@@ -492,6 +494,8 @@ def main():
         args.graph = 1 if (args.serve or args.batch) else 0
     if args.autotune is None:
         args.autotune = 1 if args.batch else 0
+    if args.bp is None:
+        args.bp = "vector" if args.workload == "c1" and not (args.batch or args.serve) else "merge"
 
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
