@@ -266,6 +266,11 @@ uint32_t default_tile_rows(uint64_t nnz, uint32_t groups, uint32_t BC);
 int batch_set_async(mlem_batch *b, uint32_t slot, const int32_t *g, const float *f0, uint64_t *allocated);
 int batch_get_image_async(mlem_batch *b, uint32_t slot, float *f_out);
 int batch_deactivate_async(mlem_batch *b, uint32_t slot);
+// The reconstruction in active slot `from` continues in free slot `to` (of a
+// pass that holds memory): its counts and image are copied on the batch's
+// stream, the flags swapped.  A slot's arithmetic does not depend on its
+// place, so the images are bitwise those it would have had in `from`.
+int batch_move_slot_async(mlem_batch *b, uint32_t from, uint32_t to);
 // Frees pass p when none of its slots is active (returns the bytes freed).
 uint64_t batch_release_pass(mlem_batch *b, uint32_t p);
 // Progress of long builds on stderr when MLEM_VERBOSE is set.

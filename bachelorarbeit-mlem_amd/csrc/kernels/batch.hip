@@ -93,6 +93,27 @@ int batch_deactivate_async(mlem_batch *b, uint32_t slot) {
     return MLEM_OK;
 }
 
+int batch_move_slot_async(mlem_batch *b, uint32_t from, uint32_t to) {
+    ARGCHK(b && from < b->slots && to < b->slots && from != to, "batch_move_slot: bad slots %u -> %u", from, to);
+    const uint32_t BC = b->BC, pf = from / BC, qf = from % BC, pt = to / BC, qt = to % BC;
+    ARGCHK(b->pass[pf] && b->pass[pt] && b->active_host[from] && !b->active_host[to],
+           "batch_move_slot: slot %u is not active or slot %u is not free in an allocated pass", from, to);
+    HIPCHK(hipSetDevice(b->m->device));
+    hipStream_t s = b->stream;
+    const uint32_t rows = b->m->rows, cols = b->m->cols;
+    // The counts in the positions the tiles hold them (row order or not), and the image; the ratios are
+    // recomputed from them by the next iteration.
+    HIPCHK(launch_strided_get(b->stage.p, b->pass[pf]->g.as<int32_t>() + qf, BC, rows, s));
+    HIPCHK(launch_strided_put(b->pass[pt]->g.as<int32_t>() + qt, BC, b->stage.p, rows, s));
+    HIPCHK(launch_strided_get(b->stage.p, b->pass[pf]->f.as<float>() + qf, BC, cols, s));
+    HIPCHK(launch_strided_put(b->pass[pt]->f.as<float>() + qt, BC, b->stage.p, cols, s));
+    HIPCHK(launch_fill_u32(b->active.as<uint32_t>() + to, 1u, 1, s));
+    HIPCHK(launch_fill_u32(b->active.as<uint32_t>() + from, 0u, 1, s));
+    b->active_host[to] = 1;
+    b->active_host[from] = 0;
+    return MLEM_OK;
+}
+
 uint64_t batch_release_pass(mlem_batch *b, uint32_t p) {
     if (!b || p >= b->passes || !b->pass[p]) return 0;
     for (uint32_t q = 0; q < b->BC; ++q)

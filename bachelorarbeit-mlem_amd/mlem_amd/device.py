@@ -92,6 +92,7 @@ def hip_lib() -> C.CDLL:
             "mlem_server_step": (cint, [vp, u32, P(u32), P(u32)]),
             "mlem_server_poll": (cint, [vp, u64, vp, P(f64), P(cint)]),
             "mlem_server_info": (cint, [vp, P(u64), P(u64), P(u32), P(u64)]),
+            "mlem_server_stats": (cint, [vp, P(u64), P(u64)]),
             "mlem_server_destroy": (cint, [vp]),
             "mlem_matrix_destroy": (cint, [vp]),
             "mlem_matrix_info": (cint, [vp, P(u32), P(u32), P(u64), P(cint), P(u64)]),
@@ -464,6 +465,12 @@ class Server:
         b, u, m, s = u64(), u64(), u32(), u64()
         _check(hip_lib().mlem_server_info(self.h, C.byref(b), C.byref(u), C.byref(m), C.byref(s)))
         return dict(budget=b.value, used=u.value, matrices=m.value, steps=s.value)
+
+    def stats(self):
+        """Passes over a matrix started so far, and requests moved to a lower pass to empty one."""
+        sw, mv = u64(), u64()
+        _check(hip_lib().mlem_server_stats(self.h, C.byref(sw), C.byref(mv)))
+        return dict(pass_sweeps=sw.value, slot_moves=mv.value)
 
     def close(self):
         if self.h:

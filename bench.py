@@ -409,6 +409,9 @@ def run_serve(args, M, threads):
                           one_at_a_time_p50=pct(ser, 50), one_at_a_time_p99=pct(ser, 99)))
     head = sweep[0]
     info = srv.info()
+    stats = srv.stats()
+    log(f"passes over a matrix started: {stats['pass_sweeps']}; requests moved to a lower pass to empty one: "
+        f"{stats['slot_moves']}")
     srv.close()
     for d in mats:
         d["m"].close()
@@ -419,7 +422,8 @@ def run_serve(args, M, threads):
         "config": {"workload": "served mix: " + " + ".join(d["label"] for d in mats), "matrices": names,
                    "requests": n, "iterations_per_request": iters_desc, "slots_per_matrix": args.serve_slots,
                    "per_pass": args.per_pass, "values": args.values, "graph": bool(args.graph),
-                   "lor_order": bool(args.lor_order), "hbm_used_bytes": info["used"]},
+                   "lor_order": bool(args.lor_order), "hbm_used_bytes": info["used"],
+                   "pass_sweeps": stats["pass_sweeps"], "slot_moves": stats["slot_moves"]},
         "latency_ms": {"arrivals": f"Poisson, {head['rate']:.3f} req/s ({head['load']} x backlog throughput)",
                        "p50": head["p50"], "p99": head["p99"], "max": head["max"],
                        "backlog_p50": pct(lat1, 50), "backlog_p99": pct(lat1, 99)},

@@ -465,6 +465,13 @@ int mlem_server_poll(mlem_server *s, uint64_t request_id, float *f_out,
                      double *latency_s, int *done);
 int mlem_server_info(const mlem_server *s, uint64_t *budget, uint64_t *used,
                      uint32_t *matrices, uint64_t *steps);
+/* Work done so far: *pass_sweeps = passes over a matrix started (a pass
+ * costs a full sweep of its matrix however few of its slots are active),
+ * *slot_moves = requests moved to a lower pass so that a pass empties (the
+ * requests of the highest pass with active slots move whenever they fit in
+ * free slots of lower passes; the images stay bitwise the same).  Either may
+ * be NULL.  No counterpart in the reference. */
+int mlem_server_stats(const mlem_server *s, uint64_t *pass_sweeps, uint64_t *slot_moves);
 int mlem_server_destroy(mlem_server *s);
 
 /* ------------------------------------------------------------------------ */

@@ -98,6 +98,36 @@ def test_mixed_requests_match_alone_and_oracle(served, per_pass, values, use_gra
     srv.close()
 
 
+@pytest.mark.parametrize("values,use_graph", [(0, 0), (3, 1)])
+def test_survivors_move_to_fewer_passes(served, values, use_graph):
+    """Requests of 3 and 12 iterations alternate over 8 slots (two passes of 4): after the third iteration each pass
+    holds two survivors; those of the second pass move into the first, so the other 9 iterations sweep one pass
+    instead of two (15 pass sweeps instead of 24), and every image is bitwise the one its request gets alone."""
+    d = served[0]
+    srv = M.Server(per_pass=4, values=values, use_graph=use_graph)
+    mid = srv.add_matrix(d["m"], slots=8)
+    iters = [3, 12, 3, 12, 3, 12, 3, 12]
+    rids = [srv.submit(mid, d["sinos"][k % 6], d["f0"], it) for k, it in enumerate(iters)]
+    done = {}
+    for _ in range(200):
+        srv.step()
+        for k, r in enumerate(rids):
+            if k not in done:
+                out = srv.poll(r, mid)
+                if out is not None:
+                    done[k] = out[0]
+        if len(done) == len(rids):
+            break
+    assert len(done) == len(rids)
+    st = srv.stats()
+    print(f"values={values} graph={use_graph}: {st}", flush=True)
+    assert st["slot_moves"] == 2 and st["pass_sweeps"] == 15
+    for k, it in enumerate(iters):
+        ref = alone(d, d["sinos"][k % 6], it, 4, values, use_graph)
+        assert np.array_equal(done[k].view(np.uint32), ref.view(np.uint32)), f"request {k} differs from alone"
+    srv.close()
+
+
 def test_more_requests_than_slots_queue(served):
     d = served[1]
     srv = M.Server(per_pass=4)
