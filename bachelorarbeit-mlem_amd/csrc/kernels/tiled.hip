@@ -311,12 +311,15 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
 
     f32x4 pv[D][2] = {};
     u32x4 pq[D] = {}, pg[D] = {};
-    f32x4 px[D][XF / 4];
+    f32x4 px[D][XF / 4] = {};
     // Loads of unit u into register set k; an invalid u re-reads the start of
     // the run's first tile and lanes past the unit's end re-read lane 0's
     // entries (every address stays inside a unit, so the wait counts are
     // exact).
-    auto prefetch = [&](int k, const Unit &u) {
+    // x only for a unit that is staged: the run's first (need_x) or one that
+    // starts a tile (a tile spans about three units on the 3D scanner matrix,
+    // and loading x for every unit was a third of the workgroup's loads).
+    auto prefetch = [&](int k, const Unit &u, bool need_x) {
         const uint4 t = uniform_tile(tl, u.valid ? u.ti : 0u);
         const uint64_t off = ((uint64_t)t.w << 32) | t.z;
         const uint32_t c = u.valid ? u.c : 0u;
@@ -331,9 +334,11 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
         }
         pg[k] = ldnt(reinterpret_cast<const u32x4 *>(base + VB * n8) + le / 8);
         // x is padded to whole segments (xlen >= nseg * W): no clamping.
-        const float *xb = x + (uint64_t)t.x * W * BC;
+        if (need_x || (u.valid && u.c == 0)) {
+            const float *xb = x + (uint64_t)t.x * W * BC;
 #pragma unroll
-        for (int q = 0; q < XF / 4; ++q) px[k][q] = *(reinterpret_cast<const f32x4 *>(xb) + tid + kWG * q);
+            for (int q = 0; q < XF / 4; ++q) px[k][q] = *(reinterpret_cast<const f32x4 *>(xb) + tid + kWG * q);
+        }
     };
 
     Unit su[D];  // the unit whose loads are in register set k
@@ -342,7 +347,7 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
     for (int k = 0; k < D; ++k) {
         if (k) iss = advance(iss);
         su[k] = iss;
-        prefetch(k, iss);
+        prefetch(k, iss, k == 0);
     }
     bool fresh = true;  // first unit of the run: its x segment is not staged yet
     // Every round runs all D stages, so the loop has one exit and the loads'
@@ -367,7 +372,7 @@ __global__ __launch_bounds__(kWG, (min_waves<W, VT, BC>())) void blocked_spmv_ke
             auto issue_next = [&]() {
                 iss = advance(iss);
                 su[k] = iss;
-                prefetch(k, iss);
+                prefetch(k, iss, false);
             };
             if constexpr (D == 1) issue_next();
 
