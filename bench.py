@@ -704,6 +704,10 @@ def main():
                **host}
         log(f"CPU baseline {cpu['value']} it/s on {ct} threads ({host['physical_cores']} physical cores of "
             f"{host['nproc']} CPUs; runs {run_s} s)")
+        if args.workload == "c1":  # the small matrix on one core as well (same protocol, a shorter budget)
+            rate1, k1, run_s1, _ = cpu_baseline_f32(O, A, AT, g, s32, f0, 1, budget_s=9.0)
+            cpu["one_core"] = {"value": round(rate1, 4), "unit": "iters/s", "iterations": k1, "runs_s": run_s1}
+            log(f"CPU baseline on one core: {cpu['one_core']['value']} it/s (runs {run_s1} s)")
         if args.values == "f32":
             o_w = M.opts(spmv=0 if fp_merge else 1, using_trans=0, secsize_fp=args.secsize, secsize_bp=args.secsize,
                          use_graph=args.graph, tile_width=args.tile, tile_rows=args.tile_rows,
