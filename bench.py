@@ -441,7 +441,8 @@ def run_serve(args, M, threads):
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=100, help="timed MLEM iterations")
+    ap.add_argument("--steps", type=int, default=None,
+                    help="timed MLEM iterations (default: 100; 2000 on the 2D ring, whose iterations take about 25 us)")
     ap.add_argument("--warmup", type=int, default=5)
     ap.add_argument("--workload", default="c2", choices=["c2", "mid", "c1", "c3"],
                     help="c3: the 100 GB-class matrix, resident as the tiles of A and A^T only (no CPU baseline: "
@@ -460,7 +461,7 @@ def main():
                     help="work-items per workgroup of the kernel for 4 or 8 per pass: 128, 256, 512 (0: default)")
     ap.add_argument("--graph", type=int, default=None,
                     help="replay captured hipGraphs (default: 1 for batches and serving, whose workloads ask for "
-                         "graph-captured iterations, else 0)")
+                         "graph-captured iterations, and for the launch-bound 2D ring, else 0)")
     ap.add_argument("--autotune", type=int, nargs="?", const=1, default=None,
                     help="time the candidate tile shapes and workgroup counts first (mlem_autotune) and use the best "
                          "(default: 1 for batches, whose workload asks for autotuned shapes, else 0)")
@@ -495,7 +496,9 @@ def main():
     if args.values is None:
         args.values = "uf16" if (args.batch or args.serve) else "f32"
     if args.graph is None:
-        args.graph = 1 if (args.serve or args.batch) else 0
+        args.graph = 1 if (args.serve or args.batch or args.workload == "c1") else 0
+    if args.steps is None:
+        args.steps = 2000 if args.workload == "c1" and not (args.serve or args.batch) else 100
     if args.autotune is None:
         args.autotune = 1 if args.batch else 0
     if args.bp is None:
