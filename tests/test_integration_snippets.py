@@ -1,6 +1,7 @@
-"""The code INTEGRATION.md gives a maintainer compiles against the headers it
-names: every C++ block is compiled (g++ -fsyntax-only) inside a function whose
-parameters are the variables the block uses, with the types the reference
+"""The code the integration guide at the repository root (DOC) gives a
+maintainer compiles against the headers it names: every C++ block is
+compiled (g++ -fsyntax-only) inside a function whose parameters are the
+variables the block uses, with the types the reference
 gives them (main.cu:65-68, 167-172: unsigned long row pointers, unsigned int
 columns, int counts, float values), and the ctypes stub is valid Python with
 a field list equal to mlem_opts in include/mlem_hip.h.  A changed signature
@@ -48,7 +49,7 @@ def blocks(lang):
 @pytest.mark.skipif(shutil.which("g++") is None, reason="no g++")
 def test_cpp_blocks_compile_against_the_headers(tmp_path):
     cpp = blocks("cpp")
-    assert len(cpp) == len(HARNESS), "a C++ block was added to or removed from INTEGRATION.md: update HARNESS"
+    assert len(cpp) == len(HARNESS), "a C++ block was added to or removed from the integration guide: update HARNESS"
     src = ['#include <cstdint>', '#include <cstdio>', '#include <vector>', '#include "mlem_hip.h"',
            '#include "mlem_host.h"']
     for k, (body, params) in enumerate(zip(cpp, HARNESS)):
