@@ -48,6 +48,33 @@ __device__ __forceinline__ void emit(uint32_t r, float y, const EpiArgs &ea) {
     }
 }
 
+// The epilogue's inputs of row r (RATIO: the count; UPDATE: the image value
+// and the sensitivity), read before the row sum so that their memory round
+// trip overlaps the SpMV's instead of following it; emit_with then forms
+// exactly what emit does.
+struct EpiIn {
+    float a, b;
+};
+
+template <int EPI>
+__device__ __forceinline__ EpiIn epi_load(uint32_t r, const EpiArgs &ea) {
+    if constexpr (EPI == EPI_RATIO) return {(float)ea.g[r], 0.0f};
+    else if constexpr (EPI == EPI_UPDATE) return {ea.f[r], ea.norm[r]};
+    else return {0.0f, 0.0f};
+}
+
+template <int EPI>
+__device__ __forceinline__ void emit_with(uint32_t r, float y, const EpiIn &in, const EpiArgs &ea) {
+    if constexpr (EPI == EPI_PLAIN) {
+        ea.out[r] = y;
+    } else if constexpr (EPI == EPI_RATIO) {
+        ea.out[r] = (y != 0.0f) ? in.a / y : y;
+    } else {
+        const float v = in.a * y;
+        ea.f[r] = (in.b == 0.0f) ? v : v / in.b;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // CSR-vector
 // ---------------------------------------------------------------------------
@@ -114,8 +141,9 @@ __global__ __launch_bounds__(kWG) void csr_vector_kernel(const uint64_t *__restr
     const uint32_t row = blockIdx.x * (uint32_t)(kWG / G) + threadIdx.x / G;
     const uint32_t lane = threadIdx.x % G;
     if (row >= rows) return;  // whole groups leave together
+    const EpiIn in = epi_load<EPI>(row, ea);  // every work-item of the group: one address, one request
     const float acc = group_sum<G>(row_partial<G>(col, val, x, rowptr[row], rowptr[row + 1], lane));
-    if (lane == 0) emit<EPI>(row, acc, ea);
+    if (lane == 0) emit_with<EPI>(row, acc, in, ea);
 }
 
 // ---------------------------------------------------------------------------

@@ -753,26 +753,55 @@ __global__ __launch_bounds__(NT) void stream_spmv_kernel(uint32_t R, uint32_t ns
 
 // The epilogue of row r from its sum y over the block's workgroups: the
 // row's value scale (16-bit values), then the ratio, the update or the plain
-// sum.  Vectors are [rows][BC].
+// sum.  Vectors are [rows][BC].  Its inputs are read by reduce_load before
+// the partials are summed, so that their memory round trip overlaps the
+// sum's instead of following it (a small matrix's reduction is a chain of
+// dependent round trips).
+template <int BC>
+struct ReduceIn {
+    float sc;     // the row's value scale (1 without scales)
+    float a[BC];  // RATIO: the counts; UPDATE: the image values
+    float n;      // UPDATE: the sensitivity
+    bool act[BC]; // UPDATE: the right-hand side is active
+};
+
 template <int EPI, int BC>
-__device__ __forceinline__ void reduce_epilogue(uint32_t r, const float (&y)[BC], const float *__restrict__ scale,
-                                                const EpiArgs &ea) {
-    const float sc = scale ? scale[r] : 1.0f;
+__device__ __forceinline__ ReduceIn<BC> reduce_load(uint32_t r, const float *__restrict__ scale, const EpiArgs &ea) {
+    ReduceIn<BC> in;
+    in.sc = scale ? scale[r] : 1.0f;
+    in.n = 0.0f;
 #pragma unroll
     for (int q = 0; q < BC; ++q) {
         const uint64_t k = (uint64_t)r * BC + q;
-        const float yq = scale ? y[q] * sc : y[q];
+        in.a[q] = 0.0f;
+        in.act[q] = true;
+        if constexpr (EPI == EPI_RATIO) in.a[q] = (float)ea.g[k];
+        if constexpr (EPI == EPI_UPDATE) {
+            in.a[q] = ea.f[k];
+            in.act[q] = !ea.active || ea.active[q];
+        }
+    }
+    if constexpr (EPI == EPI_UPDATE) in.n = ea.norm[r];
+    return in;
+}
+
+template <int EPI, int BC>
+__device__ __forceinline__ void reduce_epilogue(uint32_t r, const float (&y)[BC], const ReduceIn<BC> &in,
+                                                bool scaled, const EpiArgs &ea) {
+#pragma unroll
+    for (int q = 0; q < BC; ++q) {
+        const uint64_t k = (uint64_t)r * BC + q;
+        const float yq = scaled ? y[q] * in.sc : y[q];
         if constexpr (EPI == EPI_PLAIN) {
             ea.out[k] = yq;
         } else if constexpr (EPI == EPI_RATIO) {
             // 16-bit values: the back projection's tiles hold the same codes
             // without the row scales, so the ratio g / (scale * sum) goes on
             // multiplied by the scale, that is g / sum.
-            ea.out[k] = (y[q] != 0.0f) ? (float)ea.g[k] / y[q] : 0.0f;
+            ea.out[k] = (y[q] != 0.0f) ? in.a[q] / y[q] : 0.0f;
         } else {
-            const float v = ea.f[k] * yq;
-            const float n = ea.norm[r];
-            if (!ea.active || ea.active[q]) ea.f[k] = (n == 0.0f) ? v : v / n;
+            const float v = in.a[q] * yq;
+            if (in.act[q]) ea.f[k] = (in.n == 0.0f) ? v : v / in.n;
         }
     }
 }
@@ -815,11 +844,12 @@ __global__ void blocked_reduce_kernel(const float *__restrict__ part, const uint
     const uint32_t r = blockIdx.x * blockDim.x + threadIdx.x;
     if (r >= rows) return;
     const uint32_t b = r >> rshift, i = r & ((1u << rshift) - 1u);
+    const ReduceIn<BC> in = reduce_load<EPI, BC>(r, scale, ea);
     float y[BC];
 #pragma unroll
     for (int q = 0; q < BC; ++q) y[q] = 0.0f;
     sum_partials<BC, 1>(y, part, wl_first[b], wl_first[b + 1], rshift, i);
-    reduce_epilogue<EPI, BC>(r, y, scale, ea);
+    reduce_epilogue<EPI, BC>(r, y, in, scale != nullptr, ea);
 }
 
 // The same for blocks with many workgroups (a small matrix spread over the
@@ -838,7 +868,9 @@ __global__ __launch_bounds__(64 * kSlices) void blocked_reduce_sliced_kernel(
     float y[BC];
 #pragma unroll
     for (int q = 0; q < BC; ++q) y[q] = 0.0f;
+    ReduceIn<BC> in{};
     if (r < rows) {
+        if (j == 0) in = reduce_load<EPI, BC>(r, scale, ea);  // wave 0 applies the epilogue
         const uint32_t b = r >> rshift, i = r & ((1u << rshift) - 1u);
         sum_partials<BC, kSlices>(y, part, wl_first[b] + j, wl_first[b + 1], rshift, i);
     }
@@ -850,7 +882,7 @@ __global__ __launch_bounds__(64 * kSlices) void blocked_reduce_sliced_kernel(
     for (uint32_t jj = 1; jj < kSlices; ++jj)
 #pragma unroll
         for (int q = 0; q < BC; ++q) y[q] += s_y[jj][x][q];
-    reduce_epilogue<EPI, BC>(r, y, scale, ea);
+    reduce_epilogue<EPI, BC>(r, y, in, scale != nullptr, ea);
 }
 
 // ----------------------------------------------------------------- builder
