@@ -438,7 +438,10 @@ def run_serve(args, M, threads):
     print(json.dumps(out), flush=True)
 
 
-def main():
+def parse_args(argv=None):
+    """The command line, with the defaults that depend on the workload resolved: 16-bit (UF16) values, graphs and
+    the autotuner for batches and serving; graphs, 2000 timed iterations and CSR-vector BP (the workload's own
+    pairing with merge-based FP) for the launch-bound 2D ring."""
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=None,
@@ -492,7 +495,7 @@ def main():
                     help="also write the matrix as a csr4 file and time the streamed loader on it")
     ap.add_argument("--calibrate-copy", action="store_true",
                     help="also run a 1 GiB device-to-device copy (HBM counter calibration)")
-    args = ap.parse_args()
+    args = ap.parse_args(argv)
     if args.values is None:
         args.values = "uf16" if (args.batch or args.serve) else "f32"
     if args.graph is None:
@@ -503,7 +506,11 @@ def main():
         args.autotune = 1 if args.batch else 0
     if args.bp is None:
         args.bp = "vector" if args.workload == "c1" and not (args.batch or args.serve) else "merge"
+    return args
 
+
+def main():
+    args = parse_args()
     world = int(os.environ.get("WORLD_SIZE", "1"))
     if args.gpus > 1 or world > 1:
         print(f"bench.py: this engine keeps the whole system matrix resident on ONE GPU; "
