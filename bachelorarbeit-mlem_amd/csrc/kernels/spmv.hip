@@ -33,25 +33,12 @@ __device__ __forceinline__ T ldnt(const T *p) {
     return __builtin_nontemporal_load(p);
 }
 
-template <int EPI>
-__device__ __forceinline__ void emit(uint32_t r, float y, const EpiArgs &ea) {
-    if constexpr (EPI == EPI_PLAIN) {
-        ea.out[r] = y;
-    } else if constexpr (EPI == EPI_RATIO) {
-        // calcCorrel (kernel.cu:16-18): g / fp where fp != 0, else fp stays.
-        ea.out[r] = (y != 0.0f) ? (float)ea.g[r] / y : y;
-    } else {
-        // calcUpdate (kernel.cu:33-37): (f * bp) / norm, or f * bp if norm == 0.
-        const float v = ea.f[r] * y;
-        const float n = ea.norm[r];
-        ea.f[r] = (n == 0.0f) ? v : v / n;
-    }
-}
-
-// The epilogue's inputs of row r (RATIO: the count; UPDATE: the image value
-// and the sensitivity), read before the row sum so that their memory round
-// trip overlaps the SpMV's instead of following it; emit_with then forms
-// exactly what emit does.
+// The epilogue of row r with row sum y: calcCorrel (kernel.cu:16-18), g / y
+// where y != 0, else y stays; calcUpdate (kernel.cu:33-37), (f * y) / norm,
+// or f * y where norm == 0; or the plain sum.  epi_load reads the row's
+// inputs (RATIO: the count; UPDATE: the image value and the sensitivity) and
+// emit_with forms the result, so that a kernel can issue those reads before
+// its row sum and their memory round trip overlaps the sum's.
 struct EpiIn {
     float a, b;
 };
@@ -73,6 +60,11 @@ __device__ __forceinline__ void emit_with(uint32_t r, float y, const EpiIn &in, 
         const float v = in.a * y;
         ea.f[r] = (in.b == 0.0f) ? v : v / in.b;
     }
+}
+
+template <int EPI>
+__device__ __forceinline__ void emit(uint32_t r, float y, const EpiArgs &ea) {
+    emit_with<EPI>(r, y, epi_load<EPI>(r, ea), ea);
 }
 
 // ---------------------------------------------------------------------------
